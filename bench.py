@@ -1,0 +1,178 @@
+"""Benchmark: 3D hex8 linear elasticity (96^3 elements, 2,738,019 DOF) on one
+MI355X.
+
+One step = one full solve of the resident system to a true relative residual
+of 1e-8 with the chosen form (FP32-inner Jacobi-PCG on FP32 3x3-block values
+with FP64 iterative refinement).  value = DOF * PCG iterations / second over
+the timed steps.  The plain FP64 CSR PCG is measured beside it, the FP64
+assembly is timed separately, the dominant kernel (the SpMV of the inner
+solve) is timed with HIP events for the roofline, and the oracle (serial CPU
+restatement of the reference path) is timed on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "hex8 3D solve: DOF·iter/s & SpMV HBM GB/s; served solves/s + p99 at 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sample_n, budget_s):
+    """Oracle (serial restatement of SerialAssembler + sparse.cpp PCG) on an
+    n^3 version of the same workload: times the PCG iterations it completes
+    within the budget."""
+    from betanoox_amd import meshgen
+    from oracle import oracle as O
+    prob = meshgen.elastic_cube(sample_n)
+    t0 = time.perf_counter()
+    eq = O.assemble_problem(prob)
+    t_asm = time.perf_counter() - t0
+    b = eq.rhs()
+    # calibrate iterations to the budget with a short run
+    t0 = time.perf_counter()
+    eq.pcg(b, eps=1e-30, maxit=5, stop_rule=1)
+    per_it = (time.perf_counter() - t0) / 5
+    iters = max(10, int(budget_s / max(per_it, 1e-6)))
+    t0 = time.perf_counter()
+    st, x, it, nrm = eq.pcg(b, eps=1e-30, maxit=iters, stop_rule=1)
+    t_pcg = time.perf_counter() - t0
+    return {
+        "value": prob.n_dof * it / t_pcg,
+        "unit": "DOF*iter/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "oracle serial Jacobi-PCG (sparse.cpp:114-242 order) on the same generator at %d^3 "
+                  "(%d DOF), %d iterations in %.1f s; oracle assembly %.2f s (%.3g elements/s)"
+                  % (sample_n, prob.n_dof, it, t_pcg, t_asm, prob.n_el / t_asm),
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=96)
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--cpu-sample", type=int, default=40)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md, 'Sharding'); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from betanoox_amd import capi, meshgen, setup_problem
+
+    t0 = time.perf_counter()
+    prob = meshgen.elastic_cube(args.n)
+    log("generated %s: %d nodes, %d elements, %d DOF in %.1f s"
+        % (prob.name, prob.n_nodes, prob.n_el, prob.n_dof, time.perf_counter() - t0))
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+    setup_problem(ctx, prob)
+    t_pattern = time.perf_counter() - t0
+    nl, nf = ctx.nnz()
+    log("pattern: nnz lower %d, full %d (%.2f s incl. upload)" % (nl, nf, t_pattern))
+    # assembly (FP64 integration + scatter), best of 3
+    t_asm = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.assemble(capi.ASM_STIFFNESS)
+        t_asm.append(time.perf_counter() - t0)
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    log("assembly best %.2f ms" % (1e3 * min(t_asm)))
+
+    result_fp64 = None
+    if not args.no_fp64:
+        s = ctx.solve(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64, maxit=20000)
+        ms64, by64 = ctx.bench_spmv(fp32=False, reps=20)
+        result_fp64 = {
+            "iters": s["iters"], "ms": s["ms_total"], "rel_resid": s["rel_resid"],
+            "dof_iter_per_s": prob.n_dof * s["iters"] / (s["ms_total"] / 1e3),
+            "spmv_ms": ms64, "spmv_GBps": by64 / ms64 / 1e6,
+        }
+        log("fp64 PCG: %s" % json.dumps(result_fp64))
+
+    opts = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
+                             inner_tol=args.inner_tol, maxit=40000)
+    for w in range(args.warmup):
+        s = ctx.solve(opts)
+        log("warmup %d: %s" % (w, json.dumps(s)))
+    iters = 0
+    outer = 0
+    resid = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s = ctx.solve(opts)
+        iters += s["iters"]
+        outer += s["outer_iters"]
+        resid = max(resid, s["rel_resid"])
+        log("step %d: %s" % (k, json.dumps(s)))
+    dt = time.perf_counter() - t0
+    value = prob.n_dof * iters / dt
+
+    ms32, by32 = ctx.bench_spmv(fp32=True, reps=50)
+    achieved = by32 / ms32 / 1e6
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_bsr3_spmv<16,1> (FP32 3x3-block SpMV fused with p.Ap)",
+                "bytes_per_launch": by32, "ms_per_launch": ms32}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF*iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 inner / fp64 refinement",
+        "data": "synthetic (seeded generator, betanoox_amd/meshgen.py)",
+        "config": {
+            "workload": "3D linear elasticity, hex8 %d^3 (%d DOF), 2x2x2 Gauss, E log-uniform [1,100], "
+                        "nu U[0.2,0.35], jitter 0.1h, z=0 clamped, unit pressure z=1; solve to 1e-8 "
+                        "true relative residual" % (args.n, prob.n_dof),
+            "n_dof": prob.n_dof, "n_el": prob.n_el, "nnz_full": nf, "nnz_lower": nl,
+            "iters_per_solve": iters / args.steps, "outer_per_solve": outer / args.steps,
+            "rel_resid_max": resid, "inner_tol": args.inner_tol,
+        },
+        "roofline": roofline,
+        "plain_fp64": result_fp64,
+        "assembly": {"ms": 1e3 * min(t_asm), "elements_per_s": prob.n_el / min(t_asm)},
+        "spmv_fp32_GBps": achieved,
+    }
+    if result_fp64:
+        line["speedup_vs_fp64_dof_iter"] = value / result_fp64["dof_iter_per_s"]
+        line["speedup_vs_fp64_time_to_solution"] = result_fp64["ms"] / (1e3 * dt / args.steps)
+    ctx.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_budget)
+        line["cpu_baseline"]["cores"] = 1
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,314 @@
+"""ctypes binding of libbeta_hip.so (include/beta_hip.h).
+
+The product path: every call goes to the HIP library.  If the library is
+missing or no GPU is visible the constructor raises; there is no CPU
+fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libbeta_hip.so")
+
+# status codes (beta_hip.h)
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_OOM = -3
+ERR_NOT_CONVERGED = -4
+ERR_ZERO_DIAG = -5
+ERR_NEG_DIAG = -6
+ERR_NEG_DETJ = -7
+ERR_STATE = -8
+ERR_UNSUPPORTED = -9
+ERR_CAPACITY = -10
+
+MAT_ISO_ELASTIC = 1
+MAT_CMAT3D = 2
+MAT_PLANE_STRESS = 3
+MAT_HEAT = 4
+MAT_J2 = 5
+
+ASM_STIFFNESS = 1
+ASM_HEAT_TRANSIENT = 2
+ASM_PLASTIC_TANGENT = 3
+
+STOP_REF_R_OVER_X = 0
+STOP_REL_RESID = 1
+PREC_FP64 = 0
+PREC_FP32_IR = 1
+
+# every symbol the header declares
+EXPORTS = [
+    "bhip_create", "bhip_destroy", "bhip_last_error", "bhip_version", "bhip_set_mesh",
+    "bhip_num_dof", "bhip_get_dof_first", "bhip_set_restraints", "bhip_set_prescribed",
+    "bhip_build_pattern", "bhip_pattern_nnz", "bhip_get_pattern_lower", "bhip_get_values_lower",
+    "bhip_get_csr", "bhip_set_materials", "bhip_set_integration", "bhip_assemble",
+    "bhip_get_pseudo_load", "bhip_set_load", "bhip_build_rhs", "bhip_get_rhs", "bhip_set_rhs",
+    "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv",
+    "bhip_bench_spmv",
+]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxit", C.c_int), ("stop_rule", C.c_int),
+                ("precision", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
+                ("check_every", C.c_int), ("use_x0", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("outer_iters", C.c_int), ("ref_norm", C.c_double),
+                ("rel_resid", C.c_double), ("ms_total", C.c_double), ("ms_spmv_avg", C.c_double),
+                ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BhipError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("status %d: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+
+_P_D = C.POINTER(C.c_double)
+_P_I = C.POINTER(C.c_int)
+_P_U8 = C.POINTER(C.c_ubyte)
+_P_LL = C.POINTER(C.c_longlong)
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library and declare its signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError("libbeta_hip.so not built at %s (run __graft_entry__.build())" % path)
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "bhip_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "bhip_destroy": (None, [vp]),
+        "bhip_last_error": (C.c_char_p, [vp]),
+        "bhip_version": (C.c_char_p, []),
+        "bhip_set_mesh": (C.c_int, [vp, C.c_int, C.c_int, _P_D, C.c_int, C.c_int, _P_I, _P_I, C.c_int]),
+        "bhip_num_dof": (C.c_int, [vp]),
+        "bhip_get_dof_first": (C.c_int, [vp, _P_I]),
+        "bhip_set_restraints": (C.c_int, [vp, _P_U8, _P_D]),
+        "bhip_set_prescribed": (C.c_int, [vp, _P_D]),
+        "bhip_build_pattern": (C.c_int, [vp]),
+        "bhip_pattern_nnz": (C.c_int, [vp, _P_LL, _P_LL]),
+        "bhip_get_pattern_lower": (C.c_int, [vp, _P_I, _P_I]),
+        "bhip_get_values_lower": (C.c_int, [vp, _P_D]),
+        "bhip_get_csr": (C.c_int, [vp, _P_I, _P_I, _P_D]),
+        "bhip_set_materials": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_I]),
+        "bhip_set_integration": (C.c_int, [vp, C.c_int, C.c_double]),
+        "bhip_assemble": (C.c_int, [vp, C.c_int, _P_D, _P_D]),
+        "bhip_get_pseudo_load": (C.c_int, [vp, _P_D]),
+        "bhip_set_load": (C.c_int, [vp, _P_D]),
+        "bhip_build_rhs": (C.c_int, [vp]),
+        "bhip_get_rhs": (C.c_int, [vp, _P_D]),
+        "bhip_set_rhs": (C.c_int, [vp, _P_D]),
+        "bhip_set_x0": (C.c_int, [vp, _P_D]),
+        "bhip_solve": (C.c_int, [vp, C.POINTER(SolveOpts), C.POINTER(Stats)]),
+        "bhip_get_solution": (C.c_int, [vp, _P_D]),
+        "bhip_pcg_solve": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
+        "bhip_spmv": (C.c_int, [vp, _P_D, _P_D]),
+        "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_P_D)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_P_I)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """One device context: mesh, pattern, matrix, solver state on one GPU."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.bhip_create(device, C.byref(h))
+        if st != OK:
+            raise BhipError(st, "bhip_create failed (is a GPU visible?)")
+        self.h = h
+        self.n_dof = 0
+
+    def close(self):
+        if self.h:
+            self.lib.bhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, st, allow=()):
+        if st != OK and st not in allow:
+            raise BhipError(st, self.lib.bhip_last_error(self.h).decode())
+        return st
+
+    # ---- mesh / restraints
+    def set_mesh(self, dim, xyz, conn, ndof_per_node=None, ndof_uniform=None):
+        xyz = _f64(xyz)
+        conn = _i32(conn)
+        nn = xyz.shape[0]
+        ne, nnpe = conn.shape
+        nd = None if ndof_per_node is None else _i32(ndof_per_node)
+        if ndof_uniform is None:
+            ndof_uniform = dim
+        self._keep = (xyz, conn, nd)
+        self._chk(self.lib.bhip_set_mesh(self.h, dim, nn, _d(xyz), ne, nnpe, _i(conn), _i(nd), ndof_uniform))
+        self.n_dof = self.lib.bhip_num_dof(self.h)
+        return self.n_dof
+
+    def dof_first(self):
+        out = np.zeros(len(self._keep[0]), dtype=np.int32)
+        self._chk(self.lib.bhip_get_dof_first(self.h, _i(out)))
+        return out
+
+    def set_restraints(self, restrained, values=None):
+        r = np.ascontiguousarray(restrained, dtype=np.uint8)
+        v = None if values is None else _f64(values)
+        self._chk(self.lib.bhip_set_restraints(self.h, r.ctypes.data_as(_P_U8), _d(v)))
+
+    def set_prescribed(self, values):
+        self._chk(self.lib.bhip_set_prescribed(self.h, _d(_f64(values))))
+
+    # ---- pattern
+    def build_pattern(self):
+        self._chk(self.lib.bhip_build_pattern(self.h))
+
+    def nnz(self):
+        lo, full = C.c_longlong(), C.c_longlong()
+        self._chk(self.lib.bhip_pattern_nnz(self.h, C.byref(lo), C.byref(full)))
+        return lo.value, full.value
+
+    def pattern_lower(self):
+        nl, _ = self.nnz()
+        rp = np.zeros(self.n_dof + 1, dtype=np.int32)
+        ci = np.zeros(max(nl, 1), dtype=np.int32)
+        self._chk(self.lib.bhip_get_pattern_lower(self.h, _i(rp), _i(ci)))
+        return rp, ci[:nl]
+
+    def values_lower(self):
+        nl, _ = self.nnz()
+        v = np.zeros(max(nl, 1))
+        self._chk(self.lib.bhip_get_values_lower(self.h, _d(v)))
+        return v[:nl]
+
+    def csr(self):
+        _, nf = self.nnz()
+        rp = np.zeros(self.n_dof + 1, dtype=np.int32)
+        ci = np.zeros(max(nf, 1), dtype=np.int32)
+        v = np.zeros(max(nf, 1))
+        self._chk(self.lib.bhip_get_csr(self.h, _i(rp), _i(ci), _d(v)))
+        return rp, ci[:nf], v[:nf]
+
+    # ---- materials / assembly
+    def set_materials(self, kind, params, elem_group=None):
+        p = _f64(params)
+        ng = p.shape[0] if p.ndim == 2 else 1
+        g = None if elem_group is None else _i32(elem_group)
+        self._chk(self.lib.bhip_set_materials(self.h, kind, ng, _d(p), _i(g)))
+
+    def set_integration(self, order, factor=1.0):
+        self._chk(self.lib.bhip_set_integration(self.h, order, factor))
+
+    def assemble(self, what=ASM_STIFFNESS, coef=None, u_state=None):
+        c = None if coef is None else _f64(coef)
+        u = None if u_state is None else _f64(u_state)
+        self._chk(self.lib.bhip_assemble(self.h, what, _d(c), _d(u)))
+
+    def pseudo_load(self):
+        f = np.zeros(self.n_dof)
+        self._chk(self.lib.bhip_get_pseudo_load(self.h, _d(f)))
+        return f
+
+    def set_load(self, load):
+        self._chk(self.lib.bhip_set_load(self.h, _d(None if load is None else _f64(load))))
+
+    def build_rhs(self):
+        self._chk(self.lib.bhip_build_rhs(self.h))
+
+    def rhs(self):
+        b = np.zeros(self.n_dof)
+        self._chk(self.lib.bhip_get_rhs(self.h, _d(b)))
+        return b
+
+    def set_rhs(self, b):
+        self._chk(self.lib.bhip_set_rhs(self.h, _d(_f64(b))))
+
+    def set_x0(self, x0):
+        self._chk(self.lib.bhip_set_x0(self.h, _d(None if x0 is None else _f64(x0))))
+
+    # ---- solve
+    @staticmethod
+    def opts(tol=1e-8, maxit=10000, stop_rule=STOP_REL_RESID, precision=PREC_FP64, inner_tol=1e-3,
+             max_outer=50, check_every=32, use_x0=0):
+        return SolveOpts(tol, maxit, stop_rule, precision, inner_tol, max_outer, check_every, use_x0)
+
+    def solve(self, opts=None, allow_not_converged=False, **kw):
+        o = opts if opts is not None else self.opts(**kw)
+        s = Stats()
+        allow = (ERR_NOT_CONVERGED,) if allow_not_converged else ()
+        st = self._chk(self.lib.bhip_solve(self.h, C.byref(o), C.byref(s)), allow)
+        d = s.as_dict()
+        d["status"] = st
+        return d
+
+    def solution(self):
+        x = np.zeros(self.n_dof)
+        self._chk(self.lib.bhip_get_solution(self.h, _d(x)))
+        return x
+
+    def pcg_solve(self, b, x0=None, opts=None, **kw):
+        """LargeMatrix::solve(b, x0) mirror: returns (solution, stats)."""
+        bb = _f64(b).copy()
+        o = opts if opts is not None else self.opts(**kw)
+        s = Stats()
+        st = self._chk(self.lib.bhip_pcg_solve(self.h, _d(bb), _d(None if x0 is None else _f64(x0)),
+                                               C.byref(o), C.byref(s)), (ERR_NOT_CONVERGED,))
+        d = s.as_dict()
+        d["status"] = st
+        return bb, d
+
+    def spmv(self, x):
+        y = np.zeros(self.n_dof)
+        self._chk(self.lib.bhip_spmv(self.h, _d(_f64(x)), _d(y)))
+        return y
+
+    def bench_spmv(self, fp32=False, reps=20):
+        ms, by = C.c_double(), C.c_double()
+        self._chk(self.lib.bhip_bench_spmv(self.h, int(fp32), reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value

@@ -1,0 +1,624 @@
+// assemble.hip -- element integration fused with the global scatter.
+//
+// Reference path: SerialAssembler::assembleK (SerialAssembler.cpp:35-59) calls
+// update("K") per element (ElasticityElement3D.cpp:983-1051: Jacobian,
+// B matrix, upper-triangle Ke += C_x_B . b . detJ.w at every Gauss point,
+// 560-641) and then Equations::addMatrix (equation.cpp:445-562), which adds
+// Ke(I,J) into the lower-triangle store for free I >= J, sets K(I,I) = 1 for
+// a restrained I, and lifts F[I] -= Ke(I,J) u_J for free I, restrained J.
+//
+// MI355X design ("owner computes", no colouring, no atomics): one thread per
+// (node n, neighbour m) entry of the node adjacency computes the ndof x ndof
+// block K(n,m) = sum over the elements holding both nodes, in ascending
+// element order (the reference's serial summation order), of the
+// Gauss-point integral, and writes it once into the full symmetric CSR.
+// The pair (m,n) computes the same canonical block (lower node index first)
+// and writes its transpose, so the stored matrix is exactly symmetric.  The
+// lift and element-force vectors are gathered per node by a second kernel
+// that re-integrates only the blocks it needs.
+#include "bhip_internal.hpp"
+
+// Gauss tables, filled by the host from the reference's Gauss-Legendre
+// construction (gauss.cpp:14-60, 303-365, 825-877).
+__constant__ double c_w[kMaxIp];
+__constant__ double c_dN[kMaxIp * 24];  // hex8: [ip][xi|eta|zeta][8]; quad4: [ip][xi|eta][4]
+__constant__ double c_N[kMaxIp * 8];    // shape values [ip][node]
+
+namespace {
+
+// ------------------------------------------------------------ geometry
+// IsoElement::jacobian, 3-direction branch (isoElement.cpp:324-372);
+// returns detJ and the global gradients of nodes a and b.
+__device__ __forceinline__ double hex8_grads(const double *x, const double *y, const double *z,
+                                             int ip, int a, int b, double *ga, double *gb) {
+    const double *dxi = c_dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
+    double px1 = 0, px2 = 0, px3 = 0, py1 = 0, py2 = 0, py3 = 0, pz1 = 0, pz2 = 0, pz3 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        px1 += dxi[i] * x[i];
+        px2 += deta[i] * x[i];
+        px3 += dzeta[i] * x[i];
+        py1 += dxi[i] * y[i];
+        py2 += deta[i] * y[i];
+        py3 += dzeta[i] * y[i];
+        pz1 += dxi[i] * z[i];
+        pz2 += deta[i] * z[i];
+        pz3 += dzeta[i] * z[i];
+    }
+    double d1 = py2 * pz3 - py3 * pz2, d2 = px2 * pz3 - px3 * pz2, d3 = px2 * py3 - px3 * py2;
+    double d4 = py1 * pz3 - py3 * pz1, d5 = px1 * pz3 - px3 * pz1, d6 = px1 * py3 - px3 * py1;
+    double d7 = py1 * pz2 - py2 * pz1, d8 = px1 * pz2 - px2 * pz1, d9 = px1 * py2 - px2 * py1;
+    double detJ = px1 * d1 - py1 * d2 + pz1 * d3;
+    double inv = 1. / detJ;
+    double a11 = d1 * inv, a12 = -d4 * inv, a13 = d7 * inv;
+    double a21 = -d2 * inv, a22 = d5 * inv, a23 = -d8 * inv;
+    double a31 = d3 * inv, a32 = -d6 * inv, a33 = d9 * inv;
+    ga[0] = a11 * dxi[a] + a12 * deta[a] + a13 * dzeta[a];
+    ga[1] = a21 * dxi[a] + a22 * deta[a] + a23 * dzeta[a];
+    ga[2] = a31 * dxi[a] + a32 * deta[a] + a33 * dzeta[a];
+    gb[0] = a11 * dxi[b] + a12 * deta[b] + a13 * dzeta[b];
+    gb[1] = a21 * dxi[b] + a22 * deta[b] + a23 * dzeta[b];
+    gb[2] = a31 * dxi[b] + a32 * deta[b] + a33 * dzeta[b];
+    return detJ;
+}
+
+// 2-direction branch (isoElement.cpp:290-322).
+__device__ __forceinline__ double quad4_grads(const double *x, const double *y, int ip, int a,
+                                              int b, double *ga, double *gb) {
+    const double *dxi = c_dN + ip * 24, *deta = dxi + 4;
+    double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        J00 += dxi[i] * x[i];
+        J01 += dxi[i] * y[i];
+        J10 += deta[i] * x[i];
+        J11 += deta[i] * y[i];
+    }
+    double detJ = J00 * J11 - J01 * J10;
+    double t = 1 / detJ;
+    double i00 = J11 * t, i01 = -J01 * t, i10 = -J10 * t, i11 = J00 * t;
+    ga[0] = i00 * dxi[a] + i01 * deta[a];
+    ga[1] = i10 * dxi[a] + i11 * deta[a];
+    gb[0] = i00 * dxi[b] + i01 * deta[b];
+    gb[1] = i10 * dxi[b] + i11 * deta[b];
+    return detJ;
+}
+
+struct AsmArgs {
+    const double *xyz;
+    const int *conn;
+    const double *mat;
+    const int *elem_group;
+    int n_el, n_groups, mat_stride, nnpe, nip;
+    double factor;
+    // plasticity: per (element, ip) tangent descriptor: sp[6], 1/denom, flag
+    const double *tan;
+    int *flags;
+};
+
+__device__ __forceinline__ int group_of(const AsmArgs &A, int e) {
+    if (A.elem_group) return A.elem_group[e];
+    return A.n_groups == A.n_el ? e : 0;
+}
+
+// -------------------------------------------------------------- policies
+// Each policy accumulates the canonical block (lo, hi) of one element into
+// K (row comps of lo, column comps of hi) and, when useful, a second block.
+
+// 3D elasticity, isotropic (E, nu) or full 6x6 C.  Block of B_a^T C B_b with
+// Voigt order xx yy zz xy yz xz and engineering shear
+// (ElasticityElement3D::calculateBmatrix, ElasticityElement3D.cpp:57-87).
+template <int KIND>
+struct ElasHex8 {
+    static constexpr int ND = 3, NN = 8, NB = 9;
+    static __device__ __forceinline__ void accumulate(const AsmArgs &A, int e, const int *ec, int a,
+                                                      int b, bool self, double *K, double *K2) {
+        double x[8], y[8], z[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const double *q = A.xyz + 3 * (long long)ec[i];
+            x[i] = q[0];
+            y[i] = q[1];
+            z[i] = q[2];
+        }
+        const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
+        double lam = 0, mu = 0;
+        double C[36];
+        if (KIND == BHIP_MAT_ISO_ELASTIC || KIND == BHIP_MAT_J2) {
+            double E = mp[0], nu = mp[1];
+            lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+            mu = E / (2 * (1 + nu));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 36; i++) C[i] = mp[i];
+        }
+        int bad = 0;
+        double Ke[9];
+        if (self)
+#pragma unroll
+            for (int i = 0; i < 9; i++) Ke[i] = 0;
+        for (int ip = 0; ip < A.nip; ip++) {
+            double ga[3], gb[3];
+            double detJ = hex8_grads(x, y, z, ip, a, b, ga, gb);
+            if (detJ < 0) bad = 1;
+            double f = detJ * c_w[ip] * A.factor;
+            double blk[9];
+            if (KIND == BHIP_MAT_ISO_ELASTIC || KIND == BHIP_MAT_J2) {
+                double gg = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++)
+                        blk[c * 3 + d] = lam * ga[c] * gb[d] + mu * gb[c] * ga[d] + (c == d ? gg : 0.0);
+                if (KIND == BHIP_MAT_J2) {
+                    const double *t = A.tan + ((long long)e * A.nip + ip) * 8;
+                    if (t[7] != 0.0) {
+                        double sa[3] = {ga[0] * t[0] + ga[1] * t[3] + ga[2] * t[5],
+                                        ga[1] * t[1] + ga[0] * t[3] + ga[2] * t[4],
+                                        ga[2] * t[2] + ga[1] * t[4] + ga[0] * t[5]};
+                        double sb[3] = {gb[0] * t[0] + gb[1] * t[3] + gb[2] * t[5],
+                                        gb[1] * t[1] + gb[0] * t[3] + gb[2] * t[4],
+                                        gb[2] * t[2] + gb[1] * t[4] + gb[0] * t[5]};
+#pragma unroll
+                        for (int c = 0; c < 3; c++)
+#pragma unroll
+                            for (int d = 0; d < 3; d++) blk[c * 3 + d] -= sa[c] * sb[d] * t[6];
+                    }
+                }
+            } else {
+                // CB = C * B_b (6x3), then B_a^T * CB
+                double CB[6][3];
+#pragma unroll
+                for (int s = 0; s < 6; s++) {
+                    CB[s][0] = gb[0] * C[s * 6 + 0] + gb[1] * C[s * 6 + 3] + gb[2] * C[s * 6 + 5];
+                    CB[s][1] = gb[1] * C[s * 6 + 1] + gb[0] * C[s * 6 + 3] + gb[2] * C[s * 6 + 4];
+                    CB[s][2] = gb[2] * C[s * 6 + 2] + gb[1] * C[s * 6 + 4] + gb[0] * C[s * 6 + 5];
+                }
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    blk[0 * 3 + d] = ga[0] * CB[0][d] + ga[1] * CB[3][d] + ga[2] * CB[5][d];
+                    blk[1 * 3 + d] = ga[1] * CB[1][d] + ga[0] * CB[3][d] + ga[2] * CB[4][d];
+                    blk[2 * 3 + d] = ga[2] * CB[2][d] + ga[1] * CB[4][d] + ga[0] * CB[5][d];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 9; i++) K[i] += blk[i] * f;
+            if (self)
+#pragma unroll
+                for (int i = 0; i < 9; i++) Ke[i] += blk[i] * f;
+        }
+        if (bad && A.flags) atomicOr(&A.flags[0], 1);
+        // BasicElement::checkForNegativeDiagonal (BasicElement.cpp:116-130)
+        if (self && KIND != BHIP_MAT_J2 && A.flags)
+            if (Ke[0] <= 0 || Ke[4] <= 0 || Ke[8] <= 0) atomicOr(&A.flags[0], 2);
+    }
+};
+
+// 2D plane stress quad4 (ElasticityElement2D.cpp:22-119, pStress.cpp:68-85).
+struct ElasQuad4 {
+    static constexpr int ND = 2, NN = 4, NB = 4;
+    static __device__ __forceinline__ void accumulate(const AsmArgs &A, int e, const int *ec, int a,
+                                                      int b, bool self, double *K, double *K2) {
+        double x[4], y[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            x[i] = A.xyz[2 * (long long)ec[i]];
+            y[i] = A.xyz[2 * (long long)ec[i] + 1];
+        }
+        const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
+        double s11 = 1 / mp[0], s12 = -mp[2] / mp[0], s22 = 1. / mp[1], s66 = 1. / mp[3];
+        double den = s11 * s22 - s12 * s12;
+        double C[9] = {s22 / den, -s12 / den, 0., -s12 / den, s11 / den, 0., 0., 0., 1. / s66};
+        int bad = 0;
+        double Ke[4] = {0, 0, 0, 0};
+        for (int ip = 0; ip < A.nip; ip++) {
+            double ga[2], gb[2];
+            double detJ = quad4_grads(x, y, ip, a, b, ga, gb);
+            if (detJ < 0) bad = 1;
+            double f = detJ * c_w[ip] * A.factor;
+            double CB[3][2];
+#pragma unroll
+            for (int s = 0; s < 3; s++) {
+                CB[s][0] = gb[0] * C[s * 3 + 0] + gb[1] * C[s * 3 + 2];
+                CB[s][1] = gb[1] * C[s * 3 + 1] + gb[0] * C[s * 3 + 2];
+            }
+            double blk[4];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                blk[0 * 2 + d] = ga[0] * CB[0][d] + ga[1] * CB[2][d];
+                blk[1 * 2 + d] = ga[1] * CB[1][d] + ga[0] * CB[2][d];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) K[i] += blk[i] * f;
+            if (self)
+#pragma unroll
+                for (int i = 0; i < 4; i++) Ke[i] += blk[i] * f;
+        }
+        if (bad && A.flags) atomicOr(&A.flags[0], 1);
+        if (self && A.flags && (Ke[0] <= 0 || Ke[3] <= 0)) atomicOr(&A.flags[0], 2);
+    }
+};
+
+// Transient heat hex8: K (HeatTransferElement3DTransient.cpp:153-197) into
+// K, consistent capacity M (60-77) into K2.
+struct HeatHex8 {
+    static constexpr int ND = 1, NN = 8, NB = 1;
+    static __device__ __forceinline__ void accumulate(const AsmArgs &A, int e, const int *ec, int a,
+                                                      int b, bool self, double *K, double *K2) {
+        double x[8], y[8], z[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const double *q = A.xyz + 3 * (long long)ec[i];
+            x[i] = q[0];
+            y[i] = q[1];
+            z[i] = q[2];
+        }
+        const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
+        double k11 = mp[0], k22 = mp[1], k33 = mp[2], cap = mp[3];
+        int bad = 0;
+        for (int ip = 0; ip < A.nip; ip++) {
+            double ga[3], gb[3];
+            double detJ = hex8_grads(x, y, z, ip, a, b, ga, gb);
+            if (detJ < 0) bad = 1;
+            double f = detJ * c_w[ip] * A.factor;
+            K[0] += (gb[0] * k11 * ga[0] + gb[1] * k22 * ga[1] + gb[2] * k33 * ga[2]) * f;
+            K2[0] += c_N[ip * 8 + a] * c_N[ip * 8 + b] * f * cap;
+        }
+        if (bad && A.flags) atomicOr(&A.flags[0], 1);
+    }
+};
+
+// Canonical block of pair (lo, hi) summed over the elements of `n` (which is
+// lo or hi) that also hold the other node, in ascending element order.
+template <class P>
+__device__ __forceinline__ void pair_block(const AsmArgs &A, const int *n2e_ptr, const int *n2e,
+                                           int n, int lo, int hi, double *K, double *K2) {
+#pragma unroll
+    for (int i = 0; i < P::NB; i++) K[i] = K2[i] = 0;
+    const bool self = (lo == hi);
+    for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
+        int e = n2e[k];
+        int ec[P::NN];
+        int a = -1, b = -1;
+#pragma unroll
+        for (int i = 0; i < P::NN; i++) {
+            ec[i] = A.conn[(long long)e * P::NN + i];
+            if (ec[i] == lo) a = i;
+            if (ec[i] == hi) b = i;
+        }
+        if (a < 0 || b < 0) continue;
+        P::accumulate(A, e, ec, a, b, self, K, K2);
+    }
+    if (self) {  // keep the upper triangle of a diagonal block (Ke is upper-only)
+#pragma unroll
+        for (int c = 0; c < P::ND; c++)
+#pragma unroll
+            for (int d = 0; d < c; d++) {
+                K[c * P::ND + d] = K[d * P::ND + c];
+                K2[c * P::ND + d] = K2[d * P::ND + c];
+            }
+    }
+}
+
+// Oriented read of the canonical block: value at (row comp c of n, col comp d of m).
+template <int ND>
+__device__ __forceinline__ double oriented(const double *K, bool n_is_lo, int c, int d) {
+    return n_is_lo ? K[c * ND + d] : K[d * ND + c];
+}
+
+struct AsmOut {
+    const int *pair_row, *nbr, *nbr_off, *first, *rowptr;
+    const unsigned char *rmask;
+    double *val, *val_aux;
+    long long n_pairs;
+    int mode;  // 0: val = K ; 1 (heat): val = (a1 K + M)/dt, val_aux = K
+    double a1, dt;
+};
+
+template <class P>
+__global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const int *n2e_ptr,
+                                                   const int *n2e) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= O.n_pairs) return;
+    int n = O.pair_row[p], m = O.nbr[p];
+    int lo = n < m ? n : m, hi = n < m ? m : n;
+    double K[P::NB], K2[P::NB];
+    pair_block<P>(A, n2e_ptr, n2e, n, lo, hi, K, K2);
+    const bool nlo = (n == lo);
+    unsigned rn = O.rmask[n], rm = O.rmask[m];
+#pragma unroll
+    for (int c = 0; c < P::ND; c++) {
+        if ((rn >> c) & 1u) continue;
+        int I = O.first[n] + c;
+        long long base = (long long)O.rowptr[I] + O.nbr_off[p];
+        int r = 0;
+#pragma unroll
+        for (int d = 0; d < P::ND; d++) {
+            if ((rm >> d) & 1u) continue;
+            double kv = oriented<P::ND>(K, nlo, c, d);
+            if (O.mode == 1) {
+                double mv = oriented<P::ND>(K2, nlo, c, d);
+                O.val[base + r] = (O.a1 * kv + mv) / O.dt;
+                O.val_aux[base + r] = kv;
+            } else {
+                O.val[base + r] = kv;
+            }
+            r++;
+        }
+    }
+}
+
+// Per node: pseudo[I] for every free DOF I of the node =
+//   - sum over restrained J of Kscat(I,J) * presc[J]         (addMatrix lift)
+//   + fcoef * sum over all J of Kraw(I,J) * u_state[J]       (element forces, heat)
+template <class P>
+__global__ void __launch_bounds__(256) k_gather_rhs(AsmArgs A, AsmOut O, const int *n2e_ptr,
+                                                    const int *n2e, const int *nbr_ptr, int n_nodes,
+                                                    const int *ndof, const double *presc,
+                                                    const double *u_state, double fcoef,
+                                                    double *pseudo) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    unsigned rn = O.rmask[n];
+    double acc[P::ND];
+#pragma unroll
+    for (int c = 0; c < P::ND; c++) acc[c] = 0;
+    const int nd_n = ndof[n];
+    if (nd_n > 0 && rn != (1u << nd_n) - 1u) {
+        for (int p = nbr_ptr[n]; p < nbr_ptr[n + 1]; p++) {
+            int m = O.nbr[p];
+            unsigned rm = O.rmask[m];
+            if (rm == 0 && u_state == nullptr) continue;
+            int lo = n < m ? n : m, hi = n < m ? m : n;
+            double K[P::NB], K2[P::NB];
+            pair_block<P>(A, n2e_ptr, n2e, n, lo, hi, K, K2);
+            const bool nlo = (n == lo);
+#pragma unroll
+            for (int c = 0; c < P::ND; c++) {
+                if ((rn >> c) & 1u) continue;
+#pragma unroll
+                for (int d = 0; d < P::ND; d++) {
+                    int J = O.first[m] + d;
+                    double kv = oriented<P::ND>(K, nlo, c, d);
+                    if ((rm >> d) & 1u) {
+                        double sv = kv;
+                        if (O.mode == 1) sv = (O.a1 * kv + oriented<P::ND>(K2, nlo, c, d)) / O.dt;
+                        acc[c] -= sv * presc[J];
+                    }
+                    if (u_state) acc[c] += fcoef * kv * u_state[J];
+                }
+            }
+        }
+    }
+    for (int c = 0; c < nd_n; c++) pseudo[O.first[n] + c] = ((rn >> c) & 1u) ? 0.0 : acc[c];
+}
+
+// Restrained rows hold 1 on the diagonal (equation.cpp:500).
+__global__ void k_restrained_diag(int n_dof, const unsigned char *rest, const int *rowptr,
+                                  double *val, double *val_aux) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_dof || !rest[i]) return;
+    val[rowptr[i]] = 1.0;
+    if (val_aux) val_aux[rowptr[i]] = 1.0;
+}
+
+// Gauss-Legendre rule, the reference's Newton construction (gauss.cpp:14-60).
+void gauss_rule(int k, double *c, double *w) {
+    const double EPS = 3e-15;
+    int m = (k + 1) / 2;
+    for (int i = 0; i < m; i++) {
+        double z = cos(M_PI * ((double)i + 1 - .25) / ((double)k + .5)), z1, P1, P2, P3, PP;
+        do {
+            P1 = 1;
+            P2 = 0;
+            for (int j = 0; j < k; j++) {
+                P3 = P2;
+                P2 = P1;
+                P1 = ((2 * j + 1) * z * P2 - j * P3) / (j + 1);
+            }
+            PP = k * (z * P1 - P2) / (z * z - 1);
+            z1 = z;
+            z = z1 - P1 / PP;
+        } while (fabs(z - z1) > EPS);
+        c[i] = -z;
+        c[k - i - 1] = z;
+        w[i] = 2 / ((1 - z * z) * PP * PP);
+        w[k - i - 1] = w[i];
+    }
+}
+
+// Upload the Gauss table for the element family: returns nip.
+int upload_gauss(bhip_ctx *ctx, int nnpe, int order, int *nip_out) {
+    double c[16], w[16];
+    gauss_rule(order, c, w);
+    std::vector<double> hw(kMaxIp, 0.0), hdN(kMaxIp * 24, 0.0), hN(kMaxIp * 8, 0.0);
+    int ip = 0;
+    if (nnpe == 8) {
+        if (order * order * order > kMaxIp) return bhip_fail(ctx, BHIP_ERR_CAPACITY, "integration order too high");
+        // HexQuadPoints order: xi outer, eta, zeta inner (gauss.cpp:331-349)
+        for (int i = 0; i < order; i++)
+            for (int j = 0; j < order; j++)
+                for (int k = 0; k < order; k++, ip++) {
+                    double xi = c[i], eta = c[j], zeta = c[k];
+                    hw[ip] = w[i] * w[j] * w[k];
+                    double *dxi = &hdN[ip * 24], *deta = dxi + 8, *dzeta = dxi + 16;
+                    // derivHex8 / shapeHex8, shape.cpp:210-260
+                    double t1 = eta * zeta;
+                    dxi[0] = -1.0 / 8.0 + eta / 8 + zeta / 8 - t1 / 8;
+                    dxi[1] = -1.0 / 8.0 - eta / 8 + zeta / 8 + t1 / 8;
+                    dxi[2] = -1.0 / 8.0 - eta / 8 - zeta / 8 - t1 / 8;
+                    dxi[3] = -1.0 / 8.0 + eta / 8 - zeta / 8 + t1 / 8;
+                    dxi[4] = 1.0 / 8.0 - eta / 8 - zeta / 8 + t1 / 8;
+                    dxi[5] = 1.0 / 8.0 + eta / 8 - zeta / 8 - t1 / 8;
+                    dxi[6] = 1.0 / 8.0 + eta / 8 + zeta / 8 + t1 / 8;
+                    dxi[7] = 1.0 / 8.0 - eta / 8 + zeta / 8 - t1 / 8;
+                    t1 = xi * zeta;
+                    deta[0] = -1.0 / 8.0 + xi / 8 + zeta / 8 - t1 / 8;
+                    deta[1] = 1.0 / 8.0 - xi / 8 - zeta / 8 + t1 / 8;
+                    deta[2] = 1.0 / 8.0 - xi / 8 + zeta / 8 - t1 / 8;
+                    deta[3] = -1.0 / 8.0 + xi / 8 - zeta / 8 + t1 / 8;
+                    deta[4] = -1.0 / 8.0 - xi / 8 + zeta / 8 + t1 / 8;
+                    deta[5] = 1.0 / 8.0 + xi / 8 - zeta / 8 - t1 / 8;
+                    deta[6] = 1.0 / 8.0 + xi / 8 + zeta / 8 + t1 / 8;
+                    deta[7] = -1.0 / 8.0 - xi / 8 - zeta / 8 - t1 / 8;
+                    t1 = xi * eta;
+                    dzeta[0] = -1.0 / 8.0 + xi / 8 + eta / 8 - t1 / 8;
+                    dzeta[1] = -1.0 / 8.0 + xi / 8 - eta / 8 + t1 / 8;
+                    dzeta[2] = 1.0 / 8.0 - xi / 8 + eta / 8 - t1 / 8;
+                    dzeta[3] = 1.0 / 8.0 - xi / 8 - eta / 8 + t1 / 8;
+                    dzeta[4] = -1.0 / 8.0 - xi / 8 + eta / 8 + t1 / 8;
+                    dzeta[5] = -1.0 / 8.0 - xi / 8 - eta / 8 - t1 / 8;
+                    dzeta[6] = 1.0 / 8.0 + xi / 8 + eta / 8 + t1 / 8;
+                    dzeta[7] = 1.0 / 8.0 + xi / 8 - eta / 8 - t1 / 8;
+                    double t2 = xi * zeta, t3 = eta * zeta, t4 = xi * eta * zeta;
+                    double *S = &hN[ip * 8];
+                    t1 = xi * eta;
+                    S[0] = 1.0 / 8.0 - xi / 8 - eta / 8 + t1 / 8 - zeta / 8 + t2 / 8 + t3 / 8 - t4 / 8;
+                    S[1] = 1.0 / 8.0 - xi / 8 + eta / 8 - t1 / 8 - zeta / 8 + t2 / 8 - t3 / 8 + t4 / 8;
+                    S[2] = 1.0 / 8.0 - xi / 8 + eta / 8 - t1 / 8 + zeta / 8 - t2 / 8 + t3 / 8 - t4 / 8;
+                    S[3] = 1.0 / 8.0 - xi / 8 - eta / 8 + t1 / 8 + zeta / 8 - t2 / 8 - t3 / 8 + t4 / 8;
+                    S[4] = 1.0 / 8.0 + xi / 8 - eta / 8 - t1 / 8 - zeta / 8 - t2 / 8 + t3 / 8 + t4 / 8;
+                    S[5] = 1.0 / 8.0 + xi / 8 + eta / 8 + t1 / 8 - zeta / 8 - t2 / 8 - t3 / 8 - t4 / 8;
+                    S[6] = 1.0 / 8.0 + xi / 8 + eta / 8 + t1 / 8 + zeta / 8 + t2 / 8 + t3 / 8 + t4 / 8;
+                    S[7] = 1.0 / 8.0 + xi / 8 - eta / 8 - t1 / 8 + zeta / 8 + t2 / 8 - t3 / 8 - t4 / 8;
+                }
+    } else if (nnpe == 4) {
+        // QuadQuadPoints order: xi outer, eta inner (gauss.cpp:851-863); derivQuad4 (shape.cpp:155-168)
+        for (int i = 0; i < order; i++)
+            for (int k = 0; k < order; k++, ip++) {
+                double xi = c[i], eta = c[k];
+                hw[ip] = w[i] * w[k];
+                double *dxi = &hdN[ip * 24], *deta = dxi + 4;
+                dxi[0] = -.25 * (1 - eta);
+                dxi[1] = .25 * (1 - eta);
+                dxi[2] = .25 * (1 + eta);
+                dxi[3] = -.25 * (1 + eta);
+                deta[0] = -.25 * (1 - xi);
+                deta[1] = -.25 * (1 + xi);
+                deta[2] = .25 * (1 + xi);
+                deta[3] = .25 * (1 - xi);
+                double *S = &hN[ip * 8];
+                S[0] = .25 * (1 - xi) * (1 - eta);
+                S[1] = .25 * (1 + xi) * (1 - eta);
+                S[2] = .25 * (1 + xi) * (1 + eta);
+                S[3] = .25 * (1 - xi) * (1 + eta);
+            }
+    } else {
+        return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "element family not built");
+    }
+    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_w), hw.data(), sizeof(double) * kMaxIp, 0,
+                                         hipMemcpyHostToDevice, ctx->stream));
+    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dN), hdN.data(), sizeof(double) * kMaxIp * 24, 0,
+                                         hipMemcpyHostToDevice, ctx->stream));
+    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_N), hN.data(), sizeof(double) * kMaxIp * 8, 0,
+                                         hipMemcpyHostToDevice, ctx->stream));
+    *nip_out = ip;
+    return BHIP_OK;
+}
+
+template <class P>
+int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, double fcoef) {
+    const int B = 256;
+    hipStream_t s = ctx->stream;
+    k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    k_restrained_diag<<<grid_for(ctx->n_dof, B), B, 0, s>>>(ctx->n_dof, ctx->rest.p, ctx->rowptr.p,
+                                                            ctx->val.p, O.mode == 1 ? ctx->val_aux.p : nullptr);
+    BHIP_TRY(ctx, hipGetLastError());
+    k_gather_rhs<P><<<grid_for(ctx->n_nodes, B), B, 0, s>>>(
+        A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->nbr_ptr.p, ctx->n_nodes, ctx->ndof.p, ctx->presc.p,
+        u_state, fcoef, ctx->pseudo.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    return BHIP_OK;
+}
+
+}  // namespace
+
+int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double *u_state) {
+    if (!ctx->pattern_ready) return bhip_fail(ctx, BHIP_ERR_STATE, "pattern not built");
+    if (!ctx->have_mat) return bhip_fail(ctx, BHIP_ERR_STATE, "materials not set");
+    BHIP_PROPAGATE(bhip_ensure_vectors(ctx));
+    bhip_invalidate_graphs(ctx);
+    int nip = 0;
+    BHIP_PROPAGATE(upload_gauss(ctx, ctx->nnpe, ctx->order, &nip));
+    BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 4, ctx->stream));
+
+    AsmArgs A;
+    A.xyz = ctx->xyz.p;
+    A.conn = ctx->conn.p;
+    A.mat = ctx->mat.p;
+    A.elem_group = ctx->elem_group.n ? ctx->elem_group.p : nullptr;
+    A.n_el = ctx->n_el;
+    A.n_groups = ctx->n_groups;
+    A.mat_stride = ctx->mat_stride;
+    A.nnpe = ctx->nnpe;
+    A.nip = nip;
+    A.factor = ctx->factor;
+    A.tan = nullptr;
+    A.flags = ctx->flags.p;
+    AsmOut O;
+    O.pair_row = ctx->pair_row.p;
+    O.nbr = ctx->nbr.p;
+    O.nbr_off = ctx->nbr_off.p;
+    O.first = ctx->dof_first.p;
+    O.rowptr = ctx->rowptr.p;
+    O.rmask = ctx->node_rmask.p;
+    O.val = ctx->val.p;
+    O.val_aux = nullptr;
+    O.n_pairs = ctx->n_pairs;
+    O.mode = 0;
+    O.a1 = 0;
+    O.dt = 1;
+
+    const int kind = ctx->mat_kind;
+    int st = BHIP_OK;
+    if (what == BHIP_ASM_STIFFNESS) {
+        if (ctx->nnpe == 8 && ctx->dim == 3 && ctx->max_ndof == 3 && ctx->uniform_ndof) {
+            if (kind == BHIP_MAT_ISO_ELASTIC)
+                st = launch_assembly<ElasHex8<BHIP_MAT_ISO_ELASTIC>>(ctx, A, O, nullptr, 0.0);
+            else if (kind == BHIP_MAT_CMAT3D)
+                st = launch_assembly<ElasHex8<BHIP_MAT_CMAT3D>>(ctx, A, O, nullptr, 0.0);
+            else if (kind == BHIP_MAT_J2)  // elastic stiffness of the J2 material
+                st = launch_assembly<ElasHex8<BHIP_MAT_ISO_ELASTIC>>(ctx, A, O, nullptr, 0.0);
+            else
+                return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "material kind not valid for hex8 stiffness");
+        } else if (ctx->nnpe == 4 && ctx->dim == 2 && ctx->max_ndof == 2 && ctx->uniform_ndof &&
+                   kind == BHIP_MAT_PLANE_STRESS) {
+            st = launch_assembly<ElasQuad4>(ctx, A, O, nullptr, 0.0);
+        } else {
+            return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "element/material combination not built");
+        }
+    } else if (what == BHIP_ASM_HEAT_TRANSIENT) {
+        if (!(ctx->nnpe == 8 && ctx->dim == 3 && ctx->max_ndof == 1 && ctx->uniform_ndof &&
+              kind == BHIP_MAT_HEAT))
+            return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "heat transient needs hex8, 1 DOF/node, heat material");
+        if (!coef) return bhip_fail(ctx, BHIP_ERR_ARG, "heat transient needs coef = {alpha, dt}");
+        double alpha = coef[0], dt = coef[1];
+        if (!(dt > 0)) return bhip_fail(ctx, BHIP_ERR_ARG, "dt must be positive");
+        BHIP_TRY(ctx, ctx->val_aux.resize(ctx->val.n));
+        O.mode = 1;
+        O.val_aux = ctx->val_aux.p;
+        O.a1 = alpha * dt;  // DiffusionModel.cpp:745
+        O.dt = dt;
+        double a2 = (1 - alpha) * dt;
+        double fcoef = -(O.a1 + a2) / dt;  // formF_s_new: -(a1+a2) K u_s / dt
+        st = launch_assembly<HeatHex8>(ctx, A, O, u_state, fcoef);
+    } else if (what == BHIP_ASM_PLASTIC_TANGENT) {
+        if (!(ctx->nnpe == 8 && ctx->dim == 3 && kind == BHIP_MAT_J2))
+            return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "plastic tangent needs hex8 and a J2 material");
+        A.tan = bhip_plastic_tangent(ctx);
+        if (!A.tan) return bhip_fail(ctx, BHIP_ERR_STATE, "plastic state not initialised");
+        st = launch_assembly<ElasHex8<BHIP_MAT_J2>>(ctx, A, O, nullptr, 0.0);
+    } else {
+        return bhip_fail(ctx, BHIP_ERR_ARG, "unknown assembly kind");
+    }
+    BHIP_PROPAGATE(st);
+    int hflags[4] = {0, 0, 0, 0};
+    BHIP_TRY(ctx, hipMemcpyAsync(hflags, ctx->flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->assembled = true;
+    ctx->f32_ready = false;
+    if (hflags[0] & 1) return bhip_fail(ctx, BHIP_ERR_NEG_DETJ, "negative Jacobian determinant (inverted element)");
+    if (hflags[0] & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "bad diagonal in an element stiffness");
+    return BHIP_OK;
+}

@@ -1,0 +1,193 @@
+// bhip_internal.hpp -- context, device buffers and shared device helpers of
+// libbeta_hip.so (MI355X / gfx950).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "beta_hip.h"
+
+// ---------------------------------------------------------------- errors
+int bhip_fail(bhip_ctx *ctx, int status, const std::string &msg);
+
+#define BHIP_TRY(ctx, call)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return bhip_fail((ctx), e_ == hipErrorOutOfMemory ? BHIP_ERR_OOM : BHIP_ERR_HIP, \
+                             std::string(#call) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+#define BHIP_PROPAGATE(expr)                                                             \
+    do {                                                                                 \
+        int s_ = (expr);                                                                 \
+        if (s_ != BHIP_OK) return s_;                                                    \
+    } while (0)
+
+// ----------------------------------------------------------- device buffer
+template <class T>
+struct dvec {
+    T *p = nullptr;
+    size_t n = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // (Re)allocate to exactly count elements; contents undefined.
+    hipError_t resize(size_t count) {
+        if (count == n && p) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Device-side solver state of one PCG (FP64 or FP32 inner).  Lives in device
+// memory so that kernels chain without host round trips; the host polls it.
+struct PcgState {
+    double rz;        // r.z of the current iterate
+    double pAp;       // p.Ap of the current iteration
+    double alpha, beta;
+    double rr, xx, bb;  // |r|^2, |x|^2, |b|^2
+    double tol;
+    double norm;      // last stopping norm
+    double rr0;       // |r0|^2 of this (inner) solve
+    int done;         // 1 once converged or out of iterations
+    int iter;         // iterations completed
+    int maxit;
+    int stop_rule;    // BHIP_STOP_* or 2 = inner relative reduction
+    int converged;
+    unsigned cnt[4];  // last-block counters (reset by the last block)
+    int pad;
+};
+
+constexpr int kMaxNnpe = 8;
+constexpr int kMaxIp = 64;    // order <= 4 in 3D
+constexpr int kRedBlocks = 2048;  // upper bound of grid-stride reduction grids
+
+struct bhip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    int n_cu = 256;
+
+    // ---- mesh
+    int dim = 0, n_nodes = 0, n_el = 0, nnpe = 0, n_dof = 0, max_ndof = 0;
+    bool uniform_ndof = true;
+    std::vector<int> h_conn, h_ndof, h_dof_first;
+    dvec<double> xyz;
+    dvec<int> conn, dof_first, ndof;
+    // ---- restraints
+    bool have_rest = false;
+    std::vector<unsigned char> h_rest;
+    dvec<unsigned char> rest;       // per DOF
+    dvec<unsigned char> node_rmask; // per node: bit c set if comp c restrained
+    dvec<double> presc;             // per DOF prescribed value
+    // ---- node graph and pattern
+    bool pattern_ready = false;
+    long long n_pairs = 0, nnz = 0, nnz_lower = 0;
+    dvec<int> n2e_ptr, n2e;         // node -> incident elements (ascending)
+    dvec<int> nbr_ptr, nbr, pair_row, nbr_off;  // node adjacency incl. self
+    dvec<int> rowptr, colind, diag_pos;          // full symmetric CSR
+    dvec<double> val;                            // FP64 values
+    dvec<double> val_aux;                        // second value array (heat: K part)
+    // ---- FP32 mirror for the inner solve
+    bool f32_ready = false;
+    bool use_bsr = false;
+    dvec<float> val32;       // CSR FP32 values (non-BSR case)
+    dvec<float> bval32;      // BSR3 FP32 values, SoA: 9 x n_pairs
+    // ---- materials / integration
+    int mat_kind = 0, n_groups = 0, mat_stride = 0;
+    dvec<double> mat;
+    dvec<int> elem_group;
+    bool have_mat = false;
+    int order = 2;
+    double factor = 1.0;
+    bool assembled = false;
+    // ---- vectors (FP64)
+    dvec<double> load, pseudo, rhs, x, x0, r, p, q, dinv, tmp;
+    // ---- vectors (FP32 inner)
+    dvec<float> r32, p32, q32, d32, dinv32;
+    // ---- reductions / state
+    dvec<double> partials;  // kRedBlocks * 4
+    dvec<PcgState> state;
+    dvec<int> flags;        // assembly / diagonal check flags
+    PcgState *h_state = nullptr;  // pinned mirror
+    // ---- graphs
+    hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
+    int graph64_iters = 0, graph32_iters = 0;
+    double spmv_bytes64 = 0, spmv_bytes32 = 0;
+};
+
+// -------------------------------------------------------------- launching
+inline int grid_for(long long n, int block, int max_blocks = 1 << 30) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_blocks) g = max_blocks;
+    return (int)g;
+}
+
+// XCD-aware block remap: gfx950 dispatches consecutive workgroups round-robin
+// over 8 XCDs, each with its own L2.  Map so that every XCD works on one
+// contiguous range of logical blocks (neighbouring rows share gathered x
+// entries, which then stay in that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    const int nx = 8;
+    int per = nblocks / nx;
+    int rem = nblocks - per * nx;
+    int xcd = bid % nx;
+    int idx = bid / nx;
+    // XCDs [0, rem) hold per+1 logical blocks, the rest hold per.
+    int base = (xcd < rem) ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+    return base + idx;
+}
+
+// wave64 reduction of a double (all lanes get the sum)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block reduction (blockDim.x multiple of 64, <= 1024); result in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *smem /* 16*NV */) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) smem[wid * NV + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0;
+            for (int w = 0; w < nw; w++) s += smem[w * NV + k];
+            v[k] = s;
+        }
+    }
+}
+
+// --------------------------------------------------------- internal entry
+int bhip_ensure_vectors(bhip_ctx *ctx);
+int bhip_pattern_build(bhip_ctx *ctx);
+int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double *u_state);
+int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *opts, bhip_stats *stats);
+int bhip_spmv_dev(bhip_ctx *ctx, const double *x, double *y);
+int bhip_export_lower(bhip_ctx *ctx, int *rowptr, int *colind, double *val);
+int bhip_bench_spmv_impl(bhip_ctx *ctx, int fp32, int reps, double *ms, double *bytes);
+void bhip_invalidate_graphs(bhip_ctx *ctx);
+void bhip_plastic_release(bhip_ctx *ctx);
+const double *bhip_plastic_tangent(bhip_ctx *ctx);

@@ -1,0 +1,174 @@
+/*
+ * beta_hip.h -- C ABI of libbeta_hip.so, the MI355X (gfx950) implementation
+ * of BETA's finite-element hot path: element integration, global assembly
+ * into a symmetric sparse matrix, and the Jacobi-PCG solve.
+ *
+ * Every entry point cites the reference interface it replaces (paths are
+ * relative to the reference root).  The reference's solver/storage plug-in
+ * is LargeMatrix (beta/math/equation/largeMatrix.cpp, sparse.cpp), its
+ * assembler plug-in is BasicAssembler (beta/assemblers/*), and the linear
+ * system facade is Equations (beta/math/equation/equation.cpp).
+ *
+ * Conventions
+ *  - The caller owns every host array; the context owns device buffers
+ *    (mesh, pattern, matrix values, preconditioner, solver vectors).
+ *  - Every call returns BHIP_OK (0) or a negative status; the message of the
+ *    last failure is available from bhip_last_error().  The reference
+ *    terminates the process in these cases (exit(1)/FatalError); the C++
+ *    adapters in betanoox_amd/host reproduce that when asked to.
+ *  - A context is used from one host thread at a time.
+ *  - DOF numbering is the reference's: node-major, cumulative first DOF per
+ *    node in node order (beta/mesh/node.cpp:206-214), element-local DOF order
+ *    node-major (beta/elements/BasicElement.cpp:49-68).
+ */
+#ifndef BETA_HIP_H
+#define BETA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define BHIP_OK                 0
+#define BHIP_ERR_ARG           -1   /* bad argument or shape */
+#define BHIP_ERR_HIP           -2   /* HIP runtime failure */
+#define BHIP_ERR_OOM           -3   /* device allocation failed */
+#define BHIP_ERR_NOT_CONVERGED -4   /* sparse.cpp:225-229 */
+#define BHIP_ERR_ZERO_DIAG     -5   /* largeMatrix.cpp:55-62 */
+#define BHIP_ERR_NEG_DIAG      -6   /* largeMatrix.cpp:50-53, BasicElement.cpp:116-130 */
+#define BHIP_ERR_NEG_DETJ      -7   /* ElasticityElement3D.cpp:653-670 */
+#define BHIP_ERR_STATE         -8   /* call out of order (e.g. solve before assemble) */
+#define BHIP_ERR_UNSUPPORTED   -9   /* element/material combination not built */
+#define BHIP_ERR_CAPACITY     -10   /* size beyond a documented limit */
+
+/* ------------------------------------------------------------- materials */
+#define BHIP_MAT_ISO_ELASTIC    1   /* 3D: E, nu                      (elastic.cpp:335-376, isotropic input) */
+#define BHIP_MAT_CMAT3D         2   /* 3D: 36 doubles, row-major C    (ElasticMaterial::globalCmat) */
+#define BHIP_MAT_PLANE_STRESS   3   /* 2D: e11 e22 nu12 G12           (pStress.cpp:49-117) */
+#define BHIP_MAT_HEAT           4   /* 3D scalar: k11 k22 k33 capacity (heatTransferMaterial.cpp:65-136) */
+#define BHIP_MAT_J2             5   /* 3D: E nu sigma0 H              (plastic.cpp, POWER curve m_n=1) */
+
+/* -------------------------------------------------------------- assembly */
+#define BHIP_ASM_STIFFNESS      1   /* update("K") + addMatrix(Ke, map, size, NULL) */
+#define BHIP_ASM_HEAT_TRANSIENT 2   /* update("KM"), formK_s, formF_s_new, addMatrix(Mbar, map, size, Fbar) */
+#define BHIP_ASM_PLASTIC_TANGENT 3  /* update("KI") with the consistent tangent from the last return map */
+
+/* ----------------------------------------------------------------- solve */
+#define BHIP_STOP_REF_R_OVER_X  0   /* sqrt(|r|^2/|x|^2) < tol   (sparse.cpp:223) */
+#define BHIP_STOP_REL_RESID     1   /* |b - K x| / |b| <= tol     (true residual checked at exit) */
+
+#define BHIP_PREC_FP64          0   /* plain FP64 Jacobi-PCG on FP64 CSR */
+#define BHIP_PREC_FP32_IR       1   /* FP32 inner Jacobi-PCG (FP32 values), FP64 iterative refinement */
+
+typedef struct bhip_ctx bhip_ctx;
+
+typedef struct bhip_solve_opts {
+    double tol;          /* stopping tolerance (reference default 1e-4, equation.cpp:1566-1567) */
+    int    maxit;        /* iteration cap (reference default 10000) */
+    int    stop_rule;    /* BHIP_STOP_* */
+    int    precision;    /* BHIP_PREC_* */
+    double inner_tol;    /* FP32_IR: relative residual reduction per inner solve */
+    int    max_outer;    /* FP32_IR: refinement cap */
+    int    check_every;  /* host polls convergence every this many iterations (0 = 32) */
+    int    use_x0;       /* 1: start from the x0 held by the context (bhip_set_x0), 0: from zero */
+} bhip_solve_opts;
+
+typedef struct bhip_stats {
+    int    iters;            /* PCG iterations (inner iterations for FP32_IR) */
+    int    outer_iters;      /* refinement steps (FP32_IR), 1 for FP64 */
+    double ref_norm;         /* final sqrt(|r|^2/|x|^2) of the recursive residual */
+    double rel_resid;        /* final |b - K x|_2 / |b|_2, true residual in FP64 */
+    double ms_total;         /* wall time of the solve on the device (HIP events) */
+    double ms_spmv_avg;      /* mean duration of one SpMV kernel inside the solve */
+    int    spmv_launches;    /* SpMV kernels launched in the timed region */
+    double spmv_bytes;       /* algorithmic bytes of one SpMV */
+} bhip_stats;
+
+/* --------------------------------------------------------------- context */
+/* LargeMatrix ctor (largeMatrix.cpp:13-24) + Equations ctor (equation.cpp:1540-1570). */
+int         bhip_create(int device, bhip_ctx **out);
+void        bhip_destroy(bhip_ctx *ctx);
+const char *bhip_last_error(const bhip_ctx *ctx);
+const char *bhip_version(void);
+
+/* ------------------------------------------------------------------ mesh */
+/* BasicMesh::ReadMesh (BasicMesh.cpp:79-190) data + Node::setNumDofPerNode
+ * (node.cpp:158-227).  xyz: n_nodes*dim doubles; conn: n_el*nnpe ints in the
+ * reference node order (hex8: BasicElement.cpp:518-525; quad4 CCW);
+ * ndof_per_node NULL means every node has ndof_uniform DOFs. */
+int bhip_set_mesh(bhip_ctx *ctx, int dim, int n_nodes, const double *xyz, int n_el, int nnpe,
+                  const int *conn, const int *ndof_per_node, int ndof_uniform);
+int bhip_num_dof(const bhip_ctx *ctx);
+/* first DOF of every node (Node::getFirstDof) */
+int bhip_get_dof_first(bhip_ctx *ctx, int *dof_first);
+
+/* ----------------------------------------------------------- restraints */
+/* Equations::setRestraint + setSolution (equation.cpp:306-341): restrained[i]
+ * != 0 marks DOF i, values[i] is its prescribed value.  Changing the set of
+ * restrained DOFs invalidates the pattern; bhip_set_prescribed only changes
+ * the values. */
+int bhip_set_restraints(bhip_ctx *ctx, const unsigned char *restrained, const double *values);
+int bhip_set_prescribed(bhip_ctx *ctx, const double *values);
+
+/* -------------------------------------------------------------- pattern */
+/* BasicModel::allocateEquationsMatrix (BasicModel.cpp:1022-1033) ->
+ * Equations::specifyNonZeroLocation (equation.cpp:1216-1267) ->
+ * SymmetricSparseMatrix::specifyNonZeroLocation/allocate (sparse.cpp:479-495),
+ * built on the device once. */
+int bhip_build_pattern(bhip_ctx *ctx);
+int bhip_pattern_nnz(const bhip_ctx *ctx, long long *nnz_lower, long long *nnz_full);
+/* The reference storage layout: lower triangle, sorted columns, diagonal last
+ * in every row (sparse.cpp:144), rowptr has n_dof+1 entries. */
+int bhip_get_pattern_lower(bhip_ctx *ctx, int *rowptr, int *colind);
+int bhip_get_values_lower(bhip_ctx *ctx, double *val);
+/* Full symmetric CSR as stored on the device (int32 indices). */
+int bhip_get_csr(bhip_ctx *ctx, int *rowptr, int *colind, double *val);
+
+/* ------------------------------------------------------------ materials */
+/* MaterialFactory + SetElementProperty (Factory_Material.cpp:13-49,
+ * BasicModel.cpp:664-771): params holds n_groups records of the kind's
+ * length; elem_group[e] selects the record of element e (NULL: element e uses
+ * record e when n_groups == n_el, record 0 when n_groups == 1). */
+int bhip_set_materials(bhip_ctx *ctx, int kind, int n_groups, const double *params,
+                       const int *elem_group);
+/* IsoElement::setIntegrationOrder (isoElement.cpp:463-478) and
+ * getIntegrationFactor() (thickness in 2D, 1 in 3D). */
+int bhip_set_integration(bhip_ctx *ctx, int order, double factor);
+
+/* ------------------------------------------------------------- assembly */
+/* SerialAssembler::assembleK (SerialAssembler.cpp:35-59) with
+ * Equations::addMatrix (equation.cpp:445-562): zeroes K and the pseudo-load,
+ * integrates every element and scatters it with restrained DOFs eliminated
+ * (identity rows, lifted columns).  coef: BHIP_ASM_HEAT_TRANSIENT takes
+ * {alpha, dt} and u_state = u_s (n_dof); others pass NULL. */
+int bhip_assemble(bhip_ctx *ctx, int what, const double *coef, const double *u_state);
+/* Equations::pseudoLoadVector after assembly (lift + element forces). */
+int bhip_get_pseudo_load(bhip_ctx *ctx, double *f);
+/* Equations::loadVector (addLoadToEqn, equation.cpp:851-868). */
+int bhip_set_load(bhip_ctx *ctx, const double *load);
+/* Equations::solve RHS build (equation.cpp:1324-1342): b[i] = load+pseudo for
+ * free i, the prescribed value for restrained i. */
+int bhip_build_rhs(bhip_ctx *ctx);
+int bhip_get_rhs(bhip_ctx *ctx, double *b);
+int bhip_set_rhs(bhip_ctx *ctx, const double *b);
+int bhip_set_x0(bhip_ctx *ctx, const double *x0);
+
+/* ---------------------------------------------------------------- solve */
+/* SymmetricSparseMatrix::solve (sparse.cpp:114-242) on the resident system:
+ * rhs and x0 already on the device, solution left on the device. */
+int bhip_solve(bhip_ctx *ctx, const bhip_solve_opts *opts, bhip_stats *stats);
+int bhip_get_solution(bhip_ctx *ctx, double *x);
+/* LargeMatrix::solve(double *b, double *x0): b holds the RHS on entry and the
+ * solution on return; x0 may be NULL (zero initial guess). */
+int bhip_pcg_solve(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_opts *opts,
+                   bhip_stats *stats);
+/* LargeMatrix::vectorProduct (sparse.cpp:452-478): y = K x. */
+int bhip_spmv(bhip_ctx *ctx, const double *x, double *y);
+/* Time `reps` FP64 (fp32 = 0) or FP32 (fp32 = 1) SpMVs on resident vectors;
+ * returns the mean kernel time in ms (HIP events). */
+int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""Parity of the HIP path (libbeta_hip.so, through the C ABI) with the oracle
+on the same seeded inputs.
+
+Tolerances (stated per check):
+  - pattern: bit-exact (integer)
+  - K entries: |K_gpu - K_orc| <= 1e-12 * max_j |K_orc(i, j)| per row i
+  - pseudo-load / RHS: <= 1e-12 * max |b|
+  - SpMV: |y_gpu - y_orc|_inf <= 1e-13 * (|K| |x|)_inf
+  - FP64 PCG: true relative residual <= tol, displacement error vs the oracle's
+    FP64 PCG solution <= 1e-8 relative (both solved to 1e-12)
+  - FP32-inner + FP64 refinement: true relative residual <= 1e-8 and
+    displacement error vs the oracle <= 1e-6 relative (error <= cond * residual)
+"""
+import numpy as np
+import pytest
+
+from betanoox_amd import capi, meshgen, setup_problem
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(kind):
+    if kind == "quad16":
+        return meshgen.plane_stress_quad(16)
+    if kind == "hex6":
+        return meshgen.elastic_cube(6)
+    if kind == "hex5_rollers":
+        p = meshgen.plastic_bar(5)
+        p.mat_kind = capi.MAT_ISO_ELASTIC
+        p.mat = np.array([[200.0, 0.3]])
+        p.prescribed = p.prescribed * 100.0  # nonzero prescribed values exercise the lift
+        return p
+    raise KeyError(kind)
+
+
+def _row_scaled_err(rp, ci, a, b):
+    err = 0.0
+    for i in range(len(rp) - 1):
+        s = slice(rp[i], rp[i + 1])
+        scale = np.abs(b[s]).max() if rp[i + 1] > rp[i] else 1.0
+        if scale == 0:
+            scale = 1.0
+        err = max(err, np.abs(a[s] - b[s]).max() / scale)
+    return err
+
+
+@pytest.mark.parametrize("kind", ["quad16", "hex6", "hex5_rollers"])
+def test_pattern_values_rhs(hip_ctx_factory, kind):
+    prob = _problem(kind)
+    print("case", kind, prob.n_dof, flush=True)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    eq = O.assemble_problem(prob)
+    rp_o, ci_o = eq.pattern()
+    rp_g, ci_g = ctx.pattern_lower()
+    assert np.array_equal(rp_o, rp_g)
+    assert np.array_equal(ci_o, ci_g)
+    nl, nf = ctx.nnz()
+    assert nl == eq.nnz()
+    assert nf == 2 * nl - prob.n_dof  # full = both triangles, one diagonal per row
+    # diagonal last in every row of the lower export (sparse.cpp:144)
+    assert np.all(ci_g[rp_g[1:] - 1] == np.arange(prob.n_dof))
+    ctx.assemble(capi.ASM_STIFFNESS)
+    v_g = ctx.values_lower()
+    v_o = eq.values()
+    assert _row_scaled_err(rp_o, ci_o, v_g, v_o) <= 1e-12
+    # pseudo-load (lift) and RHS
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    b_o = eq.rhs()
+    b_g = ctx.rhs()
+    assert np.abs(b_g - b_o).max() <= 1e-12 * max(np.abs(b_o).max(), 1e-300)
+    # full CSR is exactly symmetric
+    rp, ci, v = ctx.csr()
+    import scipy.sparse as sp
+    A = sp.csr_matrix((v, ci, rp), shape=(prob.n_dof, prob.n_dof))
+    assert (A - A.T).count_nonzero() == 0
+
+
+@pytest.mark.parametrize("kind", ["quad16", "hex6"])
+def test_spmv(hip_ctx_factory, kind):
+    prob = _problem(kind)
+    print("case", kind, flush=True)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    eq = O.assemble_problem(prob)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(prob.n_dof)
+    y_o = eq.spmv(x)
+    y_g = ctx.spmv(x)
+    rp, ci, v = ctx.csr()
+    import scipy.sparse as sp
+    A = sp.csr_matrix((np.abs(v), ci, rp), shape=(prob.n_dof, prob.n_dof))
+    scale = (A @ np.abs(x)).max()
+    assert np.abs(y_g - y_o).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("kind", ["quad16", "hex6", "hex5_rollers"])
+def test_pcg_fp64(hip_ctx_factory, kind):
+    prob = _problem(kind)
+    print("case", kind, flush=True)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    eq = O.assemble_problem(prob)
+    b = eq.rhs()
+    st, x_o, it_o, _ = eq.pcg(b, eps=1e-12, stop_rule=1)
+    assert st == 0
+    s = ctx.solve(tol=1e-12, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64)
+    x_g = ctx.solution()
+    assert s["rel_resid"] <= 1e-11
+    assert np.linalg.norm(x_g - x_o) <= 1e-8 * np.linalg.norm(x_o)
+    assert abs(s["iters"] - it_o) <= 2
+    # reference stopping rule sqrt(|r|^2/|x|^2) < eps (sparse.cpp:223)
+    st, x_o2, it_o2, _ = eq.pcg(b, eps=1e-8, stop_rule=0)
+    s2 = ctx.solve(tol=1e-8, stop_rule=capi.STOP_REF_R_OVER_X)
+    assert abs(s2["iters"] - it_o2) <= 2
+    # deterministic: the same solve twice gives the same bits
+    ctx.solve(tol=1e-8, stop_rule=capi.STOP_REF_R_OVER_X)
+    x3 = ctx.solution()
+    ctx.solve(tol=1e-8, stop_rule=capi.STOP_REF_R_OVER_X)
+    assert np.array_equal(x3, ctx.solution())
+
+
+@pytest.mark.parametrize("kind", ["quad16", "hex6", "hex5_rollers"])
+def test_pcg_fp32_refined(hip_ctx_factory, kind):
+    prob = _problem(kind)
+    print("case", kind, flush=True)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    eq = O.assemble_problem(prob)
+    st, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-12, stop_rule=1)
+    s = ctx.solve(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR, inner_tol=1e-3)
+    x_g = ctx.solution()
+    assert s["rel_resid"] <= 1e-8
+    assert s["outer_iters"] >= 1
+    assert np.linalg.norm(x_g - x_o) <= 1e-6 * np.linalg.norm(x_o)
+
+
+def test_pcg_solve_mirror(hip_ctx_factory):
+    """LargeMatrix::solve(b, x0): solution returned in b; x0 honoured."""
+    prob = _problem("hex6")
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    b = ctx.rhs()
+    x, s = ctx.pcg_solve(b, tol=1e-10)
+    assert s["status"] == 0
+    x2, s2 = ctx.pcg_solve(b, x0=x, tol=1e-10)
+    assert s2["iters"] <= 1
+
+
+def test_heat_transient_assembly(hip_ctx_factory):
+    prob = meshgen.diffusion_cube(5)
+    print("case heat5", flush=True)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    u_s = prob.extra["u0"]
+    alpha, dt = 1.0, 1e-3
+    ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[alpha, dt], u_state=u_s)
+    eq = O.assemble_heat(prob, alpha, dt, u_s)
+    rp, ci = eq.pattern()
+    assert np.array_equal(ctx.pattern_lower()[1], ci)
+    assert _row_scaled_err(rp, ci, ctx.values_lower(), eq.values()) <= 1e-12
+    f_o = eq.pseudo()
+    f_g = ctx.pseudo_load()
+    assert np.abs(f_g - f_o).max() <= 1e-12 * np.abs(f_o).max()
+
+
+def test_error_paths(hip_ctx_factory):
+    # inverted element -> BHIP_ERR_NEG_DETJ (geometryCheck, ElasticityElement3D.cpp:653-670)
+    prob = meshgen.elastic_cube(2)
+    xyz = prob.xyz.copy()
+    xyz[:, 2] = -xyz[:, 2]
+    prob.xyz = xyz
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    with pytest.raises(capi.BhipError) as ei:
+        ctx.assemble()
+    assert ei.value.status == capi.ERR_NEG_DETJ
+    # iteration cap -> BHIP_ERR_NOT_CONVERGED (sparse.cpp:225-229)
+    prob = meshgen.elastic_cube(4)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    with pytest.raises(capi.BhipError) as ei:
+        ctx.solve(tol=1e-12, maxit=1)
+    assert ei.value.status == capi.ERR_NOT_CONVERGED
+    # a node without elements has a zero diagonal -> BHIP_ERR_ZERO_DIAG
+    xyz = np.vstack([prob.xyz, [[2.0, 2.0, 2.0]]])
+    ctx = hip_ctx_factory()
+    ctx.set_mesh(3, xyz, prob.conn)
+    r = np.concatenate([prob.restrained, [0, 0, 0]])
+    ctx.set_restraints(r)
+    ctx.build_pattern()
+    ctx.set_materials(prob.mat_kind, prob.mat)
+    ctx.assemble()
+    ctx.set_load(None)
+    ctx.build_rhs()
+    with pytest.raises(capi.BhipError) as ei:
+        ctx.solve(tol=1e-8)
+    assert ei.value.status == capi.ERR_ZERO_DIAG
+    # load on a restrained DOF (addLoadToEqn, equation.cpp:862)
+    with pytest.raises(capi.BhipError):
+        ctx.set_load(r.astype(float))
