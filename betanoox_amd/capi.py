@@ -48,7 +48,8 @@ EXPORTS = [
     "bhip_get_csr", "bhip_set_materials", "bhip_set_integration", "bhip_assemble",
     "bhip_get_pseudo_load", "bhip_set_load", "bhip_build_rhs", "bhip_get_rhs", "bhip_set_rhs",
     "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv",
-    "bhip_bench_spmv",
+    "bhip_bench_spmv", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
+    "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
 ]
 
 
@@ -62,6 +63,14 @@ class Stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer_iters", C.c_int), ("ref_norm", C.c_double),
                 ("rel_resid", C.c_double), ("ms_total", C.c_double), ("ms_spmv_avg", C.c_double),
                 ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PlasticStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int), ("pcg_iters", C.c_int), ("max_residual", C.c_double),
+                ("plastic_points", C.c_int), ("max_return_iters", C.c_int), ("ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -119,6 +128,13 @@ def load_library(path=LIB_PATH):
         "bhip_pcg_solve": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_spmv": (C.c_int, [vp, _P_D, _P_D]),
         "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),
+        "bhip_plastic_init": (C.c_int, [vp]),
+        "bhip_plastic_update": (C.c_int, [vp, _P_D]),
+        "bhip_plastic_residual": (C.c_int, [vp, _P_D, _P_D]),
+        "bhip_plastic_commit": (C.c_int, [vp]),
+        "bhip_plastic_increment": (C.c_int, [vp, _P_D, C.c_double, C.c_int, C.POINTER(SolveOpts),
+                                             C.POINTER(PlasticStats)]),
+        "bhip_plastic_get_state": (C.c_int, [vp, _P_D, _P_D, _P_D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -307,6 +323,39 @@ class Context:
         y = np.zeros(self.n_dof)
         self._chk(self.lib.bhip_spmv(self.h, _d(_f64(x)), _d(y)))
         return y
+
+    # ---- plasticity
+    def plastic_init(self):
+        self._chk(self.lib.bhip_plastic_init(self.h))
+
+    def plastic_update(self, du):
+        self._chk(self.lib.bhip_plastic_update(self.h, _d(_f64(du))))
+
+    def plastic_residual(self):
+        r = np.zeros(self.n_dof)
+        m = C.c_double()
+        self._chk(self.lib.bhip_plastic_residual(self.h, _d(r), C.byref(m)))
+        return r, m.value
+
+    def plastic_commit(self):
+        self._chk(self.lib.bhip_plastic_commit(self.h))
+
+    def plastic_increment(self, presc_increment, max_allowable_residual, max_iterations, opts=None, **kw):
+        o = opts if opts is not None else self.opts(**kw)
+        s = PlasticStats()
+        p = None if presc_increment is None else _f64(presc_increment)
+        st = self._chk(self.lib.bhip_plastic_increment(self.h, _d(p), max_allowable_residual, max_iterations,
+                                                        C.byref(o), C.byref(s)), (ERR_NOT_CONVERGED,))
+        d = s.as_dict()
+        d["status"] = st
+        return d
+
+    def plastic_state(self, n_el, nip):
+        s = np.zeros(n_el * nip * 6)
+        e = np.zeros(n_el * nip)
+        u = np.zeros(self.n_dof)
+        self._chk(self.lib.bhip_plastic_get_state(self.h, _d(s), _d(e), _d(u)))
+        return s.reshape(n_el, nip, 6), e.reshape(n_el, nip), u
 
     def bench_spmv(self, fp32=False, reps=20):
         ms, by = C.c_double(), C.c_double()

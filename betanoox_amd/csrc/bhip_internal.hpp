@@ -127,6 +127,8 @@ struct bhip_ctx {
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
     int graph64_iters = 0, graph32_iters = 0;
     double spmv_bytes64 = 0, spmv_bytes32 = 0;
+    // ---- J2 plasticity state (plastic.hip)
+    struct PlasticData *plastic = nullptr;
 };
 
 // -------------------------------------------------------------- launching

@@ -1,6 +1,523 @@
-// plastic.hip -- J2 plasticity state on the device (filled in with the
-// return-map kernels; until then the tangent assembly reports BHIP_ERR_STATE).
-#include "bhip_internal.hpp"
+// plastic.hip -- small-strain J2 plasticity on the device.
+//
+// Reference: PlasticityModel::DoPlasticAnalysis (PlasticityModel.cpp:142-398)
+// with PlasticElement::update (plasticElement.cpp:39-250) actions
+//   "V": strain increment at every Gauss point -> PlasticMaterial::updateISV
+//        -> getStressIncrement (plastic.cpp:368-465, the active "version 2.0")
+//   "KI": tangent stiffness from getTangentStiffness (plastic.cpp:583-610)
+//   "F": internal force of the stress increment (addToFe with isv.dStress)
+//   "U": GaussISV::updateISV (plastic.cpp:769-785)
+// and the Newton loop of one load increment (PlasticityModel.cpp:213-366,
+// Equations::incrementalSolutionUsingResidual, equation.cpp:1638-1720).
+//
+// J2 is Hill with F=G=H=1/3, L=M=N=1; linear isotropic hardening is the
+// POWER master curve with m_E = H, m_n = 1, m_sigma0 = sigma0.  The return
+// map, yield function and tangent run in FP64 at every Gauss point.
+//
+// Per Gauss point record (16 doubles): stress[6], eps_bar, dstress[6],
+// deps_bar, plastic flag, return-map iterations.  Tangent descriptor
+// (8 doubles, read by the assembly kernel): sp = C nf [6], 1/denom, flag, so
+// that C_ep = C - sp sp^T / denom.
+#include <algorithm>
+#include <cmath>
 
-const double *bhip_plastic_tangent(bhip_ctx *ctx) { (void)ctx; return nullptr; }
-void bhip_plastic_release(bhip_ctx *ctx) { (void)ctx; }
+#include "bhip_internal.hpp"
+#include "gauss_tables.hpp"
+
+// This translation unit's Gauss tables (gauss_tables.hpp).
+__constant__ double c_w[kMaxIp];
+__constant__ double c_dN[kMaxIp * 24];
+
+struct PlasticData {
+    int nip = 0;
+    dvec<double> isv;     // n_el * nip * 16
+    dvec<double> tan;     // n_el * nip * 8
+    dvec<double> du;      // increment displacement (currentSolution), n_dof
+    dvec<double> utot;    // accumulated displacement, n_dof
+    dvec<double> corr;    // Newton correction, n_dof
+    dvec<double> red;     // reductions
+    dvec<unsigned> cnt;
+};
+
+namespace {
+
+constexpr int kIsv = 16;
+
+struct J2 {
+    double E, nu, s0, H;
+    double C[36];
+    __device__ void init(const double *p) {
+        E = p[0];
+        nu = p[1];
+        s0 = p[2];
+        H = p[3];
+        double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+        for (int i = 0; i < 36; i++) C[i] = 0;
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) C[i * 6 + j] = lam;
+            C[i * 6 + i] = lam + 2 * mu;
+            C[(i + 3) * 6 + (i + 3)] = mu;
+        }
+    }
+};
+
+// HillFunction (plastic.cpp:172-194) with originState = yStress = 0.
+__device__ __forceinline__ double hill(const double *s0, const double *ds) {
+    double s[6];
+    for (int i = 0; i < 6; i++) s[i] = s0[i] + ds[i];
+    const double F = 1.0 / 3.0, G = 1.0 / 3.0, H = 1.0 / 3.0;
+    return 0.5 * (F * (s[1] - s[2]) * (s[1] - s[2]) + G * (s[2] - s[0]) * (s[2] - s[0]) +
+                  H * (s[0] - s[1]) * (s[0] - s[1]) + 2 * s[4] * s[4] + 2 * s[5] * s[5] + 2 * s[3] * s[3]);
+}
+
+// dYield (plastic.cpp:716-735)
+__device__ __forceinline__ void dyield(const double *s0, const double *ds, double *nf) {
+    double s[6];
+    for (int i = 0; i < 6; i++) s[i] = s0[i] + ds[i];
+    const double F = 1.0 / 3.0, G = 1.0 / 3.0, H = 1.0 / 3.0;
+    nf[0] = -G * (s[2] - s[0]) + H * (s[0] - s[1]);
+    nf[1] = F * (s[1] - s[2]) - H * (s[0] - s[1]);
+    nf[2] = -F * (s[1] - s[2]) + G * (s[2] - s[0]);
+    nf[3] = 2 * s[3];
+    nf[4] = 2 * s[4];
+    nf[5] = 2 * s[5];
+}
+
+// POWER master curve with m_n = 1: m_Stress = H*ep + s0, m_dsdep = H.
+__device__ __forceinline__ double yield_stress(const J2 &m, double ep) {
+    double y = m.H * ep + m.s0;
+    return y > m.s0 ? y : m.s0;
+}
+
+__device__ __forceinline__ double yield_fn(const J2 &m, const double *s0, const double *ds, double ep) {
+    double sy = yield_stress(m, ep);
+    return hill(s0, ds) / (sy * sy / 3.0) - 1.0;
+}
+
+__device__ __forceinline__ double hard_mod(const J2 &m, const double *s0, const double *ds) {
+    double mm2 = 3.0 * hill(s0, ds);  // effective stress squared
+    return 4.0 / 9.0 * mm2 * m.H;
+}
+
+__device__ __forceinline__ void cmul(const double *C, const double *v, double *o) {
+    for (int i = 0; i < 6; i++) {
+        double s = 0;
+        for (int j = 0; j < 6; j++) s += C[i * 6 + j] * v[j];
+        o[i] = s;
+    }
+}
+
+__device__ __forceinline__ double dot6(const double *a, const double *b) {
+    double s = 0;
+    for (int i = 0; i < 6; i++) s += a[i] * b[i];
+    return s;
+}
+
+// All 8 global gradients at ip (isoElement.cpp:324-372).
+__device__ __forceinline__ double hex8_all_grads(const double *x, const double *y, const double *z, int ip,
+                                                 double (*g)[3]) {
+    const double *dxi = c_dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
+    double px1 = 0, px2 = 0, px3 = 0, py1 = 0, py2 = 0, py3 = 0, pz1 = 0, pz2 = 0, pz3 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        px1 += dxi[i] * x[i];
+        px2 += deta[i] * x[i];
+        px3 += dzeta[i] * x[i];
+        py1 += dxi[i] * y[i];
+        py2 += deta[i] * y[i];
+        py3 += dzeta[i] * y[i];
+        pz1 += dxi[i] * z[i];
+        pz2 += deta[i] * z[i];
+        pz3 += dzeta[i] * z[i];
+    }
+    double d1 = py2 * pz3 - py3 * pz2, d2 = px2 * pz3 - px3 * pz2, d3 = px2 * py3 - px3 * py2;
+    double d4 = py1 * pz3 - py3 * pz1, d5 = px1 * pz3 - px3 * pz1, d6 = px1 * py3 - px3 * py1;
+    double d7 = py1 * pz2 - py2 * pz1, d8 = px1 * pz2 - px2 * pz1, d9 = px1 * py2 - px2 * py1;
+    double detJ = px1 * d1 - py1 * d2 + pz1 * d3;
+    double inv = 1. / detJ;
+    double a11 = d1 * inv, a12 = -d4 * inv, a13 = d7 * inv;
+    double a21 = -d2 * inv, a22 = d5 * inv, a23 = -d8 * inv;
+    double a31 = d3 * inv, a32 = -d6 * inv, a33 = d9 * inv;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        g[i][0] = a11 * dxi[i] + a12 * deta[i] + a13 * dzeta[i];
+        g[i][1] = a21 * dxi[i] + a22 * deta[i] + a23 * dzeta[i];
+        g[i][2] = a31 * dxi[i] + a32 * deta[i] + a33 * dzeta[i];
+    }
+    return detJ;
+}
+
+__device__ __forceinline__ void load_coords(const double *xyz, const int *ec, double *x, double *y, double *z) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const double *q = xyz + 3 * (long long)ec[i];
+        x[i] = q[0];
+        y[i] = q[1];
+        z[i] = q[2];
+    }
+}
+
+__device__ __forceinline__ int group_of(const int *elem_group, int n_groups, int n_el, int e) {
+    if (elem_group) return elem_group[e];
+    return n_groups == n_el ? e : 0;
+}
+
+// "V": strain increment + return map (getStressIncrement, plastic.cpp:368-465).
+__global__ void k_plastic_update(int n_el, int nip, const int *conn, const double *xyz, const double *mat,
+                                 const int *elem_group, int n_groups, const double *du, double *isv,
+                                 double *tan) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n_el * nip) return;
+    int e = (int)(t / nip), ip = (int)(t - (long long)e * nip);
+    int ec[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) ec[i] = conn[(long long)e * 8 + i];
+    double x[8], y[8], z[8];
+    load_coords(xyz, ec, x, y, z);
+    double g[8][3];
+    hex8_all_grads(x, y, z, ip, g);
+    // ElasticityElement3D::calculateStrain (engineering shear)
+    double de[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        const double u = du[3 * (long long)ec[a]], v = du[3 * (long long)ec[a] + 1],
+                     w = du[3 * (long long)ec[a] + 2];
+        de[0] += g[a][0] * u;
+        de[1] += g[a][1] * v;
+        de[2] += g[a][2] * w;
+        de[3] += g[a][1] * u + g[a][0] * v;
+        de[4] += g[a][2] * v + g[a][1] * w;
+        de[5] += g[a][2] * u + g[a][0] * w;
+    }
+    J2 m;
+    m.init(mat + 4LL * group_of(elem_group, n_groups, n_el, e));
+    double *rec = isv + t * kIsv;
+    double s0[6], ds[6], dse[6], nf[6], sp[6];
+    for (int i = 0; i < 6; i++) s0[i] = rec[i];
+    const double ep = rec[6];
+    cmul(m.C, de, dse);  // elastic predictor
+    for (int i = 0; i < 6; i++) ds[i] = dse[i];
+    double dep = 0.0;
+    int iter = 0, plastic = 0;
+    double inv_denom = 0.0;
+    if (yield_fn(m, s0, ds, ep + dep) > 0.0) {
+        plastic = 1;
+        double f;
+        do {
+            double hard = hard_mod(m, s0, ds);
+            dyield(s0, ds, nf);
+            cmul(m.C, nf, sp);
+            double dl = dot6(nf, dse) / (hard + dot6(nf, sp));
+            for (int i = 0; i < 6; i++) ds[i] = dse[i] - dl * sp[i];
+            dep = 2.0 * dl * sqrt(3.0 * hill(s0, ds)) / 3.0;
+            f = yield_fn(m, s0, ds, ep + dep);
+            iter++;
+        } while (f > 1e-6 && iter < 5000);
+        dyield(s0, ds, nf);
+        cmul(m.C, nf, sp);
+        inv_denom = 1.0 / (hard_mod(m, s0, ds) + dot6(nf, sp));
+    }
+    for (int i = 0; i < 6; i++) rec[7 + i] = ds[i];
+    rec[13] = dep;
+    rec[14] = plastic;
+    rec[15] = iter;
+    double *tr = tan + t * 8;
+    for (int i = 0; i < 6; i++) tr[i] = plastic ? sp[i] : 0.0;
+    tr[6] = inv_denom;
+    tr[7] = plastic;
+}
+
+// "F" + calculateResidualNorm: per node, resultant of B^T dstress over the
+// incident elements (ascending), residual = load - resultant on free DOFs.
+__global__ void k_plastic_residual(int n_nodes, int nip, const int *n2e_ptr, const int *n2e, const int *conn,
+                                   const double *xyz, const double *isv, const unsigned char *rmask,
+                                   const double *load, double *resid) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    double acc[3] = {0, 0, 0};
+    for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
+        int e = n2e[k];
+        int ec[8], a = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            ec[i] = conn[(long long)e * 8 + i];
+            if (ec[i] == n) a = i;
+        }
+        double x[8], y[8], z[8];
+        load_coords(xyz, ec, x, y, z);
+        for (int ip = 0; ip < nip; ip++) {
+            double g[8][3];
+            double detJ = hex8_all_grads(x, y, z, ip, g);
+            double f = detJ * c_w[ip];
+            const double *s = isv + ((long long)e * nip + ip) * kIsv + 7;
+            // ElasticityElement3D::addToFe: force += sum_j b[j][i] stress[j] * f
+            acc[0] += (g[a][0] * s[0] + g[a][1] * s[3] + g[a][2] * s[5]) * f;
+            acc[1] += (g[a][1] * s[1] + g[a][0] * s[3] + g[a][2] * s[4]) * f;
+            acc[2] += (g[a][2] * s[2] + g[a][1] * s[4] + g[a][0] * s[5]) * f;
+        }
+    }
+    unsigned rm = rmask[n];
+    for (int c = 0; c < 3; c++) {
+        long long I = 3LL * n + c;
+        resid[I] = ((rm >> c) & 1u) ? 0.0 : load[I] - acc[c];
+    }
+}
+
+// max |v| and count of plastic points / max return iterations (block partial
+// maxima combined by the last block).
+__global__ void k_max_abs(long long n, const double *v, double *red, unsigned *cnt, double *out) {
+    __shared__ double sm[256];
+    __shared__ bool last;
+    double m = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(v[i]));
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        red[blockIdx.x] = sm[0];
+        __threadfence();
+        last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double r = 0;
+        for (int b = 0; b < (int)gridDim.x; b++) r = fmax(r, red[b]);
+        *out = r;
+        *cnt = 0;
+    }
+}
+
+// currentSolution at the start of an increment: prescribed values on
+// restrained DOFs, zero elsewhere (equations.zeroSolution + loads applyTo).
+__global__ void k_du_start(int n, const unsigned char *rest, const double *presc, double *du) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) du[i] = rest[i] ? presc[i] : 0.0;
+}
+
+// addIncrementalDisplacements (equation.cpp:1707-1720): free DOFs only.
+__global__ void k_add_free(int n, const unsigned char *rest, const double *d, double *du) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !rest[i]) du[i] += d[i];
+}
+
+__global__ void k_add(int n, const double *a, double *b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] += a[i];
+}
+
+// "U": GaussISV::updateISV
+__global__ void k_plastic_commit(long long npts, double *isv) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    double *r = isv + t * kIsv;
+    for (int i = 0; i < 6; i++) {
+        r[i] += r[7 + i];
+        r[7 + i] = 0.0;
+    }
+    r[6] += r[13];
+    r[13] = 0.0;
+}
+
+int plastic_check(bhip_ctx *ctx) {
+    if (!ctx->plastic) return bhip_fail(ctx, BHIP_ERR_STATE, "plastic state not initialised (bhip_plastic_init)");
+    std::vector<double> hw, hdN, hN;
+    int nip = bhip_gauss::tables(8, ctx->order, kMaxIp, hw, hdN, hN);
+    if (nip != ctx->plastic->nip) return bhip_fail(ctx, BHIP_ERR_STATE, "integration order changed after bhip_plastic_init");
+    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_w), hw.data(), sizeof(double) * kMaxIp, 0,
+                                         hipMemcpyHostToDevice, ctx->stream));
+    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dN), hdN.data(), sizeof(double) * kMaxIp * 24, 0,
+                                         hipMemcpyHostToDevice, ctx->stream));
+    return BHIP_OK;
+}
+
+int plastic_update_dev(bhip_ctx *ctx, const double *du) {
+    PlasticData *P = ctx->plastic;
+    long long npts = (long long)ctx->n_el * P->nip;
+    k_plastic_update<<<grid_for(npts, 128), 128, 0, ctx->stream>>>(
+        ctx->n_el, P->nip, ctx->conn.p, ctx->xyz.p, ctx->mat.p, ctx->elem_group.n ? ctx->elem_group.p : nullptr,
+        ctx->n_groups, du, P->isv.p, P->tan.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    return BHIP_OK;
+}
+
+int plastic_residual_dev(bhip_ctx *ctx, double *max_abs) {
+    PlasticData *P = ctx->plastic;
+    k_plastic_residual<<<grid_for(ctx->n_nodes, 128), 128, 0, ctx->stream>>>(
+        ctx->n_nodes, P->nip, ctx->n2e_ptr.p, ctx->n2e.p, ctx->conn.p, ctx->xyz.p, P->isv.p, ctx->node_rmask.p,
+        ctx->load.p, ctx->rhs.p);
+    int g = std::min(1024, grid_for(ctx->n_dof, 256));
+    k_max_abs<<<g, 256, 0, ctx->stream>>>(ctx->n_dof, ctx->rhs.p, P->red.p, P->cnt.p, P->red.p + 1024);
+    BHIP_TRY(ctx, hipGetLastError());
+    BHIP_TRY(ctx, hipMemcpyAsync(max_abs, P->red.p + 1024, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BHIP_OK;
+}
+
+}  // namespace
+
+const double *bhip_plastic_tangent(bhip_ctx *ctx) { return ctx->plastic ? ctx->plastic->tan.p : nullptr; }
+
+void bhip_plastic_release(bhip_ctx *ctx) {
+    if (!ctx->plastic) return;
+    PlasticData *P = ctx->plastic;
+    P->isv.release();
+    P->tan.release();
+    P->du.release();
+    P->utot.release();
+    P->corr.release();
+    P->red.release();
+    P->cnt.release();
+    delete P;
+    ctx->plastic = nullptr;
+}
+
+extern "C" {
+
+int bhip_plastic_init(bhip_ctx *ctx) {
+    if (!ctx) return BHIP_ERR_ARG;
+    if (!ctx->pattern_ready || !ctx->have_mat) return bhip_fail(ctx, BHIP_ERR_STATE, "pattern and materials first");
+    if (!(ctx->nnpe == 8 && ctx->dim == 3 && ctx->mat_kind == BHIP_MAT_J2 && ctx->uniform_ndof &&
+          ctx->max_ndof == 3))
+        return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "plasticity needs hex8, 3 DOF/node and a J2 material");
+    (void)hipSetDevice(ctx->device);
+    bhip_plastic_release(ctx);
+    PlasticData *P = new PlasticData();
+    ctx->plastic = P;
+    P->nip = ctx->order * ctx->order * ctx->order;
+    long long npts = (long long)ctx->n_el * P->nip;
+    BHIP_TRY(ctx, P->isv.resize(npts * kIsv));
+    BHIP_TRY(ctx, P->tan.resize(npts * 8));
+    BHIP_TRY(ctx, P->du.resize(ctx->n_dof));
+    BHIP_TRY(ctx, P->utot.resize(ctx->n_dof));
+    BHIP_TRY(ctx, P->corr.resize(ctx->n_dof));
+    BHIP_TRY(ctx, P->red.resize(1025));
+    BHIP_TRY(ctx, P->cnt.resize(1));
+    BHIP_TRY(ctx, hipMemsetAsync(P->isv.p, 0, P->isv.bytes(), ctx->stream));
+    BHIP_TRY(ctx, hipMemsetAsync(P->tan.p, 0, P->tan.bytes(), ctx->stream));
+    BHIP_TRY(ctx, hipMemsetAsync(P->du.p, 0, P->du.bytes(), ctx->stream));
+    BHIP_TRY(ctx, hipMemsetAsync(P->utot.p, 0, P->utot.bytes(), ctx->stream));
+    BHIP_TRY(ctx, hipMemsetAsync(P->cnt.p, 0, sizeof(unsigned), ctx->stream));
+    BHIP_PROPAGATE(bhip_ensure_vectors(ctx));
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BHIP_OK;
+}
+
+int bhip_plastic_update(bhip_ctx *ctx, const double *du) {
+    if (!ctx || !du) return BHIP_ERR_ARG;
+    BHIP_PROPAGATE(plastic_check(ctx));
+    (void)hipSetDevice(ctx->device);
+    PlasticData *P = ctx->plastic;
+    BHIP_TRY(ctx, hipMemcpyAsync(P->du.p, du, sizeof(double) * ctx->n_dof, hipMemcpyHostToDevice, ctx->stream));
+    BHIP_PROPAGATE(plastic_update_dev(ctx, P->du.p));
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BHIP_OK;
+}
+
+int bhip_plastic_residual(bhip_ctx *ctx, double *residual, double *max_abs) {
+    if (!ctx || !max_abs) return BHIP_ERR_ARG;
+    BHIP_PROPAGATE(plastic_check(ctx));
+    (void)hipSetDevice(ctx->device);
+    BHIP_PROPAGATE(plastic_residual_dev(ctx, max_abs));
+    if (residual) BHIP_TRY(ctx, hipMemcpy(residual, ctx->rhs.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    return BHIP_OK;
+}
+
+int bhip_plastic_commit(bhip_ctx *ctx) {
+    if (!ctx) return BHIP_ERR_ARG;
+    BHIP_PROPAGATE(plastic_check(ctx));
+    PlasticData *P = ctx->plastic;
+    long long npts = (long long)ctx->n_el * P->nip;
+    k_plastic_commit<<<grid_for(npts, 256), 256, 0, ctx->stream>>>(npts, P->isv.p);
+    k_add<<<grid_for(ctx->n_dof, 256), 256, 0, ctx->stream>>>(ctx->n_dof, P->du.p, P->utot.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BHIP_OK;
+}
+
+int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double max_allowable_residual,
+                           int max_iterations, const bhip_solve_opts *opts, bhip_plastic_stats *stats) {
+    if (!ctx || !opts) return BHIP_ERR_ARG;
+    BHIP_PROPAGATE(plastic_check(ctx));
+    (void)hipSetDevice(ctx->device);
+    PlasticData *P = ctx->plastic;
+    hipStream_t s = ctx->stream;
+    const int n = ctx->n_dof;
+    bhip_plastic_stats S;
+    memset(&S, 0, sizeof(S));
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    if (presc_increment)
+        BHIP_TRY(ctx, hipMemcpyAsync(ctx->presc.p, presc_increment, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    // currentSolution = prescribed increment on restrained DOFs
+    k_du_start<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->rest.p, ctx->presc.p, P->du.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    bhip_solve_opts o = *opts;
+    o.use_x0 = 0;
+    double maxres = 0;
+    int it = 0;
+    for (;;) {
+        // "V" then assembleKandF ("KI") then getGlobalForces ("F")
+        BHIP_PROPAGATE(plastic_update_dev(ctx, P->du.p));
+        BHIP_PROPAGATE(bhip_assemble_impl(ctx, BHIP_ASM_PLASTIC_TANGENT, nullptr, nullptr));
+        BHIP_PROPAGATE(plastic_residual_dev(ctx, &maxres));
+        if (it > 0 && maxres < 2.0 * max_allowable_residual) break;
+        // one solve before the loop plus max_iterations inside it (PlasticityModel.cpp:262-287)
+        if (it > max_iterations) break;
+        // K d = residual (rhs already holds it), then du += d on free DOFs
+        bhip_stats st;
+        int rc = bhip_solve_impl(ctx, &o, &st);
+        if (rc != BHIP_OK) return rc;
+        S.pcg_iters += st.iters;
+        k_add_free<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->rest.p, ctx->x.p, P->du.p);
+        BHIP_TRY(ctx, hipGetLastError());
+        it++;
+    }
+    S.newton_iters = it;
+    S.max_residual = maxres;
+    BHIP_PROPAGATE(bhip_plastic_commit(ctx));
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    S.ms = ms;
+    // plastic points and return-map iterations of the converged state
+    long long npts = (long long)ctx->n_el * P->nip;
+    std::vector<double> flags(npts);
+    {
+        std::vector<double> rec(npts * kIsv);
+        BHIP_TRY(ctx, hipMemcpy(rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
+        for (long long t = 0; t < npts; t++) {
+            S.plastic_points += rec[t * kIsv + 14] != 0.0;
+            S.max_return_iters = std::max(S.max_return_iters, (int)rec[t * kIsv + 15]);
+        }
+    }
+    if (stats) *stats = S;
+    if (!(maxres < 2.0 * max_allowable_residual))
+        return bhip_fail(ctx, BHIP_ERR_NOT_CONVERGED, "Newton iterations exhausted in the load increment");
+    return BHIP_OK;
+}
+
+int bhip_plastic_get_state(bhip_ctx *ctx, double *stress, double *eps_bar, double *u_total) {
+    if (!ctx) return BHIP_ERR_ARG;
+    BHIP_PROPAGATE(plastic_check(ctx));
+    PlasticData *P = ctx->plastic;
+    long long npts = (long long)ctx->n_el * P->nip;
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (stress || eps_bar) {
+        std::vector<double> rec(npts * kIsv);
+        BHIP_TRY(ctx, hipMemcpy(rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
+        for (long long t = 0; t < npts; t++) {
+            if (stress)
+                for (int i = 0; i < 6; i++) stress[t * 6 + i] = rec[t * kIsv + i];
+            if (eps_bar) eps_bar[t] = rec[t * kIsv + 6];
+        }
+    }
+    if (u_total) BHIP_TRY(ctx, hipMemcpy(u_total, P->utot.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    return BHIP_OK;
+}
+
+}  // extern "C"

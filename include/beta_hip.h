@@ -168,6 +168,43 @@ int bhip_spmv(bhip_ctx *ctx, const double *x, double *y);
  * returns the mean kernel time in ms (HIP events). */
 int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes);
 
+/* ------------------------------------------------------------ plasticity */
+typedef struct bhip_plastic_stats {
+    int    newton_iters;     /* linear solves in the increment */
+    int    pcg_iters;        /* PCG iterations over those solves */
+    double max_residual;     /* final max |load - resultant| over free DOFs */
+    int    plastic_points;   /* Gauss points in the plastic region */
+    int    max_return_iters; /* largest return-map iteration count */
+    double ms;               /* device time of the increment */
+} bhip_plastic_stats;
+
+/* GaussISV allocation (createISVSpace, PlasticityModel.cpp:121-129): zero
+ * stresses and effective plastic strain at every Gauss point; needs a J2
+ * material, hex8 and the pattern. */
+int bhip_plastic_init(bhip_ctx *ctx);
+/* update("V") on every element (plasticElement.cpp:177-180): strain
+ * increment from du (n_dof, increment displacement) and the FP64 return map
+ * (PlasticMaterial::getStressIncrement, plastic.cpp:368-465). */
+int bhip_plastic_update(bhip_ctx *ctx, const double *du);
+/* getGlobalForces + calculateResidualNorm (BasicModel.cpp:1063-1110,
+ * equation.cpp:1516-1529): residual = load - resultant of the stress
+ * increments on free DOFs, 0 on restrained DOFs; max |residual|. */
+int bhip_plastic_residual(bhip_ctx *ctx, double *residual, double *max_abs);
+/* update("U") (GaussISV::updateISV, plastic.cpp:769-785) and accumulation of
+ * the increment displacement into the total. */
+int bhip_plastic_commit(bhip_ctx *ctx);
+/* One load increment of PlasticityModel::DoPlasticAnalysis
+ * (PlasticityModel.cpp:213-366) on the resident system: prescribed values
+ * (n_dof, increment; NULL keeps the current ones), the applied load increment
+ * from bhip_set_load, Newton with the tangent reassembled every iteration
+ * until max|residual| < 2*max_allowable_residual or max_iterations extra
+ * solves, then update("U"). */
+int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double max_allowable_residual,
+                           int max_iterations, const bhip_solve_opts *opts, bhip_plastic_stats *stats);
+/* Converged state: stress (n_el*nip*6), effective plastic strain (n_el*nip),
+ * accumulated displacement (n_dof); any pointer may be NULL. */
+int bhip_plastic_get_state(bhip_ctx *ctx, double *stress, double *eps_bar, double *u_total);
+
 #ifdef __cplusplus
 }
 #endif
