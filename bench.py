@@ -135,7 +135,7 @@ def main():
     achieved = by32 / ms32 / 1e6
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_bsr3_spmv<16,1> (FP32 3x3-block SpMV fused with p.Ap)",
+                "kernel": "k_sell3_spmv<1> (FP32 3x3-block sliced-ELL SpMV with p.Ap partials)",
                 "bytes_per_launch": by32, "ms_per_launch": ms32}
 
     line = {

@@ -105,7 +105,12 @@ struct bhip_ctx {
     bool f32_ready = false;
     bool use_bsr = false;
     dvec<float> val32;       // CSR FP32 values (non-BSR case)
-    dvec<float> bval32;      // BSR3 FP32 values, SoA: 9 x n_pairs
+    // 3x3-block sliced ELL (SELL-64): slice s holds 64 consecutive nodes and
+    // w_s block columns; entry (k, lane) of slice s is at sell_ptr[s] + 64k + lane
+    int n_slices = 0;
+    long long n_sell = 0;    // padded block count
+    dvec<int> sell_ptr, sell_col;
+    dvec<float> bval32;      // SELL block values, SoA: 9 x n_sell
     // ---- materials / integration
     int mat_kind = 0, n_groups = 0, mat_stride = 0;
     dvec<double> mat;

@@ -77,7 +77,7 @@ void bhip_destroy(bhip_ctx *ctx) {
     for (auto *v : v64) v->release();
     dvec<int> *vi[] = {&ctx->conn,     &ctx->dof_first, &ctx->ndof,   &ctx->n2e_ptr, &ctx->n2e,
                        &ctx->nbr_ptr,  &ctx->nbr,       &ctx->pair_row, &ctx->nbr_off, &ctx->rowptr,
-                       &ctx->colind,   &ctx->diag_pos,  &ctx->elem_group, &ctx->flags};
+                       &ctx->colind,   &ctx->diag_pos,  &ctx->elem_group, &ctx->flags, &ctx->sell_ptr, &ctx->sell_col};
     for (auto *v : vi) v->release();
     dvec<float> *v32[] = {&ctx->val32, &ctx->bval32, &ctx->r32, &ctx->p32, &ctx->q32, &ctx->d32, &ctx->dinv32};
     for (auto *v : v32) v->release();

@@ -300,6 +300,8 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     ctx->f32_ready = false;
     ctx->val32.release();
     ctx->bval32.release();
+    ctx->sell_ptr.release();
+    ctx->sell_col.release();
     ctx->val_aux.release();
     return BHIP_OK;
 }

@@ -2,61 +2,55 @@
 //
 // Reference: SymmetricSparseMatrix::solve (sparse.cpp:114-242) and
 // vectorProduct (452-478); the fused-loop layout follows ParCG
-// (ParCG.cpp:216-300).  Per iteration the device runs three kernels:
-//   1. q = K p fused with p.q  (last block: alpha = rz / pAp)
-//   2. x += alpha p, r -= alpha q, z = r/diag fused with r.z, r.r, x.x
-//      (last block: beta, stopping rule, iteration count)
-//   3. p = z + beta p
-// All reductions are block partials summed by the last block to finish, in
-// block order, so results do not depend on scheduling.  Kernels read the
-// solver scalars from device memory and return at once when the solve is
-// done, so the host enqueues iterations in chunks and polls between chunks.
+// (ParCG.cpp:216-300).  One PCG iteration is five kernels:
+//   1. q = K p, block partials of p.q
+//   2. finalize (one block): pAp, alpha = rz / pAp
+//   3. x += alpha p, r -= alpha q, z = r/diag, block partials of r.z, r.r, x.x
+//   4. finalize (one block): beta, stopping rule, iteration count
+//   5. p = z + beta p
+// Reductions are deterministic: blocks store partials, a single-block kernel
+// sums them in block order.  (A fence-and-counter "last block reduces"
+// scheme costs an L2 write-back per block on gfx950 and halves the SpMV
+// bandwidth.)  Kernels read the solver scalars from device memory and return
+// at once when the solve is done, so iterations are enqueued in chunks
+// (replayed as a captured HIP graph) and the host polls between chunks.
 //
-// Storage: full symmetric CSR (int32 column indices).  The FP32 inner solve
-// of the refined solver uses FP32 values, as 3x3 node blocks (BSR-3, values
-// in structure-of-arrays order) when every node carries 3 DOFs.
+// Storage: full symmetric CSR (int32 column indices) for the FP64 solve and
+// the FP64 outer residual; the FP32 inner solve of the refined solver uses
+// FP32 values in 3x3 node blocks laid out as sliced ELL (SELL-64, one lane
+// per node row) when every node carries 3 DOFs, and FP32 CSR otherwise.
+#include <cstdlib>
+
 #include "bhip_internal.hpp"
 
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kFinThreads = 1024;
 
-// ------------------------------------------------------------- reduction
-// Publish NV block partials; the last block to arrive sums all partials in
-// block order and returns true (in every thread of that block) with the
-// totals in tot[] of thread 0.
+// ------------------------------------------------------------- partials
 template <int NV>
-__device__ bool block_publish(double (&v)[NV], double *partials, unsigned *counter, double *tot) {
-    __shared__ double sm[16 * 8];
-    __shared__ bool is_last;
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *partials) {
+    __shared__ double sm[16 * 4];
     block_sum<NV>(v, sm);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; k++) partials[(long long)blockIdx.x * NV + k] = v[k];
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double w[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) w[k] = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; k++) w[k] += partials[(long long)b * NV + k];
-    __syncthreads();
-    block_sum<NV>(w, sm);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) tot[k] = w[k];
-        *counter = 0;
-    }
-    return true;
 }
 
-// Contiguous row range of logical block lb out of nb over n rows.
+// One block of kFinThreads: fixed-order sum of nb partial records.
+template <int NV>
+__device__ __forceinline__ void fin_sum(const double *partials, int nb, double (&tot)[NV]) {
+    __shared__ double sm[16 * 4];
+#pragma unroll
+    for (int k = 0; k < NV; k++) tot[k] = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++) tot[k] += partials[(long long)b * NV + k];
+    block_sum<NV>(tot, sm);
+}
+
+// Contiguous range of logical block lb out of nb over n items.
 __device__ __forceinline__ void block_range(long long n, int lb, int nb, long long &b, long long &e) {
     long long per = (n + nb - 1) / nb;
     b = (long long)lb * per;
@@ -66,118 +60,103 @@ __device__ __forceinline__ void block_range(long long n, int lb, int nb, long lo
 }
 
 // ---------------------------------------------------------------- SpMV CSR
-// mode 0: y = A x.   mode 1 (PCG): y = A x, pAp = x.y -> alpha.
-// mode 2 (residual): y = b - A x, accumulate |y|^2 and |b|^2.
+// One pass: block (logical, XCD-contiguous) lb owns rows [lb*R, lb*R + R),
+// R = kThreads / LPR, LPR lanes per row.
+// MODE 0: y = A x.  MODE 1 (PCG): y = A x, partial x.y.
+// MODE 2 (residual): y = b - A x, partials |y|^2 and |b|^2.
 template <int LPR, class VT, class XT, int MODE>
 __global__ void __launch_bounds__(kThreads) k_csr_spmv(int n, const int *__restrict__ rowptr,
                                                      const int *__restrict__ colind,
                                                      const VT *__restrict__ val,
                                                      const XT *__restrict__ x, XT *__restrict__ y,
-                                                     const double *__restrict__ b, PcgState *st,
+                                                     const double *__restrict__ b, const PcgState *st,
                                                      double *partials) {
     if (MODE == 1 && st->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    long long r0, r1;
-    block_range(n, lb, gridDim.x, r0, r1);
     const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
-    const int rows_per_pass = kThreads / LPR;
-    double acc0 = 0, acc1 = 0;
-    for (long long base = r0; base < r1; base += rows_per_pass) {
-        long long row = base + g;
-        double s = 0;
-        if (row < r1) {
-            int kb = rowptr[row], ke = rowptr[row + 1];
-            for (int k = kb + l; k < ke; k += LPR) s += (double)val[k] * (double)x[colind[k]];
-        }
+    const long long row = (long long)lb * (kThreads / LPR) + g;
+    double s = 0;
+    if (row < n) {
+        const int kb = rowptr[row], ke = rowptr[row + 1];
+#pragma unroll 4
+        for (int k = kb + l; k < ke; k += LPR) s += (double)val[k] * (double)x[colind[k]];
+    }
 #pragma unroll
-        for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-        if (row < r1 && l == 0) {
-            if (MODE == 2) {
-                double bi = b[row];
-                double ri = bi - s;
-                y[row] = (XT)ri;
-                acc0 += ri * ri;
-                acc1 += bi * bi;
-            } else {
-                y[row] = (XT)s;
-                if (MODE == 1) acc0 += (double)x[row] * s;
-            }
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+    double a0 = 0, a1 = 0;
+    if (row < n && l == 0) {
+        if (MODE == 2) {
+            double bi = b[row];
+            double ri = bi - s;
+            y[row] = (XT)ri;
+            a0 = ri * ri;
+            a1 = bi * bi;
+        } else {
+            y[row] = (XT)s;
+            if (MODE == 1) a0 = (double)x[row] * s;
         }
     }
-    if (MODE == 0) return;
-    double v[2] = {acc0, acc1};
-    double tot[2];
-    if (block_publish<2>(v, partials, &st->cnt[0], tot) && threadIdx.x == 0) {
-        if (MODE == 1) {
-            st->pAp = tot[0];
-            st->alpha = st->rz / tot[0];
-        } else {
-            st->rr = tot[0];
-            st->bb = tot[1];
-        }
+    if (MODE == 1) {
+        double v[1] = {a0};
+        store_partials<1>(v, partials);
+    } else if (MODE == 2) {
+        double v[2] = {a0, a1};
+        store_partials<2>(v, partials);
     }
 }
 
-// ------------------------------------------------------------- SpMV BSR-3
-// Node block rows from the node adjacency; values SoA: bval[e*nnzb + blk].
-template <int LPB, int MODE>
-__global__ void __launch_bounds__(kThreads) k_bsr3_spmv(int n_nodes, const int *__restrict__ bptr,
-                                                      const int *__restrict__ bcol,
-                                                      const float *__restrict__ bval, long long nnzb,
-                                                      const float *__restrict__ x,
-                                                      float *__restrict__ y, PcgState *st,
-                                                      double *partials) {
+// ------------------------------------------------------- SpMV SELL-64 BSR-3
+// Lane l of slice s owns node 64 s + l and walks the slice's block columns;
+// block values (one array per entry of the 3x3 block) and column indices are
+// laid out so that the 64 lanes of a wave read 64 consecutive entries and
+// consecutive nodes gather neighbouring x entries.  No cross-lane reduction;
+// padded blocks carry zeros and the node's own column.  4 slices per block.
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) k_sell3_spmv(int n_nodes, int n_slices,
+                                                       const int *__restrict__ sptr,
+                                                       const int *__restrict__ scol,
+                                                       const float *__restrict__ sval, long long nsell,
+                                                       const float *__restrict__ x,
+                                                       float *__restrict__ y, const PcgState *st,
+                                                       double *partials) {
     if (MODE == 1 && st->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    long long r0, r1;
-    block_range(n_nodes, lb, gridDim.x, r0, r1);
-    const int g = threadIdx.x / LPB, l = threadIdx.x % LPB;
-    const int rows_per_pass = kThreads / LPB;
+    const int lane = threadIdx.x & 63;
+    const long long s = (long long)lb * (kThreads / 64) + (threadIdx.x >> 6);
     double acc = 0;
-    for (long long base = r0; base < r1; base += rows_per_pass) {
-        long long row = base + g;
-        float s0 = 0, s1 = 0, s2 = 0;
-        if (row < r1) {
-            int kb = bptr[row], ke = bptr[row + 1];
-            for (int k = kb + l; k < ke; k += LPB) {
-                int m = bcol[k];
-                float x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
-                s0 += bval[0 * nnzb + k] * x0 + bval[1 * nnzb + k] * x1 + bval[2 * nnzb + k] * x2;
-                s1 += bval[3 * nnzb + k] * x0 + bval[4 * nnzb + k] * x1 + bval[5 * nnzb + k] * x2;
-                s2 += bval[6 * nnzb + k] * x0 + bval[7 * nnzb + k] * x1 + bval[8 * nnzb + k] * x2;
-            }
+    if (s < n_slices) {
+        const long long node = s * 64 + lane;
+        const int kb = sptr[s], ke = sptr[s + 1];
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 4
+        for (int k = kb + lane; k < ke; k += 64) {
+            const int m = scol[k];
+            const float x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
+            a0 += sval[0 * nsell + k] * x0 + sval[1 * nsell + k] * x1 + sval[2 * nsell + k] * x2;
+            a1 += sval[3 * nsell + k] * x0 + sval[4 * nsell + k] * x1 + sval[5 * nsell + k] * x2;
+            a2 += sval[6 * nsell + k] * x0 + sval[7 * nsell + k] * x1 + sval[8 * nsell + k] * x2;
         }
-#pragma unroll
-        for (int off = LPB / 2; off > 0; off >>= 1) {
-            s0 += __shfl_xor(s0, off, LPB);
-            s1 += __shfl_xor(s1, off, LPB);
-            s2 += __shfl_xor(s2, off, LPB);
-        }
-        if (row < r1 && l == 0) {
-            y[3 * row] = s0;
-            y[3 * row + 1] = s1;
-            y[3 * row + 2] = s2;
+        if (node < n_nodes) {
+            y[3 * node] = a0;
+            y[3 * node + 1] = a1;
+            y[3 * node + 2] = a2;
             if (MODE == 1)
-                acc += (double)x[3 * row] * s0 + (double)x[3 * row + 1] * s1 + (double)x[3 * row + 2] * s2;
+                acc = (double)x[3 * node] * a0 + (double)x[3 * node + 1] * a1 + (double)x[3 * node + 2] * a2;
         }
     }
-    if (MODE == 0) return;
-    double v[1] = {acc};
-    double tot[1];
-    if (block_publish<1>(v, partials, &st->cnt[0], tot) && threadIdx.x == 0) {
-        st->pAp = tot[0];
-        st->alpha = st->rz / tot[0];
+    if (MODE == 1) {
+        double v[1] = {acc};
+        store_partials<1>(v, partials);
     }
 }
 
 // --------------------------------------------------------- vector kernels
-// x += alpha p; r -= alpha q; z = r*dinv; rz = r.z, rr = r.r, xx = x.x.
-// Last block: beta, stopping rule (sparse.cpp:202-223), iteration count.
+// x += alpha p; r -= alpha q; z = r*dinv; partials r.z, r.r, x.x.
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_pcg_update(long long n, T *__restrict__ x,
                                                        T *__restrict__ r, const T *__restrict__ p,
                                                        const T *__restrict__ q,
-                                                       const T *__restrict__ dinv, PcgState *st,
+                                                       const T *__restrict__ dinv, const PcgState *st,
                                                        double *partials) {
     if (st->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -196,30 +175,7 @@ __global__ void __launch_bounds__(kThreads) k_pcg_update(long long n, T *__restr
         xx += (double)xi * (double)xi;
     }
     double v[3] = {rz, rr, xx};
-    double tot[3];
-    if (block_publish<3>(v, partials, &st->cnt[1], tot) && threadIdx.x == 0) {
-        double rz_new = tot[0];
-        st->beta = rz_new / st->rz;
-        st->rz = rz_new;
-        st->rr = tot[1];
-        st->xx = tot[2];
-        st->iter += 1;
-        double nrm;
-        bool conv;
-        if (st->stop_rule == BHIP_STOP_REF_R_OVER_X) {
-            nrm = sqrt(tot[1] / tot[2]);
-            conv = nrm < st->tol;
-        } else if (st->stop_rule == BHIP_STOP_REL_RESID) {
-            nrm = sqrt(tot[1] / st->bb);
-            conv = nrm <= st->tol;
-        } else {  // inner solve: relative reduction of |r|
-            nrm = sqrt(tot[1] / st->rr0);
-            conv = nrm <= st->tol;
-        }
-        st->norm = nrm;
-        if (conv) st->converged = 1;
-        if (conv || st->iter >= st->maxit) st->done = 1;
-    }
+    store_partials<3>(v, partials);
 }
 
 // p = r*dinv + beta p
@@ -236,12 +192,11 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate(long long n, T *__rest
     for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = r[i] * dinv[i] + beta * p[i];
 }
 
-// Initial: z = r*dinv, p = z, rz = r.z, rr = r.r, xx = x.x; state reset.
+// Initial: z = r*dinv, p = z; partials r.z, r.r, x.x.
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_pcg_init(long long n, const T *__restrict__ x,
-                                                     const T *__restrict__ r,
-                                                     T *__restrict__ p, const T *__restrict__ dinv,
-                                                     PcgState *st, double *partials) {
+                                                     const T *__restrict__ r, T *__restrict__ p,
+                                                     const T *__restrict__ dinv, double *partials) {
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     long long i0, i1;
     block_range(n, lb, gridDim.x, i0, i1);
@@ -254,35 +209,95 @@ __global__ void __launch_bounds__(kThreads) k_pcg_init(long long n, const T *__r
         xx += (double)x[i] * (double)x[i];
     }
     double v[3] = {rz, rr, xx};
-    double tot[3];
-    if (block_publish<3>(v, partials, &st->cnt[2], tot) && threadIdx.x == 0) {
-        st->rz = tot[0];
-        st->rr = tot[1];
-        st->rr0 = tot[1];
-        st->xx = tot[2];
+    store_partials<3>(v, partials);
+}
+
+// ------------------------------------------------------ finalize kernels
+__global__ void __launch_bounds__(kFinThreads) k_fin_alpha(int nb, const double *partials, PcgState *st) {
+    if (st->done) return;
+    double t[1];
+    fin_sum<1>(partials, nb, t);
+    if (threadIdx.x == 0) {
+        st->pAp = t[0];
+        st->alpha = st->rz / t[0];
+    }
+}
+
+// beta, stopping rule (sparse.cpp:202-223), iteration count.
+__global__ void __launch_bounds__(kFinThreads) k_fin_update(int nb, const double *partials, PcgState *st) {
+    if (st->done) return;
+    double t[3];
+    fin_sum<3>(partials, nb, t);
+    if (threadIdx.x == 0) {
+        double rz_new = t[0];
+        st->beta = rz_new / st->rz;
+        st->rz = rz_new;
+        st->rr = t[1];
+        st->xx = t[2];
+        st->iter += 1;
+        double nrm;
+        bool conv;
+        if (st->stop_rule == BHIP_STOP_REF_R_OVER_X) {
+            nrm = sqrt(t[1] / t[2]);
+            conv = nrm < st->tol;
+        } else if (st->stop_rule == BHIP_STOP_REL_RESID) {
+            nrm = sqrt(t[1] / st->bb);
+            conv = nrm <= st->tol;
+        } else {  // inner solve: relative reduction of |r|
+            nrm = sqrt(t[1] / st->rr0);
+            conv = nrm <= st->tol;
+        }
+        st->norm = nrm;
+        if (conv) st->converged = 1;
+        if (conv || st->iter >= st->maxit) st->done = 1;
+    }
+}
+
+__global__ void __launch_bounds__(kFinThreads) k_fin_init(int nb, const double *partials, PcgState *st) {
+    double t[3];
+    fin_sum<3>(partials, nb, t);
+    if (threadIdx.x == 0) {
+        st->rz = t[0];
+        st->rr = t[1];
+        st->rr0 = t[1];
+        st->xx = t[2];
         st->iter = 0;
-        st->converged = 0;
         // a zero initial residual is already the solution (the reference would
         // divide 0/0 here and run to its iteration cap)
-        st->done = (tot[1] == 0.0) ? 1 : 0;
-        if (tot[1] == 0.0) st->converged = 1;
+        st->converged = (t[1] == 0.0) ? 1 : 0;
+        st->done = st->converged;
         st->norm = 0;
     }
 }
 
+__global__ void __launch_bounds__(kFinThreads) k_fin_resid(int nb, const double *partials, PcgState *st) {
+    double t[2];
+    fin_sum<2>(partials, nb, t);
+    if (threadIdx.x == 0) {
+        st->rr = t[0];
+        st->bb = t[1];
+    }
+}
+
+__global__ void k_state_init(PcgState *st, double tol, int maxit, int stop_rule) {
+    st->tol = tol;
+    st->maxit = maxit;
+    st->stop_rule = stop_rule;
+    st->done = 0;
+    st->iter = 0;
+    st->converged = 0;
+    st->cnt[0] = st->cnt[1] = st->cnt[2] = st->cnt[3] = 0;
+}
+
+// ---------------------------------------------------- conversion kernels
 // dinv = 1/diag; flags: bit0 zero diagonal, bit1 negative diagonal
 // (CheckAndFixZeroDiagonal, largeMatrix.cpp:43-79).
-__global__ void k_dinv(int n, const int *diag_pos, const double *val, double *dinv, float *dinv32,
-                       double repl, int *flags) {
+__global__ void k_dinv(int n, const int *diag_pos, const double *val, double *dinv, float *dinv32, int *flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double d = val[diag_pos[i]];
-    if (d == 0.0) {
-        atomicOr(flags, 1);
-        d = repl;
-    } else if (d < 0.0) {
-        atomicOr(flags, 2);
-    }
+    if (d == 0.0) atomicOr(flags, 1);
+    else if (d < 0.0) atomicOr(flags, 2);
     dinv[i] = 1.0 / d;
     if (dinv32) dinv32[i] = (float)(1.0 / d);
 }
@@ -292,13 +307,39 @@ __global__ void k_to_f32(long long n, const double *a, float *b) {
     if (i < n) b[i] = (float)a[i];
 }
 
-// FP64 CSR -> FP32 BSR-3 (SoA), restrained entries zero, restrained diagonal 1.
-__global__ void k_csr_to_bsr3(long long n_pairs, const int *pair_row, const int *nbr,
-                              const int *nbr_off, const int *rowptr, const unsigned char *rmask,
-                              const double *val, float *bval) {
+// Slice widths: w_s = max block-row length over the slice's 64 nodes.
+__global__ void k_sell_width(int n_nodes, int n_slices, const int *nbr_ptr, int *width) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slices) return;
+    int w = 0;
+    for (int l = 0; l < 64; l++) {
+        long long n = (long long)s * 64 + l;
+        if (n < n_nodes) w = max(w, nbr_ptr[n + 1] - nbr_ptr[n]);
+    }
+    width[s] = w * 64;
+}
+
+// Padding: own column (zero values are set by a fill beforehand).
+__global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, int *scol) {
+    long long s = blockIdx.x;
+    if (s >= n_slices) return;
+    for (int k = sptr[s] + threadIdx.x; k < sptr[s + 1]; k += blockDim.x) {
+        long long node = s * 64 + (k - sptr[s]) % 64;
+        scol[k] = node < n_nodes ? (int)node : 0;
+    }
+}
+
+// FP64 CSR -> FP32 SELL-64 3x3 blocks (SoA); restrained entries zero,
+// restrained diagonal 1.  Pair p = (node n, k-th neighbour m).
+__global__ void k_csr_to_sell3(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
+                               const int *nbr_off, const int *rowptr, const unsigned char *rmask,
+                               const double *val, const int *sptr, long long nsell, int *scol, float *sval) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     int n = pair_row[p], m = nbr[p];
+    int k = (int)(p - nbr_ptr[n]);
+    long long dst = (long long)sptr[n / 64] + 64LL * k + (n % 64);
+    scol[dst] = m;
     unsigned rn = rmask[n], rm = rmask[m];
     for (int c = 0; c < 3; c++) {
         int r = 0;
@@ -309,12 +350,12 @@ __global__ void k_csr_to_bsr3(long long n_pairs, const int *pair_row, const int 
             if (!rc && !rd) v = (float)val[base + r];
             else if (rc && n == m && c == d) v = 1.f;
             if (!rd) r++;
-            bval[(long long)(c * 3 + d) * n_pairs + p] = v;
+            sval[(long long)(c * 3 + d) * nsell + dst] = v;
         }
     }
 }
 
-// r32 = (float)(r * s)
+// r32 = (float)(r * s), d32 = 0
 __global__ void k_scale_to_f32(long long n, const double *r, double s, float *r32, float *d32) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -329,19 +370,9 @@ __global__ void k_axpy_f32(long long n, double s, const float *d32, double *x) {
     if (i < n) x[i] += s * (double)d32[i];
 }
 
-__global__ void k_state_init(PcgState *st, double tol, int maxit, int stop_rule) {
-    st->tol = tol;
-    st->maxit = maxit;
-    st->stop_rule = stop_rule;
-    st->done = 0;
-    st->iter = 0;
-    st->converged = 0;
-    st->cnt[0] = st->cnt[1] = st->cnt[2] = st->cnt[3] = 0;
-}
-
 // ---------------------------------------------------------------- helpers
-int pick_grid(bhip_ctx *ctx, long long work_items, int per_thread) {
-    long long want = (work_items + (long long)kThreads * per_thread - 1) / ((long long)kThreads * per_thread);
+int vec_grid(bhip_ctx *ctx, long long n) {
+    long long want = (n + (long long)kThreads * 8 - 1) / ((long long)kThreads * 8);
     long long cap = (long long)ctx->n_cu * 8;
     if (cap > kRedBlocks) cap = kRedBlocks;
     if (want > cap) want = cap;
@@ -349,69 +380,85 @@ int pick_grid(bhip_ctx *ctx, long long work_items, int per_thread) {
     return (int)want;
 }
 
-// Lanes per row for CSR: the power of two nearest the mean row length / 4.
+// Lanes per row for CSR from the mean row length (measured on gfx950: 8 lanes
+// for the ~81-entry rows of 3D hex8 elasticity).
 int pick_lpr(long long nnz, long long n) {
     double avg = n > 0 ? (double)nnz / (double)n : 1.0;
-    if (avg > 96) return 32;
-    if (avg > 40) return 16;
+    if (avg > 160) return 16;
     if (avg > 12) return 8;
     if (avg > 4) return 4;
     return 2;
 }
 
+int csr_grid(bhip_ctx *ctx) {
+    int lpr = pick_lpr(ctx->nnz, ctx->n_dof);
+    long long rpb = kThreads / lpr;
+    long long g = (ctx->n_dof + rpb - 1) / rpb;
+    return (int)(g > 0 ? g : 1);
+}
+
+int sell_grid(bhip_ctx *ctx) {
+    long long g = (ctx->n_slices + kThreads / 64 - 1) / (kThreads / 64);
+    return (int)(g > 0 ? g : 1);
+}
+
 template <class VT, class XT, int MODE>
-void launch_csr(bhip_ctx *ctx, int lpr, int grid, const VT *val, const XT *x, XT *y, const double *b,
-                PcgState *st) {
+void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *b, const PcgState *st) {
     hipStream_t s = ctx->stream;
     const int n = ctx->n_dof;
+    const int grid = csr_grid(ctx);
 #define BHIP_CSR_CASE(L)                                                                           \
     case L:                                                                                        \
         k_csr_spmv<L, VT, XT, MODE><<<grid, kThreads, 0, s>>>(n, ctx->rowptr.p, ctx->colind.p, val, \
                                                               x, y, b, st, ctx->partials.p);       \
         break;
-    switch (lpr) {
+    switch (pick_lpr(ctx->nnz, ctx->n_dof)) {
         BHIP_CSR_CASE(2)
         BHIP_CSR_CASE(4)
         BHIP_CSR_CASE(8)
         BHIP_CSR_CASE(16)
-        BHIP_CSR_CASE(32)
     }
 #undef BHIP_CSR_CASE
 }
 
-int spmv_grid(bhip_ctx *ctx, bool bsr) {
-    if (bsr) return pick_grid(ctx, (long long)ctx->n_nodes * 16, 8);
-    int lpr = pick_lpr(ctx->nnz, ctx->n_dof);
-    return pick_grid(ctx, (long long)ctx->n_dof * lpr, 8);
-}
-
-void launch_bsr(bhip_ctx *ctx, int grid, int mode, const float *x, float *y, PcgState *st) {
+void launch_sell(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgState *st) {
     hipStream_t s = ctx->stream;
     if (mode == 1)
-        k_bsr3_spmv<16, 1><<<grid, kThreads, 0, s>>>(ctx->n_nodes, ctx->nbr_ptr.p, ctx->nbr.p,
-                                                      ctx->bval32.p, ctx->n_pairs, x, y, st,
-                                                      ctx->partials.p);
+        k_sell3_spmv<1><<<sell_grid(ctx), kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
+                                                            ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
+                                                            ctx->partials.p);
     else
-        k_bsr3_spmv<16, 0><<<grid, kThreads, 0, s>>>(ctx->n_nodes, ctx->nbr_ptr.p, ctx->nbr.p,
-                                                      ctx->bval32.p, ctx->n_pairs, x, y, st,
-                                                      ctx->partials.p);
+        k_sell3_spmv<0><<<sell_grid(ctx), kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
+                                                            ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
+                                                            ctx->partials.p);
 }
 
-// Algorithmic bytes of one SpMV (values + indices + row pointers + x + y).
+// Algorithmic bytes of one SpMV (values + indices + row pointers + x + y);
+// for the blocked form the stored (padded) blocks are counted.
 double spmv_bytes(bhip_ctx *ctx, bool fp32, bool bsr) {
-    if (bsr) {
-        double nb = (double)ctx->n_pairs;
-        return nb * (9 * 4 + 4) + (ctx->n_nodes + 1.0) * 4 + 2.0 * ctx->n_dof * 4;
-    }
+    if (bsr) return (double)ctx->n_sell * (9 * 4 + 4) + (ctx->n_slices + 1.0) * 4 + 2.0 * ctx->n_dof * 4;
     double sv = fp32 ? 4 : 8;
     return (double)ctx->nnz * (sv + 4) + (ctx->n_dof + 1.0) * 4 + 2.0 * ctx->n_dof * sv;
+}
+
+// Partials large enough for every kernel of the solve (3 per block for the
+// vector kernels, 2 per block for the CSR residual).
+int ensure_partials(bhip_ctx *ctx) {
+    size_t need = (size_t)kRedBlocks * 3;
+    need = std::max(need, (size_t)csr_grid(ctx) * 2);
+    if (ctx->n_slices > 0) need = std::max(need, (size_t)sell_grid(ctx));
+    if (ctx->partials.n < need) {
+        bhip_invalidate_graphs(ctx);
+        BHIP_TRY(ctx, ctx->partials.resize(need));
+    }
+    return BHIP_OK;
 }
 
 int prepare_diag(bhip_ctx *ctx) {
     const int B = 256;
     BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 4, ctx->stream));
-    k_dinv<<<grid_for(ctx->n_dof, B), B, 0, ctx->stream>>>(ctx->n_dof, ctx->diag_pos.p, ctx->val.p,
-                                                           ctx->dinv.p, ctx->dinv32.p, 0.0, ctx->flags.p);
+    k_dinv<<<grid_for(ctx->n_dof, B), B, 0, ctx->stream>>>(ctx->n_dof, ctx->diag_pos.p, ctx->val.p, ctx->dinv.p,
+                                                           ctx->dinv32.p, ctx->flags.p);
     BHIP_TRY(ctx, hipGetLastError());
     int hf = 0;
     BHIP_TRY(ctx, hipMemcpyAsync(&hf, ctx->flags.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -421,66 +468,166 @@ int prepare_diag(bhip_ctx *ctx) {
     return BHIP_OK;
 }
 
+int build_sell(bhip_ctx *ctx) {
+    hipStream_t s = ctx->stream;
+    ctx->n_slices = (ctx->n_nodes + 63) / 64;
+    const int ns = ctx->n_slices;
+    dvec<int> width;
+    BHIP_TRY(ctx, width.resize(ns + 1));
+    BHIP_TRY(ctx, hipMemsetAsync(width.p, 0, sizeof(int) * (ns + 1), s));
+    k_sell_width<<<grid_for(ns, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, width.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    std::vector<int> hw(ns + 1);
+    BHIP_TRY(ctx, hipMemcpyAsync(hw.data(), width.p, sizeof(int) * (ns + 1), hipMemcpyDeviceToHost, s));
+    BHIP_TRY(ctx, hipStreamSynchronize(s));
+    std::vector<int> hp(ns + 1);
+    long long tot = 0;
+    for (int i = 0; i < ns; i++) {
+        hp[i] = (int)tot;
+        tot += hw[i];
+    }
+    if (tot >= (1LL << 31)) return bhip_fail(ctx, BHIP_ERR_CAPACITY, "blocked matrix too large for int32 offsets");
+    hp[ns] = (int)tot;
+    ctx->n_sell = tot;
+    bhip_invalidate_graphs(ctx);
+    BHIP_TRY(ctx, ctx->sell_ptr.resize(ns + 1));
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->sell_ptr.p, hp.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
+    BHIP_TRY(ctx, ctx->sell_col.resize(tot > 0 ? tot : 1));
+    BHIP_TRY(ctx, ctx->bval32.resize((size_t)9 * (tot > 0 ? tot : 1)));
+    k_sell_pad<<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
+    BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
+    BHIP_TRY(ctx, hipGetLastError());
+    return BHIP_OK;
+}
+
 int prepare_f32(bhip_ctx *ctx) {
     if (ctx->f32_ready) return BHIP_OK;
     const int B = 256;
     ctx->use_bsr = (ctx->uniform_ndof && ctx->max_ndof == 3 && ctx->n_dof == 3 * ctx->n_nodes);
     if (ctx->use_bsr) {
-        BHIP_TRY(ctx, ctx->bval32.resize((size_t)9 * (ctx->n_pairs > 0 ? ctx->n_pairs : 1)));
-        k_csr_to_bsr3<<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(
-            ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_off.p, ctx->rowptr.p,
-            ctx->node_rmask.p, ctx->val.p, ctx->bval32.p);
-        ctx->val32.release();
+        if (ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_col.p)
+            BHIP_PROPAGATE(build_sell(ctx));
+        k_csr_to_sell3<<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(
+            ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,
+            ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col.p, ctx->bval32.p);
+        if (ctx->val32.p) {
+            bhip_invalidate_graphs(ctx);
+            ctx->val32.release();
+        }
     } else {
-        BHIP_TRY(ctx, ctx->val32.resize(ctx->nnz > 0 ? ctx->nnz : 1));
+        if (ctx->val32.n != (size_t)(ctx->nnz > 0 ? ctx->nnz : 1)) {
+            bhip_invalidate_graphs(ctx);
+            BHIP_TRY(ctx, ctx->val32.resize(ctx->nnz > 0 ? ctx->nnz : 1));
+        }
         k_to_f32<<<grid_for(ctx->nnz, B), B, 0, ctx->stream>>>(ctx->nnz, ctx->val.p, ctx->val32.p);
-        ctx->bval32.release();
     }
     BHIP_TRY(ctx, hipGetLastError());
     ctx->f32_ready = true;
     return BHIP_OK;
 }
 
-// Run PCG iterations already initialised in *st until done; returns
-// iterations.  `fp32` selects the inner FP32 vectors and matrix.
+// Enqueue one PCG iteration (five kernels).
+void enqueue_iteration(bhip_ctx *ctx, bool fp32) {
+    hipStream_t s = ctx->stream;
+    PcgState *st = ctx->state.p;
+    const long long n = ctx->n_dof;
+    const int gv = vec_grid(ctx, n);
+    if (!fp32) {
+        launch_csr<double, double, 1>(ctx, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
+        k_fin_alpha<<<1, kFinThreads, 0, s>>>(csr_grid(ctx), ctx->partials.p, st);
+        k_pcg_update<double><<<gv, kThreads, 0, s>>>(n, ctx->x.p, ctx->r.p, ctx->p.p, ctx->q.p, ctx->dinv.p, st,
+                                                     ctx->partials.p);
+        k_fin_update<<<1, kFinThreads, 0, s>>>(gv, ctx->partials.p, st);
+        k_pcg_pupdate<double><<<gv, kThreads, 0, s>>>(n, ctx->p.p, ctx->r.p, ctx->dinv.p, st);
+    } else {
+        int gs;
+        if (ctx->use_bsr) {
+            launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
+            gs = sell_grid(ctx);
+        } else {
+            launch_csr<float, float, 1>(ctx, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
+            gs = csr_grid(ctx);
+        }
+        k_fin_alpha<<<1, kFinThreads, 0, s>>>(gs, ctx->partials.p, st);
+        k_pcg_update<float><<<gv, kThreads, 0, s>>>(n, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->q32.p,
+                                                    ctx->dinv32.p, st, ctx->partials.p);
+        k_fin_update<<<1, kFinThreads, 0, s>>>(gv, ctx->partials.p, st);
+        k_pcg_pupdate<float><<<gv, kThreads, 0, s>>>(n, ctx->p32.p, ctx->r32.p, ctx->dinv32.p, st);
+    }
+}
+
+bool graphs_enabled() {
+    const char *e = getenv("BHIP_NO_GRAPH");
+    return !(e && e[0] == '1');
+}
+
+// A captured chunk of `chunk` iterations (kernel nodes only).
+int get_graph(bhip_ctx *ctx, bool fp32, int chunk, hipGraphExec_t *out) {
+    hipGraphExec_t &g = fp32 ? ctx->graph32 : ctx->graph64;
+    int &gi = fp32 ? ctx->graph32_iters : ctx->graph64_iters;
+    if (g && gi == chunk) {
+        *out = g;
+        return BHIP_OK;
+    }
+    if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    hipGraph_t graph;
+    BHIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk; k++) enqueue_iteration(ctx, fp32);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    BHIP_TRY(ctx, e);
+    e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BHIP_TRY(ctx, e);
+    gi = chunk;
+    *out = g;
+    return BHIP_OK;
+}
+
+// Run PCG iterations already initialised in the device state until done.
 int run_iterations(bhip_ctx *ctx, bool fp32, int check_every, int *spmv_launches) {
     hipStream_t s = ctx->stream;
     PcgState *st = ctx->state.p;
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipStreamSynchronize(s));
     if (ctx->h_state->done) return BHIP_OK;
-    const long long n = ctx->n_dof;
-    const int gv = pick_grid(ctx, n, 8);
-    const bool bsr = fp32 && ctx->use_bsr;
-    const int gs = spmv_grid(ctx, bsr);
-    const int lpr = pick_lpr(ctx->nnz, ctx->n_dof);
+    const bool use_graph = graphs_enabled();
+    hipGraphExec_t g = nullptr;
+    if (use_graph) BHIP_PROPAGATE(get_graph(ctx, fp32, check_every, &g));
     // every chunk advances at least one iteration unless the device marked the
     // solve done, so maxit/check_every + 2 chunks always suffice
     const long long max_chunks = (long long)ctx->h_state->maxit / check_every + 2;
     for (long long chunk = 0;; chunk++) {
         if (chunk > max_chunks)
             return bhip_fail(ctx, BHIP_ERR_STATE, "PCG state machine did not finish (internal error)");
-        for (int k = 0; k < check_every; k++) {
-            if (!fp32) {
-                launch_csr<double, double, 1>(ctx, lpr, gs, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
-                k_pcg_update<double><<<gv, kThreads, 0, s>>>(n, ctx->x.p, ctx->r.p, ctx->p.p, ctx->q.p,
-                                                             ctx->dinv.p, st, ctx->partials.p);
-                k_pcg_pupdate<double><<<gv, kThreads, 0, s>>>(n, ctx->p.p, ctx->r.p, ctx->dinv.p, st);
-            } else {
-                if (bsr) launch_bsr(ctx, gs, 1, ctx->p32.p, ctx->q32.p, st);
-                else launch_csr<float, float, 1>(ctx, lpr, gs, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
-                k_pcg_update<float><<<gv, kThreads, 0, s>>>(n, ctx->d32.p, ctx->r32.p, ctx->p32.p,
-                                                            ctx->q32.p, ctx->dinv32.p, st, ctx->partials.p);
-                k_pcg_pupdate<float><<<gv, kThreads, 0, s>>>(n, ctx->p32.p, ctx->r32.p, ctx->dinv32.p, st);
-            }
-            if (spmv_launches) (*spmv_launches)++;
+        if (use_graph) {
+            BHIP_TRY(ctx, hipGraphLaunch(g, s));
+        } else {
+            for (int k = 0; k < check_every; k++) enqueue_iteration(ctx, fp32);
         }
         BHIP_TRY(ctx, hipGetLastError());
+        if (spmv_launches) *spmv_launches += check_every;
         BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         BHIP_TRY(ctx, hipStreamSynchronize(s));
         if (ctx->h_state->done) break;
     }
     return BHIP_OK;
+}
+
+// r = b - K x in FP64 with |r|^2, |b|^2 into the state (rr, bb).
+void enqueue_residual(bhip_ctx *ctx, double *r) {
+    PcgState *st = ctx->state.p;
+    launch_csr<double, double, 2>(ctx, ctx->val.p, ctx->x.p, r, ctx->rhs.p, st);
+    k_fin_resid<<<1, kFinThreads, 0, ctx->stream>>>(csr_grid(ctx), ctx->partials.p, st);
+}
+
+template <class T>
+void enqueue_init(bhip_ctx *ctx, const T *x, const T *r, T *p, const T *dinv) {
+    const int gv = vec_grid(ctx, ctx->n_dof);
+    k_pcg_init<T><<<gv, kThreads, 0, ctx->stream>>>(ctx->n_dof, x, r, p, dinv, ctx->partials.p);
+    k_fin_init<<<1, kFinThreads, 0, ctx->stream>>>(gv, ctx->partials.p, ctx->state.p);
 }
 
 }  // namespace
@@ -489,11 +636,11 @@ void bhip_invalidate_graphs(bhip_ctx *ctx) {
     if (ctx->graph64) (void)hipGraphExecDestroy(ctx->graph64);
     if (ctx->graph32) (void)hipGraphExecDestroy(ctx->graph32);
     ctx->graph64 = ctx->graph32 = nullptr;
+    ctx->graph64_iters = ctx->graph32_iters = 0;
 }
 
 int bhip_spmv_dev(bhip_ctx *ctx, const double *x, double *y) {
-    int lpr = pick_lpr(ctx->nnz, ctx->n_dof);
-    launch_csr<double, double, 0>(ctx, lpr, spmv_grid(ctx, false), ctx->val.p, x, y, nullptr, nullptr);
+    launch_csr<double, double, 0>(ctx, ctx->val.p, x, y, nullptr, nullptr);
     BHIP_TRY(ctx, hipGetLastError());
     return BHIP_OK;
 }
@@ -513,21 +660,18 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
 
     BHIP_PROPAGATE(prepare_diag(ctx));
     if (opts.precision == BHIP_PREC_FP32_IR) BHIP_PROPAGATE(prepare_f32(ctx));
-    const int lpr = pick_lpr(ctx->nnz, ctx->n_dof);
-    const int gs64 = spmv_grid(ctx, false);
-    const int gv = pick_grid(ctx, n, 8);
+    BHIP_PROPAGATE(ensure_partials(ctx));
 
     BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    // x = x0 or 0
     if (opts.use_x0) BHIP_TRY(ctx, hipMemcpyAsync(ctx->x.p, ctx->x0.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     else BHIP_TRY(ctx, hipMemsetAsync(ctx->x.p, 0, sizeof(double) * n, s));
 
     int st_code = BHIP_OK;
     if (opts.precision == BHIP_PREC_FP64) {
-        // r = b - K x0  (sparse.cpp:152-163), then the PCG loop.
+        // r = b - K x0 (sparse.cpp:152-163), z = p = r/diag, then the PCG loop
         k_state_init<<<1, 1, 0, s>>>(st, opts.tol, opts.maxit, opts.stop_rule);
-        launch_csr<double, double, 2>(ctx, lpr, gs64, ctx->val.p, ctx->x.p, ctx->r.p, ctx->rhs.p, st);
-        k_pcg_init<double><<<gv, kThreads, 0, s>>>(n, ctx->x.p, ctx->r.p, ctx->p.p, ctx->dinv.p, st, ctx->partials.p);
+        enqueue_residual(ctx, ctx->r.p);
+        enqueue_init<double>(ctx, ctx->x.p, ctx->r.p, ctx->p.p, ctx->dinv.p);
         BHIP_TRY(ctx, hipGetLastError());
         BHIP_PROPAGATE(run_iterations(ctx, false, opts.check_every, &S.spmv_launches));
         S.iters = ctx->h_state->iter;
@@ -546,7 +690,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
         for (outer = 0; outer <= max_outer; outer++) {
             // outer residual in FP64: r = b - K x
             k_state_init<<<1, 1, 0, s>>>(st, opts.tol, opts.maxit, opts.stop_rule);
-            launch_csr<double, double, 2>(ctx, lpr, gs64, ctx->val.p, ctx->x.p, ctx->r.p, ctx->rhs.p, st);
+            enqueue_residual(ctx, ctx->r.p);
             S.spmv_launches++;
             BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
             BHIP_TRY(ctx, hipStreamSynchronize(s));
@@ -561,8 +705,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
             double rn = sqrt(rr);
             k_scale_to_f32<<<grid_for(n, B), B, 0, s>>>(n, ctx->r.p, 1.0 / rn, ctx->r32.p, ctx->d32.p);
             k_state_init<<<1, 1, 0, s>>>(st, inner_tol, opts.maxit - total_inner, 2);
-            k_pcg_init<float><<<gv, kThreads, 0, s>>>(n, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->dinv32.p, st,
-                                                      ctx->partials.p);
+            enqueue_init<float>(ctx, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->dinv32.p);
             BHIP_TRY(ctx, hipGetLastError());
             BHIP_PROPAGATE(run_iterations(ctx, true, opts.check_every, &S.spmv_launches));
             total_inner += ctx->h_state->iter;
@@ -585,8 +728,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
     S.ms_total = ms;
 
     // true residual |b - K x| / |b| in FP64
-    k_state_init<<<1, 1, 0, s>>>(st, 0, 0, 0);
-    launch_csr<double, double, 2>(ctx, lpr, gs64, ctx->val.p, ctx->x.p, ctx->tmp.p, ctx->rhs.p, st);
+    enqueue_residual(ctx, ctx->tmp.p);
     BHIP_TRY(ctx, hipGetLastError());
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipStreamSynchronize(s));
@@ -604,27 +746,22 @@ int bhip_bench_spmv_impl(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, doub
     PcgState *st = ctx->state.p;
     BHIP_PROPAGATE(prepare_diag(ctx));
     if (fp32) BHIP_PROPAGATE(prepare_f32(ctx));
+    BHIP_PROPAGATE(ensure_partials(ctx));
     const bool bsr = fp32 && ctx->use_bsr;
-    const int gs = spmv_grid(ctx, bsr);
-    const int lpr = pick_lpr(ctx->nnz, ctx->n_dof);
     const long long n = ctx->n_dof;
     const int B = 256;
-    // p = something finite, state: rz = 1 so alpha is finite
+    // p = finite data, state not done: the PCG form of the kernel runs in full
     if (fp32) k_to_f32<<<grid_for(n, B), B, 0, s>>>(n, ctx->dinv.p, ctx->p32.p);
     else BHIP_TRY(ctx, hipMemcpyAsync(ctx->p.p, ctx->dinv.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_state_init<<<1, 1, 0, s>>>(st, 1.0, 1, 0);
-    // warm-up launch, then the timed region
-    for (int w = 0; w < 2; w++) {
-        if (bsr) launch_bsr(ctx, gs, 1, ctx->p32.p, ctx->q32.p, st);
-        else if (fp32) launch_csr<float, float, 1>(ctx, lpr, gs, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
-        else launch_csr<double, double, 1>(ctx, lpr, gs, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
-    }
+    auto launch = [&]() {
+        if (bsr) launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
+        else if (fp32) launch_csr<float, float, 1>(ctx, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
+        else launch_csr<double, double, 1>(ctx, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
+    };
+    for (int w = 0; w < 2; w++) launch();
     BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    for (int k = 0; k < reps; k++) {
-        if (bsr) launch_bsr(ctx, gs, 1, ctx->p32.p, ctx->q32.p, st);
-        else if (fp32) launch_csr<float, float, 1>(ctx, lpr, gs, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
-        else launch_csr<double, double, 1>(ctx, lpr, gs, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
-    }
+    for (int k = 0; k < reps; k++) launch();
     BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
     BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
     BHIP_TRY(ctx, hipGetLastError());
