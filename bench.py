@@ -32,6 +32,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(n, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE/WRITE_SIZE scaled by a copy-kernel calibration), for
+    the workload size that summary covers; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "spmv_hex8_%d.md" % n)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch %s =" % kernel):
+            return float(line.split("=")[1])
+    return None
+
+
 def cpu_baseline(sample_n, budget_s):
     """Oracle (serial restatement of SerialAssembler + sparse.cpp PCG) on an
     n^3 version of the same workload: times the PCG iterations it completes
@@ -133,8 +146,9 @@ def main():
 
     ms32, by32 = ctx.bench_spmv(fp32=True, reps=50)
     achieved = by32 / ms32 / 1e6
+    _, stream_gbps = ctx.bench_stream(1 << 30, 20)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.n, "k_sell3_spmv<1>"),
                 "kernel": "k_sell3_spmv<1> (FP32 3x3-block sliced-ELL SpMV with p.Ap partials)",
                 "bytes_per_launch": by32, "ms_per_launch": ms32}
 
@@ -163,6 +177,7 @@ def main():
         "plain_fp64": result_fp64,
         "assembly": {"ms": 1e3 * min(t_asm), "elements_per_s": prob.n_el / min(t_asm)},
         "spmv_fp32_GBps": achieved,
+        "stream_copy_GBps": stream_gbps,
     }
     if result_fp64:
         line["speedup_vs_fp64_dof_iter"] = value / result_fp64["dof_iter_per_s"]

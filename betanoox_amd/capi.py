@@ -48,7 +48,7 @@ EXPORTS = [
     "bhip_get_csr", "bhip_set_materials", "bhip_set_integration", "bhip_assemble",
     "bhip_get_pseudo_load", "bhip_set_load", "bhip_build_rhs", "bhip_get_rhs", "bhip_set_rhs",
     "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv",
-    "bhip_bench_spmv", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
+    "bhip_bench_spmv", "bhip_bench_stream", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
 ]
 
@@ -128,6 +128,7 @@ def load_library(path=LIB_PATH):
         "bhip_pcg_solve": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_spmv": (C.c_int, [vp, _P_D, _P_D]),
         "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),
+        "bhip_bench_stream": (C.c_int, [vp, C.c_longlong, C.c_int, _P_D, _P_D]),
         "bhip_plastic_init": (C.c_int, [vp]),
         "bhip_plastic_update": (C.c_int, [vp, _P_D]),
         "bhip_plastic_residual": (C.c_int, [vp, _P_D, _P_D]),
@@ -356,6 +357,11 @@ class Context:
         u = np.zeros(self.n_dof)
         self._chk(self.lib.bhip_plastic_get_state(self.h, _d(s), _d(e), _d(u)))
         return s.reshape(n_el, nip, 6), e.reshape(n_el, nip), u
+
+    def bench_stream(self, nbytes=1 << 30, reps=20):
+        ms, g = C.c_double(), C.c_double()
+        self._chk(self.lib.bhip_bench_stream(self.h, nbytes, reps, C.byref(ms), C.byref(g)))
+        return ms.value, g.value
 
     def bench_spmv(self, fp32=False, reps=20):
         ms, by = C.c_double(), C.c_double()

@@ -411,3 +411,38 @@ int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *b
 }
 
 }  // extern "C"
+
+namespace {
+__global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                                    long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+}  // namespace
+
+extern "C" {
+
+int bhip_bench_stream(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, double *gbps) {
+    if (!ctx || !ms_avg || !gbps || bytes < 16 || reps <= 0) return BHIP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    long long n4 = bytes / 16;
+    dvec<float4> a, b;
+    BHIP_TRY(ctx, a.resize(n4));
+    BHIP_TRY(ctx, b.resize(n4));
+    BHIP_TRY(ctx, hipMemsetAsync(a.p, 0, a.bytes(), ctx->stream));
+    const int grid = ctx->n_cu * 32;
+    for (int w = 0; w < 2; w++) k_stream_copy<<<grid, 256, 0, ctx->stream>>>(a.p, b.p, n4);
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int k = 0; k < reps; k++) k_stream_copy<<<grid, 256, 0, ctx->stream>>>(a.p, b.p, n4);
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    BHIP_TRY(ctx, hipGetLastError());
+    float ms = 0;
+    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_avg = ms / reps;
+    *gbps = 2.0 * n4 * 16 / (*ms_avg * 1e6);
+    return BHIP_OK;
+}
+
+}  // extern "C"

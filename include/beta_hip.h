@@ -167,6 +167,10 @@ int bhip_spmv(bhip_ctx *ctx, const double *x, double *y);
 /* Time `reps` FP64 (fp32 = 0) or FP32 (fp32 = 1) SpMVs on resident vectors;
  * returns the mean kernel time in ms (HIP events). */
 int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes);
+/* Diagnostic without a reference counterpart: a plain device copy of `bytes`
+ * bytes (read + write), timed over `reps` launches; reports the achieved
+ * read+write GB/s (STREAM-copy convention) of this GPU. */
+int bhip_bench_stream(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, double *gbps);
 
 /* ------------------------------------------------------------ plasticity */
 typedef struct bhip_plastic_stats {
