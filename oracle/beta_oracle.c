@@ -1168,3 +1168,114 @@ int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double
     *deps_bar = g.dep;
     return iter;
 }
+
+/* ======================================================================
+ * Plastic hex8 element: PlasticElement::update (plasticElement.cpp:39-250)
+ * ====================================================================== */
+
+/* Global gradients of all 8 nodes at Gauss point ip; returns detJ*w. */
+static double hex8_ip_grads(const double *xyz, int order, int ip, double *dx, double *dy, double *dz)
+{
+    double gxi[64], geta[64], gzeta[64], gw[64];
+    orc_hex_quad_points(order, gxi, geta, gzeta, gw);
+    double x[8], y[8], z[8], dxi[8], deta[8], dzeta[8];
+    for (int a = 0; a < 8; a++) {
+        x[a] = xyz[3 * a];
+        y[a] = xyz[3 * a + 1];
+        z[a] = xyz[3 * a + 2];
+    }
+    orc_deriv_hex8(dxi, deta, dzeta, gxi[ip], geta[ip], gzeta[ip]);
+    double detJ = orc_jacobian3d(8, dxi, deta, dzeta, x, y, z, dx, dy, dz);
+    return detJ * gw[ip];
+}
+
+/* "V": strain increment from the element increment displacement
+ * (ElasticityElement3D::calculateStrain, engineering shear) and
+ * PlasticMaterial::updateISV -> getStressIncrement at every Gauss point. */
+int orc_plastic_update(const double *xyz, const orc_j2_params *p, int order, const double *ue, double *isv)
+{
+    int nip = order * order * order;
+    for (int ip = 0; ip < nip; ip++) {
+        double dx[8], dy[8], dz[8];
+        hex8_ip_grads(xyz, order, ip, dx, dy, dz);
+        double de[6] = {0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < 8; a++) {
+            double u = ue[3 * a], v = ue[3 * a + 1], w = ue[3 * a + 2];
+            de[0] += dx[a] * u;
+            de[1] += dy[a] * v;
+            de[2] += dz[a] * w;
+            de[3] += dy[a] * u + dx[a] * v;
+            de[4] += dz[a] * v + dy[a] * w;
+            de[5] += dz[a] * u + dx[a] * w;
+        }
+        double *r = isv + (size_t)ip * ORC_PISV;
+        double dep;
+        int pl;
+        int it = orc_j2_stress_increment(p, r, r[6], de, r + 7, &dep, r + 16, &pl);
+        r[13] = dep;
+        r[14] = pl;
+        r[15] = it;
+    }
+    return ORC_OK;
+}
+
+/* "KI": Ke = sum_ip B^T Cep B detJ w with the tangent of the last return map
+ * (getTangentStiffness, plastic.cpp:583-610; the elastic C where the point is
+ * not plastic); "F": Fe = sum_ip B^T dstress detJ w (addToFe with
+ * isv.dStress).  Ke is 24x24 row-major, upper triangle. */
+int orc_plastic_ke_fe(const double *xyz, const orc_j2_params *p, int order, const double *isv, double *Ke,
+                      double *Fe)
+{
+    int nip = order * order * order;
+    for (int i = 0; i < 24 * 24; i++) Ke[i] = 0;
+    for (int i = 0; i < 24; i++) Fe[i] = 0;
+    for (int ip = 0; ip < nip; ip++) {
+        double dx[8], dy[8], dz[8];
+        double f = hex8_ip_grads(xyz, order, ip, dx, dy, dz);
+        const double *r = isv + (size_t)ip * ORC_PISV;
+        const double *C = r + 16;
+        double b[6][24];
+        for (int i = 0, c1 = 0, c2 = 1, c3 = 2; i < 8; i++, c1 += 3, c2 += 3, c3 += 3) {
+            b[0][c1] = dx[i]; b[0][c2] = 0.;    b[0][c3] = 0.;
+            b[1][c1] = 0.;    b[1][c2] = dy[i]; b[1][c3] = 0.;
+            b[2][c1] = 0.;    b[2][c2] = 0.;    b[2][c3] = dz[i];
+            b[3][c1] = dy[i]; b[3][c2] = dx[i]; b[3][c3] = 0.;
+            b[4][c1] = 0.;    b[4][c2] = dz[i]; b[4][c3] = dy[i];
+            b[5][c1] = dz[i]; b[5][c2] = 0.;    b[5][c3] = dx[i];
+        }
+        double CxB[6][24];
+        for (int ii = 0; ii < 24; ii++)
+            for (int jj = 0; jj < 6; jj++) {
+                double s = 0;
+                for (int t = 0; t < 6; t++) s += b[t][ii] * C[t * 6 + jj];
+                CxB[jj][ii] = s;
+            }
+        for (int ii = 0; ii < 24; ii++)
+            for (int jj = ii; jj < 24; jj++) {
+                double s = 0;
+                for (int t = 0; t < 6; t++) s += CxB[t][ii] * b[t][jj];
+                Ke[ii * 24 + jj] += s * f;
+            }
+        const double *ds = r + 7;
+        for (int i = 0; i < 24; i++) {
+            double s = 0;
+            for (int t = 0; t < 6; t++) s += b[t][i] * ds[t];
+            Fe[i] += s * f;
+        }
+    }
+    return ORC_OK;
+}
+
+/* "U": GaussISV::updateISV (plastic.cpp:769-785). */
+void orc_plastic_commit(double *isv, int nip)
+{
+    for (int ip = 0; ip < nip; ip++) {
+        double *r = isv + (size_t)ip * ORC_PISV;
+        for (int i = 0; i < 6; i++) {
+            r[i] += r[7 + i];
+            r[7 + i] = 0;
+        }
+        r[6] += r[13];
+        r[13] = 0;
+    }
+}

@@ -111,6 +111,16 @@ int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double
                             const double *dstrain, double *dstress, double *deps_bar,
                             double *Cep36, int *plastic);
 
+/* Plastic hex8 element (plasticElement.cpp:39-250).  Per Gauss point record
+ * of ORC_PISV doubles: stress[6], eps_bar, dstress[6], deps_bar, plastic,
+ * return-map iterations, Cep[36]. */
+#define ORC_PISV 52
+int  orc_plastic_update(const double *xyz24, const orc_j2_params *p, int order, const double *ue24,
+                        double *isv);                                            /* "V" */
+int  orc_plastic_ke_fe(const double *xyz24, const orc_j2_params *p, int order, const double *isv,
+                       double *Ke, double *Fe);                                  /* "KI" + "F" */
+void orc_plastic_commit(double *isv, int nip);                                   /* "U" */
+
 #ifdef __cplusplus
 }
 #endif

@@ -322,3 +322,86 @@ def ref_lib():
     R._Z10derivQuad4PdS_dd.argtypes = [_P_D, _P_D, C.c_double, C.c_double]
     _ref = R
     return R
+
+
+# ------------------------------------------------------------ plasticity
+PISV = 52
+
+
+def _plastic_sigs():
+    L = lib()
+    if getattr(L, "_plastic_sigs", False):
+        return L
+    L.orc_plastic_update.restype = C.c_int
+    L.orc_plastic_update.argtypes = [_P_D, _P_D, C.c_int, _P_D, _P_D]
+    L.orc_plastic_ke_fe.restype = C.c_int
+    L.orc_plastic_ke_fe.argtypes = [_P_D, _P_D, C.c_int, _P_D, _P_D, _P_D]
+    L.orc_plastic_commit.restype = None
+    L.orc_plastic_commit.argtypes = [_P_D, C.c_int]
+    L._plastic_sigs = True
+    return L
+
+
+def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, solve_eps=1e-12,
+                     presc_list=None):
+    """PlasticityModel::DoPlasticAnalysis (PlasticityModel.cpp:142-398) over
+    the oracle's sparse system: per increment, prescribed increment values on
+    restrained DOFs, then "V", assembleKandF ("KI"), getGlobalForces ("F"),
+    residual = load - resultant on free DOFs, solve, add on free DOFs; stop
+    when max|residual| < 2*max_allowable_residual (checked from the second
+    pass on) or after max_iterations extra solves; then "U".  Returns the
+    accumulated displacement, the Gauss-point records and per-increment
+    (newton solves, max residual).  presc_list optionally gives the
+    prescribed increment vector of every increment (default: prob.prescribed)."""
+    L = _plastic_sigs()
+    n = prob.n_dof
+    order = prob.order
+    nip = order ** 3
+    p = f64(prob.mat[0][:4])
+    first, nd = dof_numbering(prob.n_nodes, 3)
+    eq = Equations(n)
+    eq.set_restraints(prob.restrained, prob.prescribed)
+    eq.specify_mesh(prob.conn, first, nd)
+    rest = prob.restrained.astype(bool)
+    conn = ai32(prob.conn)
+    maps = (3 * conn[:, :, None] + np.arange(3)[None, None, :]).reshape(conn.shape[0], 24).astype(np.int32)
+    xyz_e = [f64(prob.xyz[c]).ravel().copy() for c in conn]
+    isv = np.zeros((prob.n_el, nip, PISV))
+    utot = np.zeros(n)
+    hist = []
+    load = prob.load
+    Ke = np.zeros(576)
+    Fe = np.zeros(24)
+    for inc in range(increments):
+        presc = prob.prescribed if presc_list is None else presc_list[inc]
+        du = np.zeros(n)
+        du[rest] = presc[rest]
+        it = 0
+        while True:
+            for e in range(prob.n_el):
+                rec = isv[e]
+                L.orc_plastic_update(d(xyz_e[e]), d(p), order, d(f64(du[maps[e]])), d(rec))
+            L.orc_eq_zero(eq.h)
+            res = np.zeros(n)
+            for e in range(prob.n_el):
+                L.orc_plastic_ke_fe(d(xyz_e[e]), d(p), order, d(isv[e]), d(Ke), d(Fe))
+                st = L.orc_eq_add_matrix(eq.h, d(Ke), i32(maps[e]), 24, None)
+                if st != OK:
+                    raise RuntimeError("plastic scatter failed")
+                np.add.at(res, maps[e], Fe)
+            resid = np.where(rest, 0.0, load - res)
+            maxres = np.abs(resid).max()
+            if it > 0 and maxres < 2 * max_allowable_residual:
+                break
+            if it > max_iterations:
+                break
+            st, dsol, _, _ = eq.pcg(resid, eps=solve_eps, stop_rule=1, maxit=100000)
+            if st != OK:
+                raise RuntimeError("oracle PCG failed in Newton: %d" % st)
+            du[~rest] += dsol[~rest]
+            it += 1
+        for e in range(prob.n_el):
+            L.orc_plastic_commit(d(isv[e]), nip)
+        utot += du
+        hist.append((it, maxres))
+    return utot, isv, hist
