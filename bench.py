@@ -89,8 +89,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md, 'Sharding'); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (a single solve "
+              "would shard by z-slabs with a halo exchange; it is not built); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     from betanoox_amd import capi, meshgen, setup_problem

@@ -17,7 +17,7 @@ LIB = os.path.join(OUT, "libbeta_hip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# Packed FP32 instructions stay disabled ("Numerics").
+# Packed FP32 instructions stay disabled in every HIP compile.
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
