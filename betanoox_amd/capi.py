@@ -47,7 +47,7 @@ EXPORTS = [
     "bhip_build_pattern", "bhip_pattern_nnz", "bhip_get_pattern_lower", "bhip_get_values_lower",
     "bhip_get_csr", "bhip_set_materials", "bhip_set_integration", "bhip_assemble",
     "bhip_get_pseudo_load", "bhip_set_load", "bhip_build_rhs", "bhip_get_rhs", "bhip_set_rhs",
-    "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv",
+    "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv", "bhip_solve_cases",
     "bhip_bench_spmv", "bhip_bench_stream", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
 ]
@@ -127,6 +127,7 @@ def load_library(path=LIB_PATH):
         "bhip_get_solution": (C.c_int, [vp, _P_D]),
         "bhip_pcg_solve": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_spmv": (C.c_int, [vp, _P_D, _P_D]),
+        "bhip_solve_cases": (C.c_int, [vp, C.c_int, _P_D, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),
         "bhip_bench_stream": (C.c_int, [vp, C.c_longlong, C.c_int, _P_D, _P_D]),
         "bhip_plastic_init": (C.c_int, [vp]),
@@ -319,6 +320,22 @@ class Context:
         d = s.as_dict()
         d["status"] = st
         return bb, d
+
+    def solve_cases(self, presc, load=None, opts=None, allow_not_converged=False, **kw):
+        """Solve every load case of `presc` ((nrhs, n_dof) prescribed values)
+        with the assembled stiffness; returns (X (nrhs, n_dof), stats)."""
+        P = np.ascontiguousarray(np.atleast_2d(presc), dtype=np.float64)
+        if P.shape[1] != self.n_dof:
+            raise ValueError("presc must be (nrhs, n_dof)")
+        X = np.zeros_like(P)
+        o = opts if opts is not None else self.opts(**kw)
+        s = Stats()
+        allow = (ERR_NOT_CONVERGED,) if allow_not_converged else ()
+        st = self._chk(self.lib.bhip_solve_cases(self.h, P.shape[0], _d(P), _d(None if load is None else _f64(load)),
+                                                 _d(X), C.byref(o), C.byref(s)), allow)
+        d = s.as_dict()
+        d["status"] = st
+        return X, d
 
     def spmv(self, x):
         y = np.zeros(self.n_dof)

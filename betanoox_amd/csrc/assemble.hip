@@ -441,6 +441,7 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     if (!ctx->have_mat) return bhip_fail(ctx, BHIP_ERR_STATE, "materials not set");
     BHIP_PROPAGATE(bhip_ensure_vectors(ctx));
     bhip_invalidate_graphs(ctx);
+    ctx->lift_ready = false;
     int nip = 0;
     BHIP_PROPAGATE(upload_gauss(ctx, ctx->nnpe, ctx->order, &nip));
     BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 4, ctx->stream));
@@ -519,8 +520,187 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     BHIP_TRY(ctx, hipMemcpyAsync(hflags, ctx->flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, ctx->stream));
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->assembled = true;
+    ctx->asm_kind = what;
     ctx->f32_ready = false;
+    ctx->lift_ready = false;
     if (hflags[0] & 1) return bhip_fail(ctx, BHIP_ERR_NEG_DETJ, "negative Jacobian determinant (inverted element)");
     if (hflags[0] & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "bad diagonal in an element stiffness");
+    return BHIP_OK;
+}
+
+// ------------------------------------------------------------ lift table
+// Equations::updatePseudoLoadVectorUsingGK (equation.cpp:343-381) re-lifts new
+// prescribed values through an assembled, unrestrained K: F[i] -= K(i,j) u_j
+// for free i, restrained j.  Here the free-row x restrained-column coupling
+// blocks of the last stiffness assembly are kept as a table (one entry per
+// (node with free DOFs, neighbour with restrained DOFs), neighbours in
+// ascending order), built once per assembly by one thread per entry, so each
+// load case costs one pass over the table instead of a re-integration.
+namespace {
+
+__global__ void k_lift_count(int n_nodes, const int *ndof, const unsigned char *rmask, const int *nbr_ptr,
+                             const int *nbr, long long *cnt) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n > n_nodes) return;
+    long long c = 0;
+    if (n < n_nodes) {
+        const int nd = ndof[n];
+        const unsigned rn = rmask[n];
+        if (nd > 0 && rn != (1u << nd) - 1u)
+            for (int p = nbr_ptr[n]; p < nbr_ptr[n + 1]; p++) c += rmask[nbr[p]] != 0;
+    }
+    cnt[n] = c;  // entry n_nodes stays 0: the scan's last element is the total
+}
+
+__global__ void k_lift_fill(int n_nodes, const unsigned char *rmask, const int *nbr_ptr, const int *nbr,
+                            const long long *lptr, int *lrow, int *lcol) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    long long o = lptr[n];
+    if (lptr[n + 1] == o) return;
+    for (int p = nbr_ptr[n]; p < nbr_ptr[n + 1]; p++) {
+        int m = nbr[p];
+        if (rmask[m] != 0) {
+            lrow[o] = n;
+            lcol[o] = m;
+            o++;
+        }
+    }
+}
+
+template <class P>
+__global__ void __launch_bounds__(256) k_lift_blocks(AsmArgs A, const int *n2e_ptr, const int *n2e,
+                                                     long long n_lift, const int *lrow, const int *lcol,
+                                                     double *lval) {
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_lift) return;
+    const int n = lrow[k], m = lcol[k];
+    const int lo = n < m ? n : m, hi = n < m ? m : n;
+    double K[P::NB], K2[P::NB];
+    pair_block<P>(A, n2e_ptr, n2e, n, lo, hi, K, K2);
+    const bool nlo = (n == lo);
+#pragma unroll
+    for (int c = 0; c < P::ND; c++)
+#pragma unroll
+        for (int d = 0; d < P::ND; d++) lval[k * P::NB + c * P::ND + d] = oriented<P::ND>(K, nlo, c, d);
+}
+
+// b_r[I] = restrained ? presc_r[I] : load[I] - sum over the node's table
+// entries (ascending neighbour, then component) of K(I,J) presc_r[J],
+// J restrained -- the order in which the assembly's gather accumulates.
+template <int ND>
+__global__ void k_lift_rhs(int n_nodes, int R, long long n_dof, const int *first, const unsigned char *rmask,
+                           const long long *lptr, const int *lcol, const double *lval, const double *presc,
+                           const double *load, double *b) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const unsigned rn = rmask[n];
+    const int f = first[n];
+    const long long kb = lptr[n], ke = lptr[n + 1];
+    for (int r = 0; r < R; r++) {
+        const double *pr = presc + (long long)r * n_dof;
+        double acc[ND];
+#pragma unroll
+        for (int c = 0; c < ND; c++) acc[c] = 0;
+        for (long long k = kb; k < ke; k++) {
+            const int m = lcol[k];
+            const unsigned rm = rmask[m];
+            const int fm = first[m];
+#pragma unroll
+            for (int c = 0; c < ND; c++) {
+                if ((rn >> c) & 1u) continue;
+#pragma unroll
+                for (int d = 0; d < ND; d++)
+                    if ((rm >> d) & 1u) acc[c] -= lval[k * ND * ND + c * ND + d] * pr[fm + d];
+            }
+        }
+        double *br = b + (long long)r * n_dof;
+#pragma unroll
+        for (int c = 0; c < ND; c++) {
+            const long long I = f + c;
+            br[I] = ((rn >> c) & 1u) ? pr[I] : (load ? load[I] : 0.0) + acc[c];
+        }
+    }
+}
+
+int lift_build(bhip_ctx *ctx) {
+    if (ctx->lift_ready) return BHIP_OK;
+    if (!ctx->assembled || ctx->asm_kind != BHIP_ASM_STIFFNESS)
+        return bhip_fail(ctx, BHIP_ERR_STATE, "re-lifting needs an assembled stiffness (BHIP_ASM_STIFFNESS)");
+    const int kind = ctx->mat_kind;
+    const bool hex = ctx->nnpe == 8 && ctx->max_ndof == 3 && ctx->uniform_ndof &&
+                     (kind == BHIP_MAT_ISO_ELASTIC || kind == BHIP_MAT_J2 || kind == BHIP_MAT_CMAT3D);
+    const bool quad = ctx->nnpe == 4 && ctx->max_ndof == 2 && ctx->uniform_ndof && kind == BHIP_MAT_PLANE_STRESS;
+    if (!hex && !quad) return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "re-lifting not built for this element/material");
+    hipStream_t s = ctx->stream;
+    const int B = 256;
+    const int nn = ctx->n_nodes;
+    dvec<long long> cnt;
+    BHIP_TRY(ctx, cnt.resize((size_t)nn + 1));
+    BHIP_TRY(ctx, ctx->lift_ptr.resize((size_t)nn + 1));
+    k_lift_count<<<grid_for(nn + 1, B), B, 0, s>>>(nn, ctx->ndof.p, ctx->node_rmask.p, ctx->nbr_ptr.p, ctx->nbr.p,
+                                                   cnt.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    BHIP_PROPAGATE(bhip_scan_i64(ctx, cnt.p, ctx->lift_ptr.p, (long long)nn + 1));
+    long long total = 0;
+    BHIP_TRY(ctx, hipMemcpy(&total, ctx->lift_ptr.p + nn, sizeof(long long), hipMemcpyDeviceToHost));
+    ctx->n_lift = total;
+    const size_t ne = total > 0 ? (size_t)total : 1;
+    const int ND = hex ? 3 : 2;
+    BHIP_TRY(ctx, ctx->lift_row.resize(ne));
+    BHIP_TRY(ctx, ctx->lift_col.resize(ne));
+    BHIP_TRY(ctx, ctx->lift_val.resize(ne * ND * ND));
+    if (total > 0) {
+        k_lift_fill<<<grid_for(nn, B), B, 0, s>>>(nn, ctx->node_rmask.p, ctx->nbr_ptr.p, ctx->nbr.p, ctx->lift_ptr.p,
+                                                  ctx->lift_row.p, ctx->lift_col.p);
+        int nip = 0;
+        BHIP_PROPAGATE(upload_gauss(ctx, ctx->nnpe, ctx->order, &nip));
+        AsmArgs A;
+        A.xyz = ctx->xyz.p;
+        A.conn = ctx->conn.p;
+        A.mat = ctx->mat.p;
+        A.elem_group = ctx->elem_group.n ? ctx->elem_group.p : nullptr;
+        A.n_el = ctx->n_el;
+        A.n_groups = ctx->n_groups;
+        A.mat_stride = ctx->mat_stride;
+        A.nnpe = ctx->nnpe;
+        A.nip = nip;
+        A.factor = ctx->factor;
+        A.tan = nullptr;
+        A.flags = nullptr;
+        const int g = grid_for(total, B);
+        if (quad)
+            k_lift_blocks<ElasQuad4><<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,
+                                                     ctx->lift_col.p, ctx->lift_val.p);
+        else if (kind == BHIP_MAT_CMAT3D)
+            k_lift_blocks<ElasHex8<BHIP_MAT_CMAT3D>><<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total,
+                                                                     ctx->lift_row.p, ctx->lift_col.p,
+                                                                     ctx->lift_val.p);
+        else  // isotropic, and the elastic stiffness of the J2 material
+            k_lift_blocks<ElasHex8<BHIP_MAT_ISO_ELASTIC>><<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total,
+                                                                          ctx->lift_row.p, ctx->lift_col.p,
+                                                                          ctx->lift_val.p);
+        BHIP_TRY(ctx, hipGetLastError());
+    }
+    BHIP_TRY(ctx, hipStreamSynchronize(s));
+    cnt.release();
+    ctx->lift_ready = true;
+    return BHIP_OK;
+}
+
+}  // namespace
+
+int bhip_lift_rhs(bhip_ctx *ctx, int R, const double *presc, const double *load, double *b) {
+    BHIP_PROPAGATE(lift_build(ctx));
+    const int B = 256;
+    if (ctx->max_ndof == 3)
+        k_lift_rhs<3><<<grid_for(ctx->n_nodes, B), B, 0, ctx->stream>>>(
+            ctx->n_nodes, R, ctx->n_dof, ctx->dof_first.p, ctx->node_rmask.p, ctx->lift_ptr.p, ctx->lift_col.p,
+            ctx->lift_val.p, presc, load, b);
+    else
+        k_lift_rhs<2><<<grid_for(ctx->n_nodes, B), B, 0, ctx->stream>>>(
+            ctx->n_nodes, R, ctx->n_dof, ctx->dof_first.p, ctx->node_rmask.p, ctx->lift_ptr.p, ctx->lift_col.p,
+            ctx->lift_val.p, presc, load, b);
+    BHIP_TRY(ctx, hipGetLastError());
     return BHIP_OK;
 }

@@ -119,6 +119,14 @@ struct bhip_ctx {
     int order = 2;
     double factor = 1.0;
     bool assembled = false;
+    int asm_kind = 0;
+    // ---- lift table of the last stiffness assembly: for every node with free
+    // DOFs, its neighbours with restrained DOFs and their FP64 coupling blocks
+    bool lift_ready = false;
+    long long n_lift = 0;
+    dvec<long long> lift_ptr;   // per node
+    dvec<int> lift_row, lift_col;
+    dvec<double> lift_val;      // ndof x ndof per entry, row comp of the owner node first
     // ---- vectors (FP64)
     dvec<double> load, pseudo, rhs, x, x0, r, p, q, dinv, tmp;
     // ---- vectors (FP32 inner)
@@ -134,6 +142,8 @@ struct bhip_ctx {
     double spmv_bytes64 = 0, spmv_bytes32 = 0;
     // ---- J2 plasticity state (plastic.hip)
     struct PlasticData *plastic = nullptr;
+    // ---- multi-case solve buffers and graph (multi.hip)
+    struct MultiData *multi = nullptr;
 };
 
 // -------------------------------------------------------------- launching
@@ -197,4 +207,13 @@ int bhip_export_lower(bhip_ctx *ctx, int *rowptr, int *colind, double *val);
 int bhip_bench_spmv_impl(bhip_ctx *ctx, int fp32, int reps, double *ms, double *bytes);
 void bhip_invalidate_graphs(bhip_ctx *ctx);
 void bhip_plastic_release(bhip_ctx *ctx);
+void bhip_multi_invalidate(bhip_ctx *ctx);
+void bhip_multi_release(bhip_ctx *ctx);
+int bhip_scan_i64(bhip_ctx *ctx, const long long *in, long long *out, long long n);
+// Right-hand sides of R load cases from the lift table (built on first use
+// after a stiffness assembly): b_r = restrained ? presc_r : load + (-K_fr presc_r).
+// presc and b are device arrays, R x n_dof case-major; load may be null.
+int bhip_lift_rhs(bhip_ctx *ctx, int R, const double *presc, const double *load, double *b);
+int bhip_prepare_solver(bhip_ctx *ctx, bool fp32);
+int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb);
 const double *bhip_plastic_tangent(bhip_ctx *ctx);

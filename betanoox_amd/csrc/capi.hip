@@ -71,6 +71,7 @@ void bhip_destroy(bhip_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     bhip_invalidate_graphs(ctx);
     bhip_plastic_release(ctx);
+    bhip_multi_release(ctx);
     dvec<double> *v64[] = {&ctx->xyz,  &ctx->presc, &ctx->val, &ctx->val_aux, &ctx->mat, &ctx->load,
                            &ctx->pseudo, &ctx->rhs, &ctx->x,   &ctx->x0,      &ctx->r,   &ctx->p,
                            &ctx->q,    &ctx->dinv,  &ctx->tmp, &ctx->partials};
@@ -83,6 +84,10 @@ void bhip_destroy(bhip_ctx *ctx) {
     for (auto *v : v32) v->release();
     ctx->rest.release();
     ctx->node_rmask.release();
+    ctx->lift_ptr.release();
+    ctx->lift_row.release();
+    ctx->lift_col.release();
+    ctx->lift_val.release();
     ctx->state.release();
     if (ctx->h_state) (void)hipHostFree(ctx->h_state);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

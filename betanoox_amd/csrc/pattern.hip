@@ -182,6 +182,10 @@ int exclusive_scan(bhip_ctx *ctx, const T *in, T *out, long long n) {
 
 }  // namespace
 
+int bhip_scan_i64(bhip_ctx *ctx, const long long *in, long long *out, long long n) {
+    return exclusive_scan<long long>(ctx, in, out, n);
+}
+
 int bhip_pattern_build(bhip_ctx *ctx) {
     hipStream_t s = ctx->stream;
     const int B = 256;

@@ -637,6 +637,27 @@ void bhip_invalidate_graphs(bhip_ctx *ctx) {
     if (ctx->graph32) (void)hipGraphExecDestroy(ctx->graph32);
     ctx->graph64 = ctx->graph32 = nullptr;
     ctx->graph64_iters = ctx->graph32_iters = 0;
+    bhip_multi_invalidate(ctx);
+}
+
+int bhip_prepare_solver(bhip_ctx *ctx, bool fp32) {
+    BHIP_PROPAGATE(prepare_diag(ctx));
+    if (fp32) BHIP_PROPAGATE(prepare_f32(ctx));
+    BHIP_PROPAGATE(ensure_partials(ctx));
+    return BHIP_OK;
+}
+
+// r = b - K x (FP64 CSR), returns |r|^2 and |b|^2 (synchronises the stream).
+int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb) {
+    PcgState *st = ctx->state.p;
+    launch_csr<double, double, 2>(ctx, ctx->val.p, x, r, b, st);
+    k_fin_resid<<<1, kFinThreads, 0, ctx->stream>>>(csr_grid(ctx), ctx->partials.p, st);
+    BHIP_TRY(ctx, hipGetLastError());
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, ctx->stream));
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *rr = ctx->h_state->rr;
+    *bb = ctx->h_state->bb;
+    return BHIP_OK;
 }
 
 int bhip_spmv_dev(bhip_ctx *ctx, const double *x, double *y) {

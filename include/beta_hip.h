@@ -164,6 +164,20 @@ int bhip_pcg_solve(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_
                    bhip_stats *stats);
 /* LargeMatrix::vectorProduct (sparse.cpp:452-478): y = K x. */
 int bhip_spmv(bhip_ctx *ctx, const double *x, double *y);
+/* Several load cases on the assembled stiffness: the multi-column solve of
+ * MultiFieldMacroElement::CalculateK / solveForStiffnessMatrixColumn
+ * (MultiFieldMacroElement.cpp:135-407), where each case re-lifts its prescribed
+ * displacements through the assembled K (Equations::updatePseudoLoadVectorUsingGK,
+ * equation.cpp:343-381) and is solved with the same matrix.
+ * presc: nrhs x n_dof prescribed values (case-major; read on restrained DOFs);
+ * load: n_dof external load shared by all cases, or NULL; x: nrhs x n_dof out.
+ * 1 <= nrhs <= 8.  Needs a BHIP_ASM_STIFFNESS assembly.  With BHIP_PREC_FP32_IR
+ * on a 3-DOF-per-node mesh the cases share one fused inner solve (each keeps its
+ * own CG recurrence and stopping test); otherwise they are solved in turn.
+ * stats: iters = inner iterations of the fused solve, rel_resid = worst case.
+ * The context's own prescribed values, load and RHS are left unchanged. */
+int bhip_solve_cases(bhip_ctx *ctx, int nrhs, const double *presc, const double *load, double *x,
+                     const bhip_solve_opts *opts, bhip_stats *stats);
 /* Time `reps` FP64 (fp32 = 0) or FP32 (fp32 = 1) SpMVs on resident vectors;
  * returns the mean kernel time in ms (HIP events). */
 int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes);

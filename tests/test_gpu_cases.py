@@ -1,0 +1,149 @@
+"""Several load cases on one assembled stiffness (bhip_solve_cases), the
+multi-column solve of MultiFieldMacroElement::CalculateK
+(MultiFieldMacroElement.cpp:302-407): every case re-lifts its prescribed
+displacements through the assembled K and is solved with the same matrix.
+
+Checks and tolerances:
+  - homogeneous cube, affine boundary displacements: the hex8 field reproduces
+    the affine solution exactly (patch test), so every node must carry
+    u = eps . x; |u - eps.x|_inf <= 1e-7 * |eps.x|_inf at a 1e-10 solve
+  - each case of a fused FP32-inner solve against the oracle (reference
+    assembly order, FP64 PCG to 1e-13 relative residual): relative error
+    <= 1e-7 at a 1e-10 solve
+  - fused cases against the single-case solver of the same library: relative
+    error <= 1e-7 (both at 1e-10)
+  - plain FP64 (cases solved in turn) is bitwise identical to single solves
+    (the lift table reproduces the assembly's lift exactly); a 2-DOF-per-node
+    mesh agrees with single solves to 1e-7
+  - argument and state errors are reported, not run
+"""
+import numpy as np
+import pytest
+
+from betanoox_amd import capi, meshgen, setup_problem
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rve(n, seed, n_incl=None):
+    return meshgen.rve_request(n, seed, n_incl=n_incl)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _fp32ir(tol=1e-10):
+    return capi.Context.opts(tol=tol, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
+                             inner_tol=1e-3, maxit=20000)
+
+
+def test_affine_patch_all_cases(hip_ctx_factory):
+    prob = _rve(8, 11, n_incl=0)
+    print("case affine hex8 8^3", prob.n_dof, flush=True)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    P = np.stack(prob.extra["cases"])
+    X, st = ctx.solve_cases(P, opts=_fp32ir())
+    assert st["rel_resid"] <= 1e-10
+    for k in range(6):
+        eps = np.zeros((3, 3))
+        if k < 3:
+            eps[k, k] = 1.0
+        else:
+            a, b = [(0, 1), (1, 2), (0, 2)][k - 3]
+            eps[a, b] = eps[b, a] = 0.5
+        u = (prob.xyz @ eps.T).ravel()
+        assert np.abs(X[k] - u).max() <= 1e-7 * np.abs(u).max(), k
+
+
+@pytest.mark.parametrize("n,seed", [(6, 3), (10, 5)])
+def test_cases_vs_oracle_and_single(hip_ctx_factory, n, seed):
+    prob = _rve(n, seed)
+    print("case rve", n, seed, prob.n_dof, flush=True)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    P = np.stack(prob.extra["cases"])
+    X, st = ctx.solve_cases(P, opts=_fp32ir())
+    assert st["status"] == capi.OK and st["rel_resid"] <= 1e-10
+    group = prob.extra["group"]
+    for k in range(6):
+        prob.prescribed = P[k]
+        eq = O.assemble_problem(prob, elem_group=group)
+        status, x_o, it, nrm = eq.pcg(eq.rhs(), eps=1e-13, maxit=20000, stop_rule=1)
+        assert status == 0
+        assert _rel(X[k], x_o) <= 1e-7, k
+        # single-case path of the same library
+        ctx.set_prescribed(P[k])
+        ctx.assemble(capi.ASM_STIFFNESS)
+        ctx.set_load(prob.load)
+        ctx.build_rhs()
+        ctx.solve(_fp32ir())
+        assert _rel(X[k], ctx.solution()) <= 1e-7, k
+
+
+def test_repeat_and_subset_identical(hip_ctx_factory):
+    """The fused solve is deterministic: a repeat is bitwise identical, a
+    permutation of the cases permutes the results bitwise (each case keeps its
+    own recurrence and reductions), and a subset agrees to 1e-8."""
+    prob = _rve(8, 21)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    P = np.stack(prob.extra["cases"])
+    X1, _ = ctx.solve_cases(P, opts=_fp32ir())
+    X2, _ = ctx.solve_cases(P, opts=_fp32ir())
+    assert np.array_equal(X1, X2)
+    Xr, _ = ctx.solve_cases(P[::-1].copy(), opts=_fp32ir())
+    assert np.array_equal(X1, Xr[::-1])
+    X3, _ = ctx.solve_cases(P[2:5].copy(), opts=_fp32ir())
+    for k in range(3):
+        assert _rel(X3[k], X1[2 + k]) <= 1e-8
+
+
+def test_fp64_and_quad_paths(hip_ctx_factory):
+    prob = _rve(6, 9)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    P = np.stack(prob.extra["cases"])[:3]
+    o64 = capi.Context.opts(tol=1e-12, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64, maxit=20000)
+    X, st = ctx.solve_cases(P, opts=o64)
+    for k in range(3):
+        ctx.set_prescribed(P[k])
+        ctx.assemble(capi.ASM_STIFFNESS)
+        ctx.set_load(prob.load)
+        ctx.build_rhs()
+        ctx.solve(o64)
+        # the lift table reproduces the assembly's lift bit for bit, and the
+        # case-by-case FP64 path then runs the same solver on the same RHS
+        assert np.array_equal(X[k], ctx.solution()), _rel(X[k], ctx.solution())
+    # plane stress quad (2 DOF per node): the inner solve is not the blocked one
+    q = meshgen.plane_stress_quad(12)
+    qc = setup_problem(hip_ctx_factory(), q)
+    qc.assemble(capi.ASM_STIFFNESS)
+    rng = np.random.default_rng(4)
+    Pq = np.where(q.restrained.astype(bool)[None, :], rng.uniform(-1, 1, (2, q.n_dof)), 0.0)
+    Xq, _ = qc.solve_cases(Pq, load=q.load, opts=_fp32ir())
+    for k in range(2):
+        qc.set_prescribed(Pq[k])
+        qc.assemble(capi.ASM_STIFFNESS)
+        qc.set_load(q.load)
+        qc.build_rhs()
+        qc.solve(_fp32ir())
+        assert _rel(Xq[k], qc.solution()) <= 1e-7
+
+
+def test_errors(hip_ctx_factory):
+    prob = _rve(4, 1)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    P = np.stack(prob.extra["cases"])
+    with pytest.raises(capi.BhipError) as e:
+        ctx.solve_cases(P, opts=_fp32ir())
+    assert e.value.status == capi.ERR_STATE
+    ctx.assemble(capi.ASM_STIFFNESS)
+    with pytest.raises(capi.BhipError) as e:
+        ctx.solve_cases(np.zeros((9, prob.n_dof)), opts=_fp32ir())
+    assert e.value.status == capi.ERR_ARG
+    # all-zero cases: solution zero
+    X, st = ctx.solve_cases(np.zeros((2, prob.n_dof)), opts=_fp32ir())
+    assert np.all(X == 0)
