@@ -50,6 +50,8 @@ EXPORTS = [
     "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv", "bhip_solve_cases",
     "bhip_bench_spmv", "bhip_bench_stream", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
+    "bhip_server_create", "bhip_server_destroy", "bhip_server_submit", "bhip_server_poll", "bhip_server_wait",
+    "bhip_server_budget", "bhip_request_footprint",
 ]
 
 
@@ -66,6 +68,31 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("max_slots", C.c_int), ("hbm_budget", C.c_double), ("tol", C.c_double),
+                ("inner_tol", C.c_double), ("maxit", C.c_int), ("precision", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("xyz", C.POINTER(C.c_double)), ("n_el", C.c_int),
+                ("conn", C.POINTER(C.c_int)), ("rest", C.POINTER(C.c_ubyte)), ("mat_kind", C.c_int),
+                ("n_groups", C.c_int), ("mat", C.POINTER(C.c_double)), ("elem_group", C.POINTER(C.c_int)),
+                ("n_cases", C.c_int), ("presc", C.POINTER(C.c_double)), ("load", C.POINTER(C.c_double)),
+                ("order", C.c_int)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("slot", C.c_int), ("reused_pattern", C.c_int), ("iters", C.c_int),
+                ("outer_iters", C.c_int), ("rel_resid", C.c_double), ("queue_ms", C.c_double),
+                ("service_ms", C.c_double), ("latency_ms", C.c_double), ("start_ms", C.c_double),
+                ("end_ms", C.c_double), ("message", C.c_char * 256)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "message"}
+        d["message"] = self.message.decode(errors="replace")
+        return d
 
 
 class PlasticStats(C.Structure):
@@ -137,6 +164,13 @@ def load_library(path=LIB_PATH):
         "bhip_plastic_increment": (C.c_int, [vp, _P_D, C.c_double, C.c_int, C.POINTER(SolveOpts),
                                              C.POINTER(PlasticStats)]),
         "bhip_plastic_get_state": (C.c_int, [vp, _P_D, _P_D, _P_D]),
+        "bhip_server_create": (C.c_int, [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]),
+        "bhip_server_destroy": (None, [vp]),
+        "bhip_server_submit": (C.c_int, [vp, C.POINTER(Request), _P_LL]),
+        "bhip_server_poll": (C.c_int, [vp, C.c_longlong, _P_I]),
+        "bhip_server_wait": (C.c_int, [vp, C.c_longlong, _P_D, C.POINTER(RequestStats)]),
+        "bhip_server_budget": (C.c_double, [vp]),
+        "bhip_request_footprint": (C.c_double, [C.c_int, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -384,3 +418,80 @@ class Context:
         ms, by = C.c_double(), C.c_double()
         self._chk(self.lib.bhip_bench_spmv(self.h, int(fp32), reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+
+class Server:
+    """Resident request loop (bhip_server_*): hex8 elasticity requests with
+    several prescribed-displacement load cases, solved by up to `max_slots`
+    contexts at once on one GPU."""
+
+    def __init__(self, device=0, max_slots=4, hbm_budget=0.0, tol=1e-8, inner_tol=1e-3, maxit=40000,
+                 precision=PREC_FP32_IR):
+        self.lib = load_library()
+        o = ServerOpts(max_slots, hbm_budget, tol, inner_tol, maxit, precision)
+        h = C.c_void_p()
+        st = self.lib.bhip_server_create(device, C.byref(o), C.byref(h))
+        if st != OK:
+            raise BhipError(st, "bhip_server_create failed (is a GPU visible?)")
+        self.h = h
+        self._shape = {}
+
+    def close(self):
+        if self.h:
+            self.lib.bhip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def budget(self):
+        return self.lib.bhip_server_budget(self.h)
+
+    def submit(self, prob, cases=None, elem_group=None):
+        """Queue a hex8 Problem; `cases` ((n_cases, n_dof) prescribed values)
+        defaults to prob.extra["cases"], elem_group to prob.extra["group"]."""
+        P = np.ascontiguousarray(np.atleast_2d(cases if cases is not None else np.stack(prob.extra["cases"])),
+                                 dtype=np.float64)
+        xyz, conn = _f64(prob.xyz), _i32(prob.conn)
+        rest = np.ascontiguousarray(prob.restrained, dtype=np.uint8)
+        mat = _f64(prob.mat)
+        g = elem_group if elem_group is not None else prob.extra.get("group")
+        g = None if g is None else _i32(g)
+        load = None if prob.load is None or not np.any(prob.load) else _f64(prob.load)
+        q = Request(prob.n_nodes, _d(xyz), prob.n_el, conn.ctypes.data_as(_P_I), rest.ctypes.data_as(_P_U8),
+                    prob.mat_kind, mat.shape[0], _d(mat), None if g is None else g.ctypes.data_as(_P_I),
+                    P.shape[0], _d(P), _d(load), prob.order)
+        rid = C.c_longlong()
+        st = self.lib.bhip_server_submit(self.h, C.byref(q), C.byref(rid))
+        if st != OK:
+            raise BhipError(st, "bhip_server_submit failed")
+        self._shape[rid.value] = P.shape
+        return rid.value
+
+    def poll(self, rid):
+        d = C.c_int()
+        st = self.lib.bhip_server_poll(self.h, rid, C.byref(d))
+        if st != OK:
+            raise BhipError(st, "unknown request id")
+        return bool(d.value)
+
+    def wait(self, rid):
+        """(X (n_cases, n_dof), stats dict); raises if the request failed."""
+        shape = self._shape.pop(rid)
+        X = np.zeros(shape)
+        s = RequestStats()
+        st = self.lib.bhip_server_wait(self.h, rid, _d(X), C.byref(s))
+        d = s.as_dict()
+        if st != OK:
+            raise BhipError(st, d["message"])
+        return X, d

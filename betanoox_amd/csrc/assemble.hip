@@ -19,20 +19,14 @@
 #include "bhip_internal.hpp"
 #include "gauss_tables.hpp"
 
-// Gauss tables, filled by the host from the reference's Gauss-Legendre
-// construction (gauss.cpp:14-60, 303-365, 825-877).
-__constant__ double c_w[kMaxIp];
-__constant__ double c_dN[kMaxIp * 24];  // hex8: [ip][xi|eta|zeta][8]; quad4: [ip][xi|eta][4]
-__constant__ double c_N[kMaxIp * 8];    // shape values [ip][node]
-
 namespace {
 
 // ------------------------------------------------------------ geometry
 // IsoElement::jacobian, 3-direction branch (isoElement.cpp:324-372);
 // returns detJ and the global gradients of nodes a and b.
-__device__ __forceinline__ double hex8_grads(const double *x, const double *y, const double *z,
-                                             int ip, int a, int b, double *ga, double *gb) {
-    const double *dxi = c_dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
+__device__ __forceinline__ double hex8_grads(const double *dN, const double *x, const double *y,
+                                             const double *z, int ip, int a, int b, double *ga, double *gb) {
+    const double *dxi = dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
     double px1 = 0, px2 = 0, px3 = 0, py1 = 0, py2 = 0, py3 = 0, pz1 = 0, pz2 = 0, pz3 = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -64,9 +58,9 @@ __device__ __forceinline__ double hex8_grads(const double *x, const double *y, c
 }
 
 // 2-direction branch (isoElement.cpp:290-322).
-__device__ __forceinline__ double quad4_grads(const double *x, const double *y, int ip, int a,
-                                              int b, double *ga, double *gb) {
-    const double *dxi = c_dN + ip * 24, *deta = dxi + 4;
+__device__ __forceinline__ double quad4_grads(const double *dN, const double *x, const double *y, int ip,
+                                              int a, int b, double *ga, double *gb) {
+    const double *dxi = dN + ip * 24, *deta = dxi + 4;
     double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -95,6 +89,9 @@ struct AsmArgs {
     // plasticity: per (element, ip) tangent descriptor: sp[6], 1/denom, flag
     const double *tan;
     int *flags;
+    // Gauss tables (ctx->gauss): weights, derivatives [ip][xi|eta|zeta][8]
+    // (quad4: [ip][xi|eta][4]), shape values [ip][node]
+    const double *gw, *gdN, *gN;
 };
 
 __device__ __forceinline__ int group_of(const AsmArgs &A, int e) {
@@ -140,9 +137,9 @@ struct ElasHex8 {
             for (int i = 0; i < 9; i++) Ke[i] = 0;
         for (int ip = 0; ip < A.nip; ip++) {
             double ga[3], gb[3];
-            double detJ = hex8_grads(x, y, z, ip, a, b, ga, gb);
+            double detJ = hex8_grads(A.gdN, x, y, z, ip, a, b, ga, gb);
             if (detJ < 0) bad = 1;
-            double f = detJ * c_w[ip] * A.factor;
+            double f = detJ * A.gw[ip] * A.factor;
             double blk[9];
             if (KIND == BHIP_MAT_ISO_ELASTIC || KIND == BHIP_MAT_J2) {
                 double gg = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
@@ -214,9 +211,9 @@ struct ElasQuad4 {
         double Ke[4] = {0, 0, 0, 0};
         for (int ip = 0; ip < A.nip; ip++) {
             double ga[2], gb[2];
-            double detJ = quad4_grads(x, y, ip, a, b, ga, gb);
+            double detJ = quad4_grads(A.gdN, x, y, ip, a, b, ga, gb);
             if (detJ < 0) bad = 1;
-            double f = detJ * c_w[ip] * A.factor;
+            double f = detJ * A.gw[ip] * A.factor;
             double CB[3][2];
 #pragma unroll
             for (int s = 0; s < 3; s++) {
@@ -259,11 +256,11 @@ struct HeatHex8 {
         int bad = 0;
         for (int ip = 0; ip < A.nip; ip++) {
             double ga[3], gb[3];
-            double detJ = hex8_grads(x, y, z, ip, a, b, ga, gb);
+            double detJ = hex8_grads(A.gdN, x, y, z, ip, a, b, ga, gb);
             if (detJ < 0) bad = 1;
-            double f = detJ * c_w[ip] * A.factor;
+            double f = detJ * A.gw[ip] * A.factor;
             K[0] += (gb[0] * k11 * ga[0] + gb[1] * k22 * ga[1] + gb[2] * k33 * ga[2]) * f;
-            K2[0] += c_N[ip * 8 + a] * c_N[ip * 8 + b] * f * cap;
+            K2[0] += A.gN[ip * 8 + a] * A.gN[ip * 8 + b] * f * cap;
         }
         if (bad && A.flags) atomicOr(&A.flags[0], 1);
     }
@@ -403,19 +400,9 @@ __global__ void k_restrained_diag(int n_dof, const unsigned char *rest, const in
     if (val_aux) val_aux[rowptr[i]] = 1.0;
 }
 
-// Upload the Gauss table for the element family (gauss_tables.hpp).
+// Gauss tables of the context (weights, shape derivatives and values).
 int upload_gauss(bhip_ctx *ctx, int nnpe, int order, int *nip_out) {
-    std::vector<double> hw, hdN, hN;
-    int nip = bhip_gauss::tables(nnpe, order, kMaxIp, hw, hdN, hN);
-    if (nip == 0) return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "element family or integration order not built");
-    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_w), hw.data(), sizeof(double) * kMaxIp, 0,
-                                         hipMemcpyHostToDevice, ctx->stream));
-    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dN), hdN.data(), sizeof(double) * kMaxIp * 24, 0,
-                                         hipMemcpyHostToDevice, ctx->stream));
-    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_N), hN.data(), sizeof(double) * kMaxIp * 8, 0,
-                                         hipMemcpyHostToDevice, ctx->stream));
-    *nip_out = nip;
-    return BHIP_OK;
+    return bhip_gauss_upload(ctx, nnpe, order, nip_out);
 }
 
 template <class P>
@@ -456,6 +443,9 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     A.mat_stride = ctx->mat_stride;
     A.nnpe = ctx->nnpe;
     A.nip = nip;
+    A.gw = ctx->gauss.p;
+    A.gdN = ctx->gauss.p + kGaussDN;
+    A.gN = ctx->gauss.p + kGaussN;
     A.factor = ctx->factor;
     A.tan = nullptr;
     A.flags = ctx->flags.p;
@@ -643,7 +633,7 @@ int lift_build(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipGetLastError());
     BHIP_PROPAGATE(bhip_scan_i64(ctx, cnt.p, ctx->lift_ptr.p, (long long)nn + 1));
     long long total = 0;
-    BHIP_TRY(ctx, hipMemcpy(&total, ctx->lift_ptr.p + nn, sizeof(long long), hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, &total, ctx->lift_ptr.p + nn, sizeof(long long), hipMemcpyDeviceToHost));
     ctx->n_lift = total;
     const size_t ne = total > 0 ? (size_t)total : 1;
     const int ND = hex ? 3 : 2;
@@ -665,6 +655,9 @@ int lift_build(bhip_ctx *ctx) {
         A.mat_stride = ctx->mat_stride;
         A.nnpe = ctx->nnpe;
         A.nip = nip;
+        A.gw = ctx->gauss.p;
+        A.gdN = ctx->gauss.p + kGaussDN;
+        A.gN = ctx->gauss.p + kGaussN;
         A.factor = ctx->factor;
         A.tan = nullptr;
         A.flags = nullptr;
@@ -702,5 +695,32 @@ int bhip_lift_rhs(bhip_ctx *ctx, int R, const double *presc, const double *load,
             ctx->n_nodes, R, ctx->n_dof, ctx->dof_first.p, ctx->node_rmask.p, ctx->lift_ptr.p, ctx->lift_col.p,
             ctx->lift_val.p, presc, load, b);
     BHIP_TRY(ctx, hipGetLastError());
+    return BHIP_OK;
+}
+
+// Gauss tables from the reference's Gauss-Legendre construction
+// (gauss.cpp:14-60, 303-365, 825-877), kept per context.
+int bhip_gauss_upload(bhip_ctx *ctx, int nnpe, int order, int *nip_out) {
+    if (ctx->gauss.p && ctx->gauss_nnpe == nnpe && ctx->gauss_order == order) {
+        *nip_out = ctx->gauss_nip;
+        return BHIP_OK;
+    }
+    std::vector<double> hw, hdN, hN;
+    const int nip = bhip_gauss::tables(nnpe, order, kMaxIp, hw, hdN, hN);
+    if (nip == 0) return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "element family or integration order not built");
+    std::vector<double> all((size_t)kGaussN + kMaxIp * 8, 0.0);
+    std::copy(hw.begin(), hw.begin() + kMaxIp, all.begin());
+    std::copy(hdN.begin(), hdN.begin() + kMaxIp * 24, all.begin() + kGaussDN);
+    std::copy(hN.begin(), hN.begin() + kMaxIp * 8, all.begin() + kGaussN);
+    // earlier kernels of this context may still read the old tables
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    BHIP_TRY(ctx, ctx->gauss.resize(all.size()));
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->gauss.p, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->gauss_nnpe = nnpe;
+    ctx->gauss_order = order;
+    ctx->gauss_nip = nip;
+    *nip_out = nip;
     return BHIP_OK;
 }

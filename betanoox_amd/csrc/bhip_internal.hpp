@@ -118,6 +118,10 @@ struct bhip_ctx {
     bool have_mat = false;
     int order = 2;
     double factor = 1.0;
+    // Gauss weights | shape derivatives | shape values of (gauss_nnpe, gauss_order),
+    // per context so that contexts on concurrent streams never share them
+    dvec<double> gauss;
+    int gauss_nnpe = 0, gauss_order = 0, gauss_nip = 0;
     bool assembled = false;
     int asm_kind = 0;
     // ---- lift table of the last stiffness assembly: for every node with free
@@ -145,6 +149,14 @@ struct bhip_ctx {
     // ---- multi-case solve buffers and graph (multi.hip)
     struct MultiData *multi = nullptr;
 };
+
+// Blocking copy ordered on the context's stream.  Context streams are
+// non-blocking, so the legacy-stream hipMemcpy would neither wait for the
+// context's kernels nor be allowed while another context captures a graph.
+inline hipError_t bhip_copy(bhip_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(ctx->stream);
+}
 
 // -------------------------------------------------------------- launching
 inline int grid_for(long long n, int block, int max_blocks = 1 << 30) {
@@ -199,6 +211,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *smem /* 16*NV
 
 // --------------------------------------------------------- internal entry
 int bhip_ensure_vectors(bhip_ctx *ctx);
+// New coordinates for the current mesh; pattern and restraints stay.
+int bhip_update_coords(bhip_ctx *ctx, const double *xyz);
 int bhip_pattern_build(bhip_ctx *ctx);
 int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double *u_state);
 int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *opts, bhip_stats *stats);
@@ -209,6 +223,11 @@ void bhip_invalidate_graphs(bhip_ctx *ctx);
 void bhip_plastic_release(bhip_ctx *ctx);
 void bhip_multi_invalidate(bhip_ctx *ctx);
 void bhip_multi_release(bhip_ctx *ctx);
+// Gauss tables of (nnpe, order) in ctx->gauss (uploaded when they change):
+// w = gauss, dN = gauss + kGaussDN, N = gauss + kGaussN.
+constexpr int kGaussDN = kMaxIp;
+constexpr int kGaussN = kMaxIp + kMaxIp * 24;
+int bhip_gauss_upload(bhip_ctx *ctx, int nnpe, int order, int *nip_out);
 int bhip_scan_i64(bhip_ctx *ctx, const long long *in, long long *out, long long n);
 // Right-hand sides of R load cases from the lift table (built on first use
 // after a stiffness assembly): b_r = restrained ? presc_r : load + (-K_fr presc_r).

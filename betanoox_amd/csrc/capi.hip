@@ -88,6 +88,7 @@ void bhip_destroy(bhip_ctx *ctx) {
     ctx->lift_row.release();
     ctx->lift_col.release();
     ctx->lift_val.release();
+    ctx->gauss.release();
     ctx->state.release();
     if (ctx->h_state) (void)hipHostFree(ctx->h_state);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -174,6 +175,21 @@ int bhip_get_dof_first(bhip_ctx *ctx, int *dof_first) {
     return BHIP_OK;
 }
 
+}  // extern "C"
+
+int bhip_update_coords(bhip_ctx *ctx, const double *xyz) {
+    if (!ctx || !xyz) return BHIP_ERR_ARG;
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
+    (void)hipSetDevice(ctx->device);
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->xyz.p, xyz, sizeof(double) * ctx->n_nodes * ctx->dim, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->assembled = ctx->f32_ready = ctx->lift_ready = false;
+    return BHIP_OK;
+}
+
+extern "C" {
+
 int bhip_set_restraints(bhip_ctx *ctx, const unsigned char *restrained, const double *values) {
     if (!ctx || !restrained) return BHIP_ERR_ARG;
     if (ctx->n_dof == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
@@ -234,9 +250,9 @@ int bhip_get_csr(bhip_ctx *ctx, int *rowptr, int *colind, double *val) {
     if (!ctx) return BHIP_ERR_ARG;
     if (!ctx->pattern_ready) return bhip_fail(ctx, BHIP_ERR_STATE, "pattern not built");
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (rowptr) BHIP_TRY(ctx, hipMemcpy(rowptr, ctx->rowptr.p, sizeof(int) * (ctx->n_dof + 1), hipMemcpyDeviceToHost));
-    if (colind) BHIP_TRY(ctx, hipMemcpy(colind, ctx->colind.p, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost));
-    if (val) BHIP_TRY(ctx, hipMemcpy(val, ctx->val.p, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost));
+    if (rowptr) BHIP_TRY(ctx, bhip_copy(ctx, rowptr, ctx->rowptr.p, sizeof(int) * (ctx->n_dof + 1), hipMemcpyDeviceToHost));
+    if (colind) BHIP_TRY(ctx, bhip_copy(ctx, colind, ctx->colind.p, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost));
+    if (val) BHIP_TRY(ctx, bhip_copy(ctx, val, ctx->val.p, sizeof(double) * ctx->nnz, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }
 
@@ -312,7 +328,7 @@ int bhip_assemble(bhip_ctx *ctx, int what, const double *coef, const double *u_s
 int bhip_get_pseudo_load(bhip_ctx *ctx, double *f) {
     if (!ctx || !f) return BHIP_ERR_ARG;
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    BHIP_TRY(ctx, hipMemcpy(f, ctx->pseudo.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, f, ctx->pseudo.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }
 
@@ -353,7 +369,7 @@ int bhip_build_rhs(bhip_ctx *ctx) {
 int bhip_get_rhs(bhip_ctx *ctx, double *b) {
     if (!ctx || !b) return BHIP_ERR_ARG;
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    BHIP_TRY(ctx, hipMemcpy(b, ctx->rhs.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, b, ctx->rhs.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }
 
@@ -381,7 +397,7 @@ int bhip_solve(bhip_ctx *ctx, const bhip_solve_opts *opts, bhip_stats *stats) {
 int bhip_get_solution(bhip_ctx *ctx, double *x) {
     if (!ctx || !x) return BHIP_ERR_ARG;
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    BHIP_TRY(ctx, hipMemcpy(x, ctx->x.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, x, ctx->x.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }
 
@@ -394,7 +410,7 @@ int bhip_pcg_solve(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_
     if (x0) BHIP_PROPAGATE(bhip_set_x0(ctx, x0));
     int st = bhip_solve_impl(ctx, &o, stats);
     if (st != BHIP_OK && st != BHIP_ERR_NOT_CONVERGED) return st;
-    BHIP_TRY(ctx, hipMemcpy(b, ctx->x.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, b, ctx->x.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
     return st;
 }
 
@@ -405,7 +421,7 @@ int bhip_spmv(bhip_ctx *ctx, const double *x, double *y) {
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->tmp.p, x, sizeof(double) * ctx->n_dof, hipMemcpyHostToDevice, ctx->stream));
     BHIP_PROPAGATE(bhip_spmv_dev(ctx, ctx->tmp.p, ctx->q.p));
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    BHIP_TRY(ctx, hipMemcpy(y, ctx->q.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, y, ctx->q.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }
 

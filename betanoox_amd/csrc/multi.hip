@@ -669,7 +669,7 @@ extern "C" int bhip_solve_cases(bhip_ctx *ctx, int nrhs, const double *presc, co
     std::vector<double> rr, bb;
     BHIP_PROPAGATE(outer_residuals(ctx, M, nrhs, rr, bb));
     for (int r = 0; r < nrhs; r++) S.rel_resid = std::max(S.rel_resid, bb[r] > 0 ? sqrt(rr[r] / bb[r]) : sqrt(rr[r]));
-    BHIP_TRY(ctx, hipMemcpy(x, M.x64.p, sizeof(double) * n * nrhs, hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, x, M.x64.p, sizeof(double) * n * nrhs, hipMemcpyDeviceToHost));
     if (stats) *stats = S;
     if (st == BHIP_ERR_NOT_CONVERGED)
         return bhip_fail(ctx, st, "multi-case solve not converged within the iteration caps");

@@ -246,7 +246,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     }
     keys2.release();
     int h_nsel = 0;
-    BHIP_TRY(ctx, hipMemcpy(&h_nsel, nsel.p, sizeof(int), hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, &h_nsel, nsel.p, sizeof(int), hipMemcpyDeviceToHost));
     ctx->n_pairs = h_nsel;
     long long np = ctx->n_pairs;
     BHIP_TRY(ctx, ctx->pair_row.resize(np > 0 ? np : 1));
@@ -272,7 +272,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipGetLastError());
     BHIP_PROPAGATE(exclusive_scan(ctx, rowlen.p, rowscan.p, nd + 1));
     long long h_nnz = 0;
-    BHIP_TRY(ctx, hipMemcpy(&h_nnz, rowscan.p + nd, sizeof(long long), hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, &h_nnz, rowscan.p + nd, sizeof(long long), hipMemcpyDeviceToHost));
     if (h_nnz >= (1LL << 31) - 1)
         return bhip_fail(ctx, BHIP_ERR_CAPACITY,
                          "pattern has " + std::to_string(h_nnz) +
@@ -293,7 +293,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     k_lower_len<<<grid_for(nd, B), B, 0, s>>>(nd, ctx->rowptr.p, ctx->diag_pos.p, rowlen.p);
     BHIP_PROPAGATE(exclusive_scan(ctx, rowlen.p, rowscan.p, nd + 1));
     long long h_low = 0;
-    BHIP_TRY(ctx, hipMemcpy(&h_low, rowscan.p + nd, sizeof(long long), hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, &h_low, rowscan.p + nd, sizeof(long long), hipMemcpyDeviceToHost));
     ctx->nnz_lower = h_low;
 
     BHIP_TRY(ctx, ctx->val.resize(h_nnz > 0 ? h_nnz : 1));
@@ -322,7 +322,7 @@ int bhip_export_lower(bhip_ctx *ctx, int *rowptr, int *colind, double *val) {
     k_lower_len<<<grid_for(nd, B), B, 0, s>>>(nd, ctx->rowptr.p, ctx->diag_pos.p, len.p);
     BHIP_PROPAGATE(exclusive_scan(ctx, len.p, scan.p, nd + 1));
     std::vector<long long> h(nd + 1);
-    BHIP_TRY(ctx, hipMemcpy(h.data(), scan.p, sizeof(long long) * (nd + 1), hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, h.data(), scan.p, sizeof(long long) * (nd + 1), hipMemcpyDeviceToHost));
     if (rowptr)
         for (int i = 0; i <= nd; i++) rowptr[i] = (int)h[i];
     long long nl = h[nd];
@@ -334,7 +334,7 @@ int bhip_export_lower(bhip_ctx *ctx, int *rowptr, int *colind, double *val) {
                                                colind ? dcol.p : nullptr, val ? dval.p : nullptr);
     BHIP_TRY(ctx, hipGetLastError());
     BHIP_TRY(ctx, hipStreamSynchronize(s));
-    if (colind) BHIP_TRY(ctx, hipMemcpy(colind, dcol.p, sizeof(int) * nl, hipMemcpyDeviceToHost));
-    if (val) BHIP_TRY(ctx, hipMemcpy(val, dval.p, sizeof(double) * nl, hipMemcpyDeviceToHost));
+    if (colind) BHIP_TRY(ctx, bhip_copy(ctx, colind, dcol.p, sizeof(int) * nl, hipMemcpyDeviceToHost));
+    if (val) BHIP_TRY(ctx, bhip_copy(ctx, val, dval.p, sizeof(double) * nl, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }

@@ -24,9 +24,6 @@
 #include "bhip_internal.hpp"
 #include "gauss_tables.hpp"
 
-// This translation unit's Gauss tables (gauss_tables.hpp).
-__constant__ double c_w[kMaxIp];
-__constant__ double c_dN[kMaxIp * 24];
 
 struct PlasticData {
     int nip = 0;
@@ -114,9 +111,9 @@ __device__ __forceinline__ double dot6(const double *a, const double *b) {
 }
 
 // All 8 global gradients at ip (isoElement.cpp:324-372).
-__device__ __forceinline__ double hex8_all_grads(const double *x, const double *y, const double *z, int ip,
-                                                 double (*g)[3]) {
-    const double *dxi = c_dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
+__device__ __forceinline__ double hex8_all_grads(const double *dN, const double *x, const double *y,
+                                                 const double *z, int ip, double (*g)[3]) {
+    const double *dxi = dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
     double px1 = 0, px2 = 0, px3 = 0, py1 = 0, py2 = 0, py3 = 0, pz1 = 0, pz2 = 0, pz3 = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -163,9 +160,9 @@ __device__ __forceinline__ int group_of(const int *elem_group, int n_groups, int
 }
 
 // "V": strain increment + return map (getStressIncrement, plastic.cpp:368-465).
-__global__ void k_plastic_update(int n_el, int nip, const int *conn, const double *xyz, const double *mat,
-                                 const int *elem_group, int n_groups, const double *du, double *isv,
-                                 double *tan) {
+__global__ void k_plastic_update(int n_el, int nip, const double *gtab, const int *conn, const double *xyz,
+                                 const double *mat, const int *elem_group, int n_groups, const double *du,
+                                 double *isv, double *tan) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n_el * nip) return;
     int e = (int)(t / nip), ip = (int)(t - (long long)e * nip);
@@ -175,7 +172,7 @@ __global__ void k_plastic_update(int n_el, int nip, const int *conn, const doubl
     double x[8], y[8], z[8];
     load_coords(xyz, ec, x, y, z);
     double g[8][3];
-    hex8_all_grads(x, y, z, ip, g);
+    hex8_all_grads(gtab + kGaussDN, x, y, z, ip, g);
     // ElasticityElement3D::calculateStrain (engineering shear)
     double de[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -229,7 +226,8 @@ __global__ void k_plastic_update(int n_el, int nip, const int *conn, const doubl
 
 // "F" + calculateResidualNorm: per node, resultant of B^T dstress over the
 // incident elements (ascending), residual = load - resultant on free DOFs.
-__global__ void k_plastic_residual(int n_nodes, int nip, const int *n2e_ptr, const int *n2e, const int *conn,
+__global__ void k_plastic_residual(int n_nodes, int nip, const double *gtab, const int *n2e_ptr, const int *n2e,
+                                   const int *conn,
                                    const double *xyz, const double *isv, const unsigned char *rmask,
                                    const double *load, double *resid) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,8 +245,8 @@ __global__ void k_plastic_residual(int n_nodes, int nip, const int *n2e_ptr, con
         load_coords(xyz, ec, x, y, z);
         for (int ip = 0; ip < nip; ip++) {
             double g[8][3];
-            double detJ = hex8_all_grads(x, y, z, ip, g);
-            double f = detJ * c_w[ip];
+            double detJ = hex8_all_grads(gtab + kGaussDN, x, y, z, ip, g);
+            double f = detJ * gtab[ip];
             const double *s = isv + ((long long)e * nip + ip) * kIsv + 7;
             // ElasticityElement3D::addToFe: force += sum_j b[j][i] stress[j] * f
             acc[0] += (g[a][0] * s[0] + g[a][1] * s[3] + g[a][2] * s[5]) * f;
@@ -325,13 +323,9 @@ __global__ void k_plastic_commit(long long npts, double *isv) {
 
 int plastic_check(bhip_ctx *ctx) {
     if (!ctx->plastic) return bhip_fail(ctx, BHIP_ERR_STATE, "plastic state not initialised (bhip_plastic_init)");
-    std::vector<double> hw, hdN, hN;
-    int nip = bhip_gauss::tables(8, ctx->order, kMaxIp, hw, hdN, hN);
+    int nip = 0;
+    BHIP_PROPAGATE(bhip_gauss_upload(ctx, 8, ctx->order, &nip));
     if (nip != ctx->plastic->nip) return bhip_fail(ctx, BHIP_ERR_STATE, "integration order changed after bhip_plastic_init");
-    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_w), hw.data(), sizeof(double) * kMaxIp, 0,
-                                         hipMemcpyHostToDevice, ctx->stream));
-    BHIP_TRY(ctx, hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dN), hdN.data(), sizeof(double) * kMaxIp * 24, 0,
-                                         hipMemcpyHostToDevice, ctx->stream));
     return BHIP_OK;
 }
 
@@ -339,7 +333,7 @@ int plastic_update_dev(bhip_ctx *ctx, const double *du) {
     PlasticData *P = ctx->plastic;
     long long npts = (long long)ctx->n_el * P->nip;
     k_plastic_update<<<grid_for(npts, 128), 128, 0, ctx->stream>>>(
-        ctx->n_el, P->nip, ctx->conn.p, ctx->xyz.p, ctx->mat.p, ctx->elem_group.n ? ctx->elem_group.p : nullptr,
+        ctx->n_el, P->nip, ctx->gauss.p, ctx->conn.p, ctx->xyz.p, ctx->mat.p, ctx->elem_group.n ? ctx->elem_group.p : nullptr,
         ctx->n_groups, du, P->isv.p, P->tan.p);
     BHIP_TRY(ctx, hipGetLastError());
     return BHIP_OK;
@@ -348,7 +342,7 @@ int plastic_update_dev(bhip_ctx *ctx, const double *du) {
 int plastic_residual_dev(bhip_ctx *ctx, double *max_abs) {
     PlasticData *P = ctx->plastic;
     k_plastic_residual<<<grid_for(ctx->n_nodes, 128), 128, 0, ctx->stream>>>(
-        ctx->n_nodes, P->nip, ctx->n2e_ptr.p, ctx->n2e.p, ctx->conn.p, ctx->xyz.p, P->isv.p, ctx->node_rmask.p,
+        ctx->n_nodes, P->nip, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, ctx->conn.p, ctx->xyz.p, P->isv.p, ctx->node_rmask.p,
         ctx->load.p, ctx->rhs.p);
     int g = std::min(1024, grid_for(ctx->n_dof, 256));
     k_max_abs<<<g, 256, 0, ctx->stream>>>(ctx->n_dof, ctx->rhs.p, P->red.p, P->cnt.p, P->red.p + 1024);
@@ -423,7 +417,7 @@ int bhip_plastic_residual(bhip_ctx *ctx, double *residual, double *max_abs) {
     BHIP_PROPAGATE(plastic_check(ctx));
     (void)hipSetDevice(ctx->device);
     BHIP_PROPAGATE(plastic_residual_dev(ctx, max_abs));
-    if (residual) BHIP_TRY(ctx, hipMemcpy(residual, ctx->rhs.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    if (residual) BHIP_TRY(ctx, bhip_copy(ctx, residual, ctx->rhs.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }
 
@@ -489,7 +483,7 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
     std::vector<double> flags(npts);
     {
         std::vector<double> rec(npts * kIsv);
-        BHIP_TRY(ctx, hipMemcpy(rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
+        BHIP_TRY(ctx, bhip_copy(ctx, rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
         for (long long t = 0; t < npts; t++) {
             S.plastic_points += rec[t * kIsv + 14] != 0.0;
             S.max_return_iters = std::max(S.max_return_iters, (int)rec[t * kIsv + 15]);
@@ -509,14 +503,14 @@ int bhip_plastic_get_state(bhip_ctx *ctx, double *stress, double *eps_bar, doubl
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (stress || eps_bar) {
         std::vector<double> rec(npts * kIsv);
-        BHIP_TRY(ctx, hipMemcpy(rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
+        BHIP_TRY(ctx, bhip_copy(ctx, rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
         for (long long t = 0; t < npts; t++) {
             if (stress)
                 for (int i = 0; i < 6; i++) stress[t * 6 + i] = rec[t * kIsv + i];
             if (eps_bar) eps_bar[t] = rec[t * kIsv + 6];
         }
     }
-    if (u_total) BHIP_TRY(ctx, hipMemcpy(u_total, P->utot.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
+    if (u_total) BHIP_TRY(ctx, bhip_copy(ctx, u_total, P->utot.p, sizeof(double) * ctx->n_dof, hipMemcpyDeviceToHost));
     return BHIP_OK;
 }
 

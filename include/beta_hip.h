@@ -223,6 +223,77 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
  * accumulated displacement (n_dof); any pointer may be NULL. */
 int bhip_plastic_get_state(bhip_ctx *ctx, double *stress, double *eps_bar, double *u_total);
 
+/* --------------------------------------------------------------- serving */
+/* No counterpart in the reference, which runs one model at a time on the
+ * host.  The served workflow is the reference's macro-element / RVE one
+ * (MultiFieldMacroElement::CalculateK, MultiFieldMacroElement.cpp:302-407):
+ * assemble a hex8 elasticity mesh, then solve it for several prescribed
+ * boundary-displacement load cases (bhip_solve_cases).  The server keeps up to
+ * max_slots requests resident on one GPU, each in its own context and HIP
+ * stream, starts queued requests in arrival order as soon as a slot is free
+ * and the requests' estimated device bytes fit the budget (continuous
+ * batching), and reuses a slot's pattern when the next request has the same
+ * connectivity and restraints.  A request's result does not depend on the
+ * other requests in flight. */
+typedef struct bhip_server bhip_server;
+
+typedef struct bhip_server_opts {
+    int    max_slots;    /* requests on the device at once (0: 4; at most 16) */
+    double hbm_budget;   /* device bytes all slots may hold (0: 80% of free memory at creation) */
+    double tol;          /* per case |b - K x| / |b| (0: 1e-8) */
+    double inner_tol;    /* FP32_IR inner residual reduction (0: 1e-3) */
+    int    maxit;        /* inner iteration cap per solve (0: 40000) */
+    int    precision;    /* BHIP_PREC_FP32_IR (default) or BHIP_PREC_FP64 */
+} bhip_server_opts;
+
+/* One hex8 3D elasticity request (3 DOFs per node).  Every array is copied
+ * by bhip_server_submit. */
+typedef struct bhip_request {
+    int n_nodes;
+    const double *xyz;            /* 3 * n_nodes */
+    int n_el;
+    const int *conn;              /* 8 * n_el, reference node order */
+    const unsigned char *rest;    /* 3 * n_nodes, nonzero = prescribed DOF */
+    int mat_kind;                 /* BHIP_MAT_ISO_ELASTIC or BHIP_MAT_CMAT3D */
+    int n_groups;
+    const double *mat;            /* n_groups records */
+    const int *elem_group;        /* n_el, or NULL (as bhip_set_materials) */
+    int n_cases;                  /* load cases (prescribed-value sets) */
+    const double *presc;          /* n_cases * 3 * n_nodes */
+    const double *load;           /* 3 * n_nodes shared by all cases, or NULL */
+    int order;                    /* Gauss points per direction (0: 2) */
+} bhip_request;
+
+typedef struct bhip_request_stats {
+    int    status;           /* BHIP_OK or the failing call's status */
+    int    slot;             /* slot that ran the request */
+    int    reused_pattern;   /* 1 if the slot's pattern was reused */
+    int    iters;            /* inner iterations over the case groups */
+    int    outer_iters;
+    double rel_resid;        /* worst true relative residual over the cases */
+    double queue_ms;         /* submit -> start */
+    double service_ms;       /* start -> done */
+    double latency_ms;       /* submit -> done */
+    double start_ms, end_ms; /* start and end, ms since the server was created */
+    char   message[256];     /* error text when status != BHIP_OK */
+} bhip_request_stats;
+
+int  bhip_server_create(int device, const bhip_server_opts *opts, bhip_server **out);
+/* Stops the workers after the requests they are running; queued requests are
+ * dropped.  No call may wait on the server while it is destroyed. */
+void bhip_server_destroy(bhip_server *srv);
+/* Queues a copy of the request; BHIP_ERR_CAPACITY if its estimated device
+ * bytes exceed the budget. */
+int  bhip_server_submit(bhip_server *srv, const bhip_request *req, long long *id);
+int  bhip_server_poll(bhip_server *srv, long long id, int *done);
+/* Blocks until request `id` is done, copies its solutions (n_cases * 3 *
+ * n_nodes, case-major; x may be NULL) and stats, and forgets the request.
+ * Returns the request's status. */
+int  bhip_server_wait(bhip_server *srv, long long id, double *x, bhip_request_stats *stats);
+double bhip_server_budget(const bhip_server *srv);
+/* The device-byte estimate admission uses for a request of this size. */
+double bhip_request_footprint(int n_nodes, int n_el, int n_cases);
+
 #ifdef __cplusplus
 }
 #endif
