@@ -76,6 +76,70 @@ def cpu_baseline(sample_n, budget_s):
     }
 
 
+def served_stream(count, slots, rate_frac, probe_count, seed=None):
+    """Workload 5 on a bounded prefix of the seeded 4,096-request stream: RVE
+    requests (n = round(exp(U(ln 16, ln 96))), 5-40 inclusions, six affine
+    load cases each), solved by the resident server to 1e-8.  A closed-loop
+    probe (all `probe_count` requests of a separate draw queued at once)
+    measures capacity; the stream then arrives as a Poisson process at
+    `rate_frac` of that capacity.  Requests are generated before the clock
+    starts."""
+    from betanoox_amd import capi, meshgen
+    seed = meshgen.SEED_BASE + 5 if seed is None else seed
+    rng = np.random.default_rng(seed)
+    total = 4096
+    ns = np.rint(np.exp(rng.uniform(np.log(16), np.log(96), total))).astype(int)
+    seeds = rng.integers(0, 2 ** 31 - 1, total)
+    gaps = rng.exponential(1.0, total)
+    prng = np.random.default_rng(seed + 1000)
+    pns = np.rint(np.exp(prng.uniform(np.log(16), np.log(96), probe_count))).astype(int)
+    probe = [meshgen.rve_request(int(n), int(s)) for n, s in zip(pns, prng.integers(0, 2 ** 31 - 1, probe_count))]
+    reqs = [meshgen.rve_request(int(n), int(sd)) for n, sd in zip(ns[:count], seeds[:count])]
+    out = {"requests": count, "of_stream": total, "slots": slots, "cases_per_request": 6, "tol": 1e-8,
+           "n_min": int(ns[:count].min()), "n_max": int(ns[:count].max()),
+           "dof_total": int(sum(r.n_dof for r in reqs))}
+    with capi.Server(0, max_slots=slots, tol=1e-8) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(r) for r in probe]
+        pst = [srv.wait(i)[1] for i in ids]
+        t_probe = time.perf_counter() - t0
+        cap = probe_count / t_probe
+        rate = rate_frac * cap
+        log("served probe: %d requests in %.2f s -> capacity %.2f req/s; stream rate %.2f req/s"
+            % (probe_count, t_probe, cap, rate))
+        t0 = time.perf_counter()
+        t_next = 0.0
+        ids = []
+        for k, r in enumerate(reqs):
+            t_next += gaps[k] / rate
+            dt = t0 + t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(r))
+        res = []
+        for k, i in enumerate(ids):
+            X, st = srv.wait(i)
+            res.append(st)
+            if st["rel_resid"] > 1e-8:
+                raise RuntimeError("served request %d above tolerance: %g" % (k, st["rel_resid"]))
+        t_end = time.perf_counter()
+    lat = np.array([r["latency_ms"] for r in res])
+    span = t_end - t0
+    dof_iter = sum(rq.n_dof * st["iters"] for rq, st in zip(reqs, res))
+    out.update({
+        "arrival": "poisson", "rate_req_per_s": rate, "capacity_probe_req_per_s": cap,
+        "solves_per_s": count / span, "case_solves_per_s": 6 * count / span,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+        "queue_p50_ms": float(np.percentile([r["queue_ms"] for r in res], 50)),
+        "service_p50_ms": float(np.percentile([r["service_ms"] for r in res], 50)),
+        "dof_iter_per_s": dof_iter / span, "wall_s": span,
+        "worst_rel_resid": max(r["rel_resid"] for r in res),
+        "pattern_reuse": sum(r["reused_pattern"] for r in res),
+    })
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +151,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--served", type=int, default=128, help="requests of the served stream (0: skip)")
+    ap.add_argument("--served-slots", type=int, default=4)
+    ap.add_argument("--served-load", type=float, default=0.7, help="arrival rate / probed capacity")
+    ap.add_argument("--served-probe", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (a single solve "
@@ -184,6 +252,9 @@ def main():
         line["speedup_vs_fp64_dof_iter"] = value / result_fp64["dof_iter_per_s"]
         line["speedup_vs_fp64_time_to_solution"] = result_fp64["ms"] / (1e3 * dt / args.steps)
     ctx.close()
+    if args.served > 0:
+        line["served"] = served_stream(args.served, args.served_slots, args.served_load, args.served_probe)
+        log("served: %s" % json.dumps(line["served"]))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_budget)
         line["cpu_baseline"]["cores"] = 1

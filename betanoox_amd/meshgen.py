@@ -201,42 +201,74 @@ def plastic_bar(n=64, E=200.0, nu=0.3, sigma0=0.25, H=2.0, strain=0.005, increme
                           "sigma0": sigma0, "H": H})
 
 
-def rve_request(n, seed, n_incl=None, r_range=(0.05, 0.15)):
-    """One request of workload 5: n^3 unit cube with spherical inclusions
-    (E=70, nu=0.22 inside, E=3, nu=0.35 outside, by element centroid).
-    Draws: number of inclusions (5..40) unless given, centres (U(0,1)^3),
-    radii (uniform in r_range).  Every boundary DOF is prescribed; the six
-    load cases are the affine fields u = eps . x of the unit macro-strains
-    (xx, yy, zz, xy, yz, xz with engineering shear split symmetrically)."""
-    rng = np.random.default_rng(seed)
-    if n_incl is None:
-        n_incl = int(rng.integers(5, 41))
-    cen = rng.uniform(0.0, 1.0, (n_incl, 3))
-    rad = rng.uniform(r_range[0], r_range[1], n_incl)
+_RVE_BASE = {}
+
+
+def _rve_base(n):
+    """Geometry shared by every RVE request of size n (cached): nodes,
+    connectivity, element centroids as an (n, n, n, 3) [k][j][i] view,
+    restraints and the six affine load cases."""
+    b = _RVE_BASE.get(n)
+    if b is not None:
+        return b
     xyz, conn = structured_hex8(n, n, n)
     nn = xyz.shape[0]
     cent = xyz[conn].mean(axis=1)
-    inside = np.zeros(conn.shape[0], bool)
-    for c, r in zip(cen, rad):
-        inside |= np.sum((cent - c) ** 2, axis=1) < r * r
-    mats = np.array([[3.0, 0.35], [70.0, 0.22]])
-    group = inside.astype(np.int32)
     bnd = boundary_mask_hex(n, n, n)
     rest = np.zeros(3 * nn, np.uint8)
     for c in range(3):
         rest[3 * np.where(bnd)[0] + c] = 1
+    rb = rest.astype(bool)
     cases = []
     for k in range(6):
         eps = np.zeros((3, 3))
         if k < 3:
             eps[k, k] = 1.0
         else:
-            a, b = [(0, 1), (1, 2), (0, 2)][k - 3]
-            eps[a, b] = eps[b, a] = 0.5
+            a, bb = [(0, 1), (1, 2), (0, 2)][k - 3]
+            eps[a, bb] = eps[bb, a] = 0.5
         u = xyz @ eps.T
         pv = np.zeros(3 * nn)
-        pv[rest.astype(bool)] = u.ravel()[rest.astype(bool)]
+        pv[rb] = u.ravel()[rb]
+        pv.flags.writeable = False
         cases.append(pv)
+    for a in (xyz, conn, rest):
+        a.flags.writeable = False
+    if len(_RVE_BASE) >= 16:
+        _RVE_BASE.clear()
+    b = {"xyz": xyz, "conn": conn, "cent": cent.reshape(n, n, n, 3), "rest": rest, "cases": cases}
+    _RVE_BASE[n] = b
+    return b
+
+
+def rve_request(n, seed, n_incl=None, r_range=(0.05, 0.15)):
+    """One request of workload 5: n^3 unit cube with spherical inclusions
+    (E=70, nu=0.22 inside, E=3, nu=0.35 outside, by element centroid).
+    Draws: number of inclusions (5..40) unless given, centres (U(0,1)^3),
+    radii (uniform in r_range).  Every boundary DOF is prescribed; the six
+    load cases are the affine fields u = eps . x of the unit macro-strains
+    (xx, yy, zz, xy, yz, xz with engineering shear split symmetrically).
+    Geometry, restraints and load cases are shared (read-only) between
+    requests of the same n; each inclusion tests only the elements in its
+    bounding box (one element of margin), which gives the same mask as
+    testing every centroid."""
+    rng = np.random.default_rng(seed)
+    if n_incl is None:
+        n_incl = int(rng.integers(5, 41))
+    cen = rng.uniform(0.0, 1.0, (n_incl, 3))
+    rad = rng.uniform(r_range[0], r_range[1], n_incl)
+    b = _rve_base(n)
+    cent = b["cent"]
+    inside = np.zeros((n, n, n), bool)  # [k][j][i]
+    for c, r in zip(cen, rad):
+        lo = np.clip(np.floor((c - r) * n).astype(int) - 1, 0, n)
+        hi = np.clip(np.ceil((c + r) * n).astype(int) + 1, 0, n)
+        sub = cent[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+        inside[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] |= np.sum((sub - c) ** 2, axis=-1) < r * r
+    mats = np.array([[3.0, 0.35], [70.0, 0.22]])
+    group = inside.ravel().astype(np.int32)
+    nn = b["xyz"].shape[0]
     from . import capi
-    return Problem("rve_hex8_%d" % n, 3, xyz, conn, 3, capi.MAT_ISO_ELASTIC, mats, rest, cases[0],
-                   np.zeros(3 * nn), order=2, extra={"cases": cases, "group": group, "n_incl": n_incl})
+    return Problem("rve_hex8_%d" % n, 3, b["xyz"], b["conn"], 3, capi.MAT_ISO_ELASTIC, mats, b["rest"],
+                   b["cases"][0], np.zeros(3 * nn), order=2,
+                   extra={"cases": b["cases"], "group": group, "n_incl": n_incl})
