@@ -40,6 +40,10 @@ STOP_REL_RESID = 1
 PREC_FP64 = 0
 PREC_FP32_IR = 1
 
+HEAT_X0_ZERO = 0
+HEAT_X0_PREVIOUS = 1
+HEAT_X0_REFERENCE = 2
+
 # every symbol the header declares
 EXPORTS = [
     "bhip_create", "bhip_destroy", "bhip_last_error", "bhip_version", "bhip_set_mesh",
@@ -51,7 +55,7 @@ EXPORTS = [
     "bhip_bench_spmv", "bhip_bench_stream", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
     "bhip_server_create", "bhip_server_destroy", "bhip_server_submit", "bhip_server_poll", "bhip_server_wait",
-    "bhip_server_budget", "bhip_request_footprint",
+    "bhip_server_budget", "bhip_request_footprint", "bhip_heat_run",
 ]
 
 
@@ -93,6 +97,14 @@ class RequestStats(C.Structure):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "message"}
         d["message"] = self.message.decode(errors="replace")
         return d
+
+
+class HeatStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("iters", C.c_int), ("outer_iters", C.c_int), ("max_rel_resid", C.c_double),
+                ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PlasticStats(C.Structure):
@@ -171,6 +183,7 @@ def load_library(path=LIB_PATH):
         "bhip_server_wait": (C.c_int, [vp, C.c_longlong, _P_D, C.POINTER(RequestStats)]),
         "bhip_server_budget": (C.c_double, [vp]),
         "bhip_request_footprint": (C.c_double, [C.c_int, C.c_int, C.c_int]),
+        "bhip_heat_run": (C.c_int, [vp, C.c_int, _P_D, C.c_int, C.POINTER(SolveOpts), C.POINTER(HeatStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -370,6 +383,15 @@ class Context:
         d = s.as_dict()
         d["status"] = st
         return X, d
+
+    def heat_run(self, n_steps, u0, x0_mode=1, opts=None, **kw):
+        """bhip_heat_run: n_steps of the transient diffusion after a
+        BHIP_ASM_HEAT_TRANSIENT assembly; returns (u, stats)."""
+        u = _f64(u0).copy()
+        o = opts if opts is not None else self.opts(**kw)
+        s = HeatStats()
+        self._chk(self.lib.bhip_heat_run(self.h, n_steps, _d(u), x0_mode, C.byref(o), C.byref(s)))
+        return u, s.as_dict()
 
     def spmv(self, x):
         y = np.zeros(self.n_dof)

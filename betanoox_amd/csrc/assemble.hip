@@ -493,6 +493,8 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
         O.val_aux = ctx->val_aux.p;
         O.a1 = alpha * dt;  // DiffusionModel.cpp:745
         O.dt = dt;
+        ctx->heat_a1 = O.a1;
+        ctx->heat_dt = dt;
         double a2 = (1 - alpha) * dt;
         double fcoef = -(O.a1 + a2) / dt;  // formF_s_new: -(a1+a2) K u_s / dt
         st = launch_assembly<HeatHex8>(ctx, A, O, u_state, fcoef);
@@ -575,6 +577,22 @@ __global__ void __launch_bounds__(256) k_lift_blocks(AsmArgs A, const int *n2e_p
         for (int d = 0; d < P::ND; d++) lval[k * P::NB + c * P::ND + d] = oriented<P::ND>(K, nlo, c, d);
 }
 
+// Heat (1 DOF per node): entry k holds K(n,m) and (a1 K(n,m) + M(n,m))/dt,
+// the two coefficients the assembly's gather applies to u_s and to the
+// prescribed increment.
+__global__ void __launch_bounds__(256) k_lift_blocks_heat(AsmArgs A, const int *n2e_ptr, const int *n2e,
+                                                          long long n_lift, const int *lrow, const int *lcol,
+                                                          double a1, double dt, double *lval) {
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_lift) return;
+    const int n = lrow[k], m = lcol[k];
+    const int lo = n < m ? n : m, hi = n < m ? m : n;
+    double K[1], K2[1];
+    pair_block<HeatHex8>(A, n2e_ptr, n2e, n, lo, hi, K, K2);
+    lval[2 * k] = K[0];
+    lval[2 * k + 1] = (a1 * K[0] + K2[0]) / dt;
+}
+
 // b_r[I] = restrained ? presc_r[I] : load[I] - sum over the node's table
 // entries (ascending neighbour, then component) of K(I,J) presc_r[J],
 // J restrained -- the order in which the assembly's gather accumulates.
@@ -613,15 +631,23 @@ __global__ void k_lift_rhs(int n_nodes, int R, long long n_dof, const int *first
     }
 }
 
-int lift_build(bhip_ctx *ctx) {
+}  // namespace
+
+int bhip_lift_build(bhip_ctx *ctx) {
     if (ctx->lift_ready) return BHIP_OK;
-    if (!ctx->assembled || ctx->asm_kind != BHIP_ASM_STIFFNESS)
-        return bhip_fail(ctx, BHIP_ERR_STATE, "re-lifting needs an assembled stiffness (BHIP_ASM_STIFFNESS)");
+    const bool heat = ctx->assembled && ctx->asm_kind == BHIP_ASM_HEAT_TRANSIENT;
+    if (!ctx->assembled || (ctx->asm_kind != BHIP_ASM_STIFFNESS && !heat))
+        return bhip_fail(ctx, BHIP_ERR_STATE,
+                         "the lift table needs a stiffness or heat-transient assembly");
     const int kind = ctx->mat_kind;
-    const bool hex = ctx->nnpe == 8 && ctx->max_ndof == 3 && ctx->uniform_ndof &&
+    const bool hex = !heat && ctx->nnpe == 8 && ctx->max_ndof == 3 && ctx->uniform_ndof &&
                      (kind == BHIP_MAT_ISO_ELASTIC || kind == BHIP_MAT_J2 || kind == BHIP_MAT_CMAT3D);
-    const bool quad = ctx->nnpe == 4 && ctx->max_ndof == 2 && ctx->uniform_ndof && kind == BHIP_MAT_PLANE_STRESS;
-    if (!hex && !quad) return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "re-lifting not built for this element/material");
+    const bool quad = !heat && ctx->nnpe == 4 && ctx->max_ndof == 2 && ctx->uniform_ndof &&
+                      kind == BHIP_MAT_PLANE_STRESS;
+    if (heat && !(ctx->nnpe == 8 && ctx->max_ndof == 1 && ctx->uniform_ndof && kind == BHIP_MAT_HEAT))
+        return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "heat lift table needs hex8 with 1 DOF per node");
+    if (!hex && !quad && !heat)
+        return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "re-lifting not built for this element/material");
     hipStream_t s = ctx->stream;
     const int B = 256;
     const int nn = ctx->n_nodes;
@@ -636,10 +662,10 @@ int lift_build(bhip_ctx *ctx) {
     BHIP_TRY(ctx, bhip_copy(ctx, &total, ctx->lift_ptr.p + nn, sizeof(long long), hipMemcpyDeviceToHost));
     ctx->n_lift = total;
     const size_t ne = total > 0 ? (size_t)total : 1;
-    const int ND = hex ? 3 : 2;
+    const int per = hex ? 9 : quad ? 4 : 2;
     BHIP_TRY(ctx, ctx->lift_row.resize(ne));
     BHIP_TRY(ctx, ctx->lift_col.resize(ne));
-    BHIP_TRY(ctx, ctx->lift_val.resize(ne * ND * ND));
+    BHIP_TRY(ctx, ctx->lift_val.resize(ne * per));
     if (total > 0) {
         k_lift_fill<<<grid_for(nn, B), B, 0, s>>>(nn, ctx->node_rmask.p, ctx->nbr_ptr.p, ctx->nbr.p, ctx->lift_ptr.p,
                                                   ctx->lift_row.p, ctx->lift_col.p);
@@ -662,7 +688,10 @@ int lift_build(bhip_ctx *ctx) {
         A.tan = nullptr;
         A.flags = nullptr;
         const int g = grid_for(total, B);
-        if (quad)
+        if (heat)
+            k_lift_blocks_heat<<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,
+                                               ctx->lift_col.p, ctx->heat_a1, ctx->heat_dt, ctx->lift_val.p);
+        else if (quad)
             k_lift_blocks<ElasQuad4><<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,
                                                      ctx->lift_col.p, ctx->lift_val.p);
         else if (kind == BHIP_MAT_CMAT3D)
@@ -681,10 +710,10 @@ int lift_build(bhip_ctx *ctx) {
     return BHIP_OK;
 }
 
-}  // namespace
-
 int bhip_lift_rhs(bhip_ctx *ctx, int R, const double *presc, const double *load, double *b) {
-    BHIP_PROPAGATE(lift_build(ctx));
+    if (ctx->assembled && ctx->asm_kind != BHIP_ASM_STIFFNESS)
+        return bhip_fail(ctx, BHIP_ERR_STATE, "re-lifting needs an assembled stiffness (BHIP_ASM_STIFFNESS)");
+    BHIP_PROPAGATE(bhip_lift_build(ctx));
     const int B = 256;
     if (ctx->max_ndof == 3)
         k_lift_rhs<3><<<grid_for(ctx->n_nodes, B), B, 0, ctx->stream>>>(

@@ -131,6 +131,8 @@ struct bhip_ctx {
     dvec<long long> lift_ptr;   // per node
     dvec<int> lift_row, lift_col;
     dvec<double> lift_val;      // ndof x ndof per entry, row comp of the owner node first
+                                // (heat: K and (a1 K + M)/dt per entry)
+    double heat_a1 = 0, heat_dt = 1;  // coefficients of the last heat-transient assembly
     // ---- vectors (FP64)
     dvec<double> load, pseudo, rhs, x, x0, r, p, q, dinv, tmp;
     // ---- vectors (FP32 inner)
@@ -232,6 +234,7 @@ int bhip_scan_i64(bhip_ctx *ctx, const long long *in, long long *out, long long 
 // Right-hand sides of R load cases from the lift table (built on first use
 // after a stiffness assembly): b_r = restrained ? presc_r : load + (-K_fr presc_r).
 // presc and b are device arrays, R x n_dof case-major; load may be null.
+int bhip_lift_build(bhip_ctx *ctx);
 int bhip_lift_rhs(bhip_ctx *ctx, int R, const double *presc, const double *load, double *b);
 int bhip_prepare_solver(bhip_ctx *ctx, bool fp32);
 int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb);

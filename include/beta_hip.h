@@ -223,6 +223,31 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
  * accumulated displacement (n_dof); any pointer may be NULL. */
 int bhip_plastic_get_state(bhip_ctx *ctx, double *stress, double *eps_bar, double *u_total);
 
+/* ------------------------------------------------------ transient diffusion */
+#define BHIP_HEAT_X0_ZERO       0   /* initial guess du = 0 */
+#define BHIP_HEAT_X0_PREVIOUS   1   /* du of the previous step (0 on the first) */
+#define BHIP_HEAT_X0_REFERENCE  2   /* u_s on free DOFs, the prescribed value on restrained ones,
+                                       as Equations::solve(u_s1) (equation.cpp:1367-1376) */
+
+typedef struct bhip_heat_stats {
+    int    steps;            /* steps completed */
+    int    iters;            /* PCG iterations over all steps (inner ones for FP32_IR) */
+    int    outer_iters;      /* refinement steps over all steps */
+    double max_rel_resid;    /* worst true relative residual of a step */
+    double ms;               /* device time of the run (HIP events) */
+} bhip_heat_stats;
+
+/* DiffusionModel::DoTransientNonLinearAnalysis (DiffusionModel.cpp:711-831)
+ * for a linear conductivity, after one BHIP_ASM_HEAT_TRANSIENT assembly with
+ * coef = {alpha, dt} (Mbar = (a1 K + M)/dt, a1 = alpha dt): n_steps steps of
+ * Mbar du = -K u_s + lift (assembleK_F_s_alpha_family, 322-347), u_s += du,
+ * with the context's prescribed values as the increment of every restrained
+ * DOF.  The reference reassembles K and M every step although they are
+ * constant; here the right-hand side comes from the assembled K on the device.
+ * u: n_dof, u_0 on entry, u after n_steps on return. */
+int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode, const bhip_solve_opts *opts,
+                  bhip_heat_stats *stats);
+
 /* --------------------------------------------------------------- serving */
 /* No counterpart in the reference, which runs one model at a time on the
  * host.  The served workflow is the reference's macro-element / RVE one
