@@ -217,8 +217,8 @@ def main():
     achieved = by32 / ms32 / 1e6
     _, stream_gbps = ctx.bench_stream(1 << 30, 20)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.n, "k_sell3_spmv<1>"),
-                "kernel": "k_sell3_spmv<1> (FP32 3x3-block sliced-ELL SpMV with p.Ap partials)",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.n, "k_sell_spmv<3, 1>"),
+                "kernel": "k_sell_spmv<3, 1> (FP32 3x3-block sliced-ELL SpMV with p.Ap partials)",
                 "bytes_per_launch": by32, "ms_per_launch": ms32}
 
     line = {

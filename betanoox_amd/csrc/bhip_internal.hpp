@@ -77,7 +77,8 @@ constexpr int kRedBlocks = 2048;  // upper bound of grid-stride reduction grids
 struct bhip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;    // one solve or one benchmark
+    hipEvent_t drv0 = nullptr, drv1 = nullptr;  // drivers that run several solves
     std::string err;
     int n_cu = 256;
 
@@ -103,14 +104,15 @@ struct bhip_ctx {
     dvec<double> val_aux;                        // second value array (heat: K part)
     // ---- FP32 mirror for the inner solve
     bool f32_ready = false;
-    bool use_bsr = false;
+    bool use_bsr = false;    // FP32 inner solve on SELL-64 node blocks
+    int sell_b = 0;          // block size (DOFs per node) of the SELL layout
     dvec<float> val32;       // CSR FP32 values (non-BSR case)
-    // 3x3-block sliced ELL (SELL-64): slice s holds 64 consecutive nodes and
+    // B x B-block sliced ELL (SELL-64): slice s holds 64 consecutive nodes and
     // w_s block columns; entry (k, lane) of slice s is at sell_ptr[s] + 64k + lane
     int n_slices = 0;
     long long n_sell = 0;    // padded block count
     dvec<int> sell_ptr, sell_col;
-    dvec<float> bval32;      // SELL block values, SoA: 9 x n_sell
+    dvec<float> bval32;      // SELL block values, SoA: B*B x n_sell
     // ---- materials / integration
     int mat_kind = 0, n_groups = 0, mat_stride = 0;
     dvec<double> mat;

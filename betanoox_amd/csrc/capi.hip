@@ -56,6 +56,7 @@ int bhip_create(int device, bhip_ctx **out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreate(&ctx->drv0) != hipSuccess || hipEventCreate(&ctx->drv1) != hipSuccess ||
         hipHostMalloc((void **)&ctx->h_state, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
         bhip_destroy(ctx);
         return BHIP_ERR_HIP;
@@ -93,6 +94,8 @@ void bhip_destroy(bhip_ctx *ctx) {
     if (ctx->h_state) (void)hipHostFree(ctx->h_state);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->drv0) (void)hipEventDestroy(ctx->drv0);
+    if (ctx->drv1) (void)hipEventDestroy(ctx->drv1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

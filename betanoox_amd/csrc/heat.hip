@@ -111,7 +111,7 @@ extern "C" int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode,
     BHIP_TRY(ctx, hipMemcpyAsync(us.p, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
     // Fbar = -(a1 + a2) K u_s / dt with a1 + a2 = dt for every alpha
     const double fcoef = -1.0;
-    BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    BHIP_TRY(ctx, hipEventRecord(ctx->drv0, s));
     int status = BHIP_OK;
     for (int step = 0; step < n_steps; step++) {
         launch_heat_rhs(ctx, us.p, fcoef);
@@ -142,10 +142,10 @@ extern "C" int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode,
         BHIP_TRY(ctx, hipGetLastError());
         H.steps = step + 1;
     }
-    BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
-    BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    BHIP_TRY(ctx, hipEventRecord(ctx->drv1, s));
+    BHIP_TRY(ctx, hipEventSynchronize(ctx->drv1));
     float ms = 0;
-    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->drv0, ctx->drv1));
     H.ms = ms;
     BHIP_TRY(ctx, bhip_copy(ctx, u, us.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (stats) *stats = H;

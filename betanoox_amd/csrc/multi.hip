@@ -609,10 +609,10 @@ extern "C" int bhip_solve_cases(bhip_ctx *ctx, int nrhs, const double *presc, co
     memset(&S, 0, sizeof(S));
     const bool want32 = opts->precision == BHIP_PREC_FP32_IR;
     BHIP_PROPAGATE(bhip_prepare_solver(ctx, want32));
-    const bool fused = want32 && ctx->use_bsr;
+    const bool fused = want32 && ctx->use_bsr && ctx->sell_b == 3;
     BHIP_PROPAGATE(ensure_multi(ctx, nrhs, fused));
     MultiData &M = *ctx->multi;
-    BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    BHIP_TRY(ctx, hipEventRecord(ctx->drv0, s));
     // right-hand sides: each case's prescribed values lifted through the
     // coupling blocks of the assembled K (M.r64 stages the prescribed values,
     // the context's q vector the load)
@@ -660,10 +660,10 @@ extern "C" int bhip_solve_cases(bhip_ctx *ctx, int nrhs, const double *presc, co
         BHIP_TRY(ctx, hipMemcpyAsync(ctx->rhs.p, M.save.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         BHIP_TRY(ctx, hipMemcpyAsync(ctx->x.p, M.save.p + n, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     }
-    BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
-    BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    BHIP_TRY(ctx, hipEventRecord(ctx->drv1, s));
+    BHIP_TRY(ctx, hipEventSynchronize(ctx->drv1));
     float ms = 0;
-    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->drv0, ctx->drv1));
     S.ms_total = ms;
     // true relative residual of every case in FP64; the worst is reported
     std::vector<double> rr, bb;

@@ -443,7 +443,7 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
     const int n = ctx->n_dof;
     bhip_plastic_stats S;
     memset(&S, 0, sizeof(S));
-    BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    BHIP_TRY(ctx, hipEventRecord(ctx->drv0, s));
     if (presc_increment)
         BHIP_TRY(ctx, hipMemcpyAsync(ctx->presc.p, presc_increment, sizeof(double) * n, hipMemcpyHostToDevice, s));
     // currentSolution = prescribed increment on restrained DOFs
@@ -473,10 +473,10 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
     S.newton_iters = it;
     S.max_residual = maxres;
     BHIP_PROPAGATE(bhip_plastic_commit(ctx));
-    BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
-    BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    BHIP_TRY(ctx, hipEventRecord(ctx->drv1, s));
+    BHIP_TRY(ctx, hipEventSynchronize(ctx->drv1));
     float ms = 0;
-    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->drv0, ctx->drv1));
     S.ms = ms;
     // plastic points and return-map iterations of the converged state
     long long npts = (long long)ctx->n_el * P->nip;

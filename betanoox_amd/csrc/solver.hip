@@ -17,8 +17,9 @@
 //
 // Storage: full symmetric CSR (int32 column indices) for the FP64 solve and
 // the FP64 outer residual; the FP32 inner solve of the refined solver uses
-// FP32 values in 3x3 node blocks laid out as sliced ELL (SELL-64, one lane
-// per node row) when every node carries 3 DOFs, and FP32 CSR otherwise.
+// FP32 values in B x B node blocks laid out as sliced ELL (SELL-64, one lane
+// per node row) when every node carries the same number B <= 3 of DOFs, and
+// FP32 CSR otherwise.
 #include <cstdlib>
 
 #include "bhip_internal.hpp"
@@ -105,20 +106,21 @@ __global__ void __launch_bounds__(kThreads) k_csr_spmv(int n, const int *__restr
     }
 }
 
-// ------------------------------------------------------- SpMV SELL-64 BSR-3
-// Lane l of slice s owns node 64 s + l and walks the slice's block columns;
-// block values (one array per entry of the 3x3 block) and column indices are
-// laid out so that the 64 lanes of a wave read 64 consecutive entries and
-// consecutive nodes gather neighbouring x entries.  No cross-lane reduction;
-// padded blocks carry zeros and the node's own column.  4 slices per block.
-template <int MODE>
-__global__ void __launch_bounds__(kThreads) k_sell3_spmv(int n_nodes, int n_slices,
-                                                       const int *__restrict__ sptr,
-                                                       const int *__restrict__ scol,
-                                                       const float *__restrict__ sval, long long nsell,
-                                                       const float *__restrict__ x,
-                                                       float *__restrict__ y, const PcgState *st,
-                                                       double *partials) {
+// ------------------------------------------------------- SpMV SELL-64 BSR-B
+// B x B node blocks (B = DOFs per node).  Lane l of slice s owns node
+// 64 s + l and walks the slice's block columns; block values (one array per
+// entry of the B x B block) and column indices are laid out so that the 64
+// lanes of a wave read 64 consecutive entries and consecutive nodes gather
+// neighbouring x entries.  No cross-lane reduction; padded blocks carry zeros
+// and the node's own column.  4 slices per block.
+template <int B, int MODE>
+__global__ void __launch_bounds__(kThreads) k_sell_spmv(int n_nodes, int n_slices,
+                                                      const int *__restrict__ sptr,
+                                                      const int *__restrict__ scol,
+                                                      const float *__restrict__ sval, long long nsell,
+                                                      const float *__restrict__ x,
+                                                      float *__restrict__ y, const PcgState *st,
+                                                      double *partials) {
     if (MODE == 1 && st->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63;
@@ -127,21 +129,29 @@ __global__ void __launch_bounds__(kThreads) k_sell3_spmv(int n_nodes, int n_slic
     if (s < n_slices) {
         const long long node = s * 64 + lane;
         const int kb = sptr[s], ke = sptr[s + 1];
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        float a[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) a[c] = 0.f;
 #pragma unroll 4
         for (int k = kb + lane; k < ke; k += 64) {
-            const int m = scol[k];
-            const float x0 = x[3 * m], x1 = x[3 * m + 1], x2 = x[3 * m + 2];
-            a0 += sval[0 * nsell + k] * x0 + sval[1 * nsell + k] * x1 + sval[2 * nsell + k] * x2;
-            a1 += sval[3 * nsell + k] * x0 + sval[4 * nsell + k] * x1 + sval[5 * nsell + k] * x2;
-            a2 += sval[6 * nsell + k] * x0 + sval[7 * nsell + k] * x1 + sval[8 * nsell + k] * x2;
+            const long long m = scol[k];
+            float xv[B];
+#pragma unroll
+            for (int d = 0; d < B; d++) xv[d] = x[B * m + d];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                float t = sval[(c * B) * nsell + k] * xv[0];
+#pragma unroll
+                for (int d = 1; d < B; d++) t += sval[(c * B + d) * nsell + k] * xv[d];
+                a[c] += t;
+            }
         }
         if (node < n_nodes) {
-            y[3 * node] = a0;
-            y[3 * node + 1] = a1;
-            y[3 * node + 2] = a2;
-            if (MODE == 1)
-                acc = (double)x[3 * node] * a0 + (double)x[3 * node + 1] * a1 + (double)x[3 * node + 2] * a2;
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                y[B * node + c] = a[c];
+                if (MODE == 1) acc += (double)x[B * node + c] * a[c];
+            }
         }
     }
     if (MODE == 1) {
@@ -329,11 +339,12 @@ __global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, int *scol
     }
 }
 
-// FP64 CSR -> FP32 SELL-64 3x3 blocks (SoA); restrained entries zero,
+// FP64 CSR -> FP32 SELL-64 B x B blocks (SoA); restrained entries zero,
 // restrained diagonal 1.  Pair p = (node n, k-th neighbour m).
-__global__ void k_csr_to_sell3(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
-                               const int *nbr_off, const int *rowptr, const unsigned char *rmask,
-                               const double *val, const int *sptr, long long nsell, int *scol, float *sval) {
+template <int B>
+__global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
+                              const int *nbr_off, const int *rowptr, const unsigned char *rmask,
+                              const double *val, const int *sptr, long long nsell, int *scol, float *sval) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     int n = pair_row[p], m = nbr[p];
@@ -341,16 +352,16 @@ __global__ void k_csr_to_sell3(long long n_pairs, const int *pair_row, const int
     long long dst = (long long)sptr[n / 64] + 64LL * k + (n % 64);
     scol[dst] = m;
     unsigned rn = rmask[n], rm = rmask[m];
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < B; c++) {
         int r = 0;
-        long long base = (long long)rowptr[3 * n + c] + nbr_off[p];
-        for (int d = 0; d < 3; d++) {
+        long long base = (long long)rowptr[B * n + c] + nbr_off[p];
+        for (int d = 0; d < B; d++) {
             float v = 0.f;
             bool rc = (rn >> c) & 1u, rd = (rm >> d) & 1u;
             if (!rc && !rd) v = (float)val[base + r];
             else if (rc && n == m && c == d) v = 1.f;
             if (!rd) r++;
-            sval[(long long)(c * 3 + d) * nsell + dst] = v;
+            sval[(long long)(c * B + d) * nsell + dst] = v;
         }
     }
 }
@@ -423,20 +434,26 @@ void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *
 
 void launch_sell(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgState *st) {
     hipStream_t s = ctx->stream;
-    if (mode == 1)
-        k_sell3_spmv<1><<<sell_grid(ctx), kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
-                                                            ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
-                                                            ctx->partials.p);
-    else
-        k_sell3_spmv<0><<<sell_grid(ctx), kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
-                                                            ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
-                                                            ctx->partials.p);
+#define BHIP_SELL(BB, MM)                                                                                      \
+    k_sell_spmv<BB, MM><<<sell_grid(ctx), kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,     \
+                                                            ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, \
+                                                            st, ctx->partials.p)
+    if (ctx->sell_b == 3) {
+        if (mode == 1) BHIP_SELL(3, 1); else BHIP_SELL(3, 0);
+    } else if (ctx->sell_b == 2) {
+        if (mode == 1) BHIP_SELL(2, 1); else BHIP_SELL(2, 0);
+    } else {
+        if (mode == 1) BHIP_SELL(1, 1); else BHIP_SELL(1, 0);
+    }
+#undef BHIP_SELL
 }
 
 // Algorithmic bytes of one SpMV (values + indices + row pointers + x + y);
 // for the blocked form the stored (padded) blocks are counted.
 double spmv_bytes(bhip_ctx *ctx, bool fp32, bool bsr) {
-    if (bsr) return (double)ctx->n_sell * (9 * 4 + 4) + (ctx->n_slices + 1.0) * 4 + 2.0 * ctx->n_dof * 4;
+    if (bsr)
+        return (double)ctx->n_sell * (ctx->sell_b * ctx->sell_b * 4 + 4) + (ctx->n_slices + 1.0) * 4 +
+               2.0 * ctx->n_dof * 4;
     double sv = fp32 ? 4 : 8;
     return (double)ctx->nnz * (sv + 4) + (ctx->n_dof + 1.0) * 4 + 2.0 * ctx->n_dof * sv;
 }
@@ -493,7 +510,7 @@ int build_sell(bhip_ctx *ctx) {
     BHIP_TRY(ctx, ctx->sell_ptr.resize(ns + 1));
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->sell_ptr.p, hp.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
     BHIP_TRY(ctx, ctx->sell_col.resize(tot > 0 ? tot : 1));
-    BHIP_TRY(ctx, ctx->bval32.resize((size_t)9 * (tot > 0 ? tot : 1)));
+    BHIP_TRY(ctx, ctx->bval32.resize((size_t)ctx->sell_b * ctx->sell_b * (tot > 0 ? tot : 1)));
     k_sell_pad<<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
     BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
     BHIP_TRY(ctx, hipGetLastError());
@@ -503,13 +520,21 @@ int build_sell(bhip_ctx *ctx) {
 int prepare_f32(bhip_ctx *ctx) {
     if (ctx->f32_ready) return BHIP_OK;
     const int B = 256;
-    ctx->use_bsr = (ctx->uniform_ndof && ctx->max_ndof == 3 && ctx->n_dof == 3 * ctx->n_nodes);
+    const int nb = ctx->max_ndof;
+    ctx->use_bsr = (ctx->uniform_ndof && nb >= 1 && nb <= 3 && (long long)ctx->n_dof == (long long)nb * ctx->n_nodes);
     if (ctx->use_bsr) {
-        if (ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_col.p)
+        if (ctx->sell_b != nb || ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_col.p) {
+            ctx->sell_b = nb;
             BHIP_PROPAGATE(build_sell(ctx));
-        k_csr_to_sell3<<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(
-            ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,
-            ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col.p, ctx->bval32.p);
+        }
+#define BHIP_TO_SELL(BB)                                                                                      \
+    k_csr_to_sell<BB><<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(                                       \
+        ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,             \
+        ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col.p, ctx->bval32.p)
+        if (nb == 3) BHIP_TO_SELL(3);
+        else if (nb == 2) BHIP_TO_SELL(2);
+        else BHIP_TO_SELL(1);
+#undef BHIP_TO_SELL
         if (ctx->val32.p) {
             bhip_invalidate_graphs(ctx);
             ctx->val32.release();

@@ -4,7 +4,7 @@ Runs, on the hex8 elasticity workload (default 96^3 elements), a fixed
 sequence of launches so that kernel traces and PMC counters can be
 attributed per kernel:
   reps x plain device copy of 1 GiB       (bhip_bench_stream: k_stream_copy)
-  reps x FP32 SELL-64 3x3-block SpMV      (k_sell3_spmv<1>)
+  reps x FP32 SELL-64 3x3-block SpMV      (k_sell_spmv<3, 1>)
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
   one FP32-inner + FP64-refined solve     (unless --no-solve)
 and prints the HIP-event timings of each group.
