@@ -1,5 +1,5 @@
 """Benchmark: 3D hex8 linear elasticity (96^3 elements, 2,738,019 DOF) on one
-MI355X.
+MI355X, with the other workloads of the path measured beside it.
 
 One step = one full solve of the resident system to a true relative residual
 of 1e-8 with the chosen form (FP32-inner Jacobi-PCG on FP32 3x3-block values
@@ -9,7 +9,14 @@ assembly is timed separately, the dominant kernel (the SpMV of the inner
 solve) is timed with HIP events for the roofline, and the oracle (serial CPU
 restatement of the reference path) is timed on a bounded sample.
 
+Also reported in the same JSON line:
+  heat    -- transient diffusion 128^3, 100 backward-Euler steps to 1e-8
+  served  -- RVE request stream (prefix of the seeded 4,096-request stream),
+             solves/s and p50/p99 latency at 1e-8
+  plastic -- J2 plasticity bar, 20 increments (only with --plastic-n 64; ~70 s)
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96]
+                       [--heat-n 128] [--served 128] [--plastic-n 0]
 """
 import argparse
 import json
@@ -140,6 +147,68 @@ def served_stream(count, slots, rate_frac, probe_count, seed=None):
     return out
 
 
+def heat_leg(n, steps, fp64_steps):
+    """Workload 3: transient diffusion on an n^3 hex8 cube (1 DOF per node),
+    backward Euler dt = 1e-3, Mbar assembled once, every step solved to 1e-8
+    true relative residual on the device (FP32 inner + FP64 refinement; plain
+    FP64 timed beside it on the first `fp64_steps` steps)."""
+    from betanoox_amd import capi, meshgen, setup_problem
+    prob = meshgen.diffusion_cube(n)
+    out = {"n": n, "n_dof": prob.n_dof, "steps": steps, "dt": prob.extra["dt"], "tol": 1e-8}
+    with capi.Context(0) as ctx:
+        setup_problem(ctx, prob)
+        u0 = prob.extra["u0"]
+        t0 = time.perf_counter()
+        ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[1.0, prob.extra["dt"]], u_state=u0)
+        out["assembly_ms"] = 1e3 * (time.perf_counter() - t0)
+        o32 = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
+                                inner_tol=1e-3, maxit=40000)
+        ctx.heat_run(2, u0, capi.HEAT_X0_PREVIOUS, opts=o32)  # warm-up (graphs, tables)
+        t0 = time.perf_counter()
+        u, st = ctx.heat_run(steps, u0, capi.HEAT_X0_PREVIOUS, opts=o32)
+        wall = time.perf_counter() - t0
+        out.update({"fp32_ir": st, "wall_s": wall, "steps_per_s": steps / wall,
+                    "dof_iter_per_s": prob.n_dof * st["iters"] / wall, "u_max_end": float(np.abs(u).max())})
+        if fp64_steps > 0:
+            o64 = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64, maxit=40000)
+            t0 = time.perf_counter()
+            _, s64 = ctx.heat_run(fp64_steps, u0, capi.HEAT_X0_PREVIOUS, opts=o64)
+            w64 = time.perf_counter() - t0
+            out["plain_fp64_first_steps"] = {"steps": fp64_steps, "stats": s64, "wall_s": w64,
+                                             "dof_iter_per_s": prob.n_dof * s64["iters"] / w64}
+            out["speedup_vs_fp64_per_step"] = (w64 / fp64_steps) / (wall / steps)
+        ms, by = ctx.bench_spmv(fp32=True, reps=20)
+        out["spmv_fp32_sell1"] = {"ms": ms, "GBps": by / ms / 1e6, "bytes": by}
+    return out
+
+
+def plastic_leg(n, newton_tol, max_newton):
+    """Workload 4: J2 plasticity bar n^3, 20 prescribed-displacement
+    increments to 0.5% strain, Newton with the tangent reassembled every
+    iteration until max|residual| < 2*newton_tol, linear solves to 1e-8."""
+    from betanoox_amd import capi, meshgen, setup_problem
+    prob = meshgen.plastic_bar(n)
+    out = {"n": n, "n_dof": prob.n_dof, "increments": prob.extra["increments"], "newton_tol": newton_tol}
+    with capi.Context(0) as ctx:
+        setup_problem(ctx, prob)
+        ctx.plastic_init()
+        ctx.set_load(prob.load)
+        o = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
+                              inner_tol=1e-3, maxit=40000)
+        hist = []
+        t0 = time.perf_counter()
+        for inc in range(prob.extra["increments"]):
+            st = ctx.plastic_increment(prob.prescribed, newton_tol, max_newton, opts=o)
+            hist.append(st)
+            log("plastic increment %d: %s" % (inc, json.dumps(st)))
+        wall = time.perf_counter() - t0
+    out.update({"wall_s": wall, "newton_iters": sum(h["newton_iters"] for h in hist),
+                "pcg_iters": sum(h["pcg_iters"] for h in hist),
+                "all_converged": all(h["status"] == 0 for h in hist),
+                "dof_iter_per_s": prob.n_dof * sum(h["pcg_iters"] for h in hist) / wall})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +224,11 @@ def main():
     ap.add_argument("--served-slots", type=int, default=4)
     ap.add_argument("--served-load", type=float, default=0.7, help="arrival rate / probed capacity")
     ap.add_argument("--served-probe", type=int, default=32)
+    ap.add_argument("--heat-n", type=int, default=128, help="diffusion cube size (0: skip)")
+    ap.add_argument("--heat-steps", type=int, default=100)
+    ap.add_argument("--heat-fp64-steps", type=int, default=10)
+    ap.add_argument("--plastic-n", type=int, default=0, help="J2 bar size (0: skip; the full workload is 64)")
+    ap.add_argument("--plastic-tol", type=float, default=1e-9)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (a single solve "
@@ -252,6 +326,12 @@ def main():
         line["speedup_vs_fp64_dof_iter"] = value / result_fp64["dof_iter_per_s"]
         line["speedup_vs_fp64_time_to_solution"] = result_fp64["ms"] / (1e3 * dt / args.steps)
     ctx.close()
+    if args.heat_n > 0:
+        line["heat"] = heat_leg(args.heat_n, args.heat_steps, args.heat_fp64_steps)
+        log("heat: %s" % json.dumps(line["heat"]))
+    if args.plastic_n > 0:
+        line["plastic"] = plastic_leg(args.plastic_n, args.plastic_tol, 300)
+        log("plastic: %s" % json.dumps(line["plastic"]))
     if args.served > 0:
         line["served"] = served_stream(args.served, args.served_slots, args.served_load, args.served_probe)
         log("served: %s" % json.dumps(line["served"]))

@@ -1,45 +1,81 @@
 """Kernel profiling driver for rocprofv3.
 
-Runs, on the hex8 elasticity workload (default 96^3 elements), a fixed
-sequence of launches so that kernel traces and PMC counters can be
-attributed per kernel:
+Runs a fixed sequence of launches so that kernel traces and PMC counters
+can be attributed per kernel, and prints the HIP-event timings of each group.
+
+--workload elastic (default; hex8 elasticity, --n 96):
   reps x plain device copy of 1 GiB       (bhip_bench_stream: k_stream_copy)
   reps x FP32 SELL-64 3x3-block SpMV      (k_sell_spmv<3, 1>)
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
   one FP32-inner + FP64-refined solve     (unless --no-solve)
-and prints the HIP-event timings of each group.
+--workload cases (RVE request, --n 64, six load cases):
+  one fused six-case solve                (k_sell3_spmv_multi<6>, k_update_multi<6>, ...)
+--workload heat (diffusion, --n 128, 1 DOF per node):
+  reps x FP32 SELL-64 scalar SpMV         (k_sell_spmv<1, 1>)
+  reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
+  10 backward-Euler steps                 (unless --no-solve)
 
-usage: python -m betanoox_amd.profile_kernels [--n 96] [--reps 10] [--no-solve]
+usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat] [--n N] [--reps 10] [--no-solve]
 """
 import argparse
 import json
 import sys
+
+import numpy as np
 
 from . import capi, meshgen, setup_problem
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--n", type=int, default=96)
+    ap.add_argument("--workload", choices=("elastic", "cases", "heat"), default="elastic")
+    ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--no-solve", action="store_true")
     a = ap.parse_args(argv)
-    prob = meshgen.elastic_cube(a.n)
-    out = {"n": a.n, "n_dof": prob.n_dof}
+    out = {"workload": a.workload}
     with capi.Context(0) as ctx:
-        setup_problem(ctx, prob)
-        ctx.assemble()
-        ctx.set_load(prob.load)
-        ctx.build_rhs()
-        ms, gbps = ctx.bench_stream(1 << 30, a.reps)
-        out["stream_copy"] = {"bytes_read_plus_written": 2 * (1 << 30), "ms": ms, "GBps": gbps}
-        ms, by = ctx.bench_spmv(fp32=True, reps=a.reps)
-        out["sell3_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
-        ms, by = ctx.bench_spmv(fp32=False, reps=a.reps)
-        out["csr_fp64"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
-        if not a.no_solve:
-            s = ctx.solve(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
-            out["solve_fp32_ir"] = {k: s[k] for k in ("iters", "outer_iters", "rel_resid", "ms_total")}
+        if a.workload == "elastic":
+            n = a.n or 96
+            prob = meshgen.elastic_cube(n)
+            setup_problem(ctx, prob)
+            ctx.assemble()
+            ctx.set_load(prob.load)
+            ctx.build_rhs()
+            ms, gbps = ctx.bench_stream(1 << 30, a.reps)
+            out["stream_copy"] = {"bytes_read_plus_written": 2 * (1 << 30), "ms": ms, "GBps": gbps}
+            ms, by = ctx.bench_spmv(fp32=True, reps=a.reps)
+            out["sell3_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
+            ms, by = ctx.bench_spmv(fp32=False, reps=a.reps)
+            out["csr_fp64"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
+            if not a.no_solve:
+                s = ctx.solve(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
+                out["solve_fp32_ir"] = {k: s[k] for k in ("iters", "outer_iters", "rel_resid", "ms_total")}
+        elif a.workload == "cases":
+            n = a.n or 64
+            prob = meshgen.rve_request(n, 7)
+            setup_problem(ctx, prob)
+            ctx.assemble()
+            P = np.stack(prob.extra["cases"])
+            o = capi.Context.opts(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
+            X, s = ctx.solve_cases(P, opts=o)
+            out["cases_fp32_ir"] = {k: s[k] for k in ("iters", "outer_iters", "rel_resid", "ms_total")}
+        else:
+            n = a.n or 128
+            prob = meshgen.diffusion_cube(n)
+            setup_problem(ctx, prob)
+            u0 = prob.extra["u0"]
+            ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[1.0, prob.extra["dt"]], u_state=u0)
+            ms, by = ctx.bench_spmv(fp32=True, reps=a.reps)
+            out["sell1_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
+            ms, by = ctx.bench_spmv(fp32=False, reps=a.reps)
+            out["csr_fp64"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
+            if not a.no_solve:
+                u, s = ctx.heat_run(10, u0, capi.HEAT_X0_PREVIOUS, tol=1e-8, precision=capi.PREC_FP32_IR,
+                                    inner_tol=1e-3, maxit=40000)
+                out["heat_10_steps"] = s
+        out["n"] = n
+        out["n_dof"] = prob.n_dof
     print(json.dumps(out), flush=True)
     return 0
 
