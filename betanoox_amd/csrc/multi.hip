@@ -123,35 +123,38 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_alpha_multi(int nb, const d
         for (int r = 0; r < R; r++) st->alpha[r] = st->done[r] ? 0.0 : st->rz[r] / t[r];
 }
 
-// x_r += alpha_r p_r; r_r -= alpha_r q_r; partials (r.z, r.r) per case.
+// Vector kernels run on a (blocks, R) grid: block (b, r) streams case r over
+// the b-th DOF range and writes its partials at b * 2R + 2r, the layout the
+// finalize kernels sum.  Blocks of a case that has finished return at once
+// (the finalize kernels ignore such a case's partials).
+
+// x_r += alpha_r p_r; r_r -= alpha_r q_r; partials (r.z, r.r).
 template <int R>
 __global__ void __launch_bounds__(kThreads) k_update_multi(long long n, float *__restrict__ x, float *__restrict__ rv,
                                                          const float *__restrict__ p, const float *__restrict__ q,
                                                          const float *__restrict__ dinv, const MultiState *st,
                                                          double *partials) {
-    if (st->all_done) return;
+    const int r = blockIdx.y;
+    if (st->all_done || st->done[r]) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     long long i0, i1;
     range_of(n, lb, gridDim.x, i0, i1);
-    float alpha[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) alpha[r] = (float)st->alpha[r];
-    double acc[2 * R];
-#pragma unroll
-    for (int k = 0; k < 2 * R; k++) acc[k] = 0;
+    const float alpha = (float)st->alpha[r];
+    const long long o = (long long)r * n;
+    double acc[2] = {0, 0};
     for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const float di = dinv[i];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const long long j = r * n + i;
-            const float ri = rv[j] - alpha[r] * q[j];
-            x[j] = x[j] + alpha[r] * p[j];
-            rv[j] = ri;
-            acc[2 * r] += (double)ri * (ri * di);
-            acc[2 * r + 1] += (double)ri * ri;
-        }
+        const float ri = rv[o + i] - alpha * q[o + i];
+        x[o + i] = x[o + i] + alpha * p[o + i];
+        rv[o + i] = ri;
+        acc[0] += (double)ri * (ri * dinv[i]);
+        acc[1] += (double)ri * ri;
     }
-    store_partials_n<2 * R>(acc, partials);
+    __shared__ double sm[(kThreads / 64) * 2];
+    block_sum<2>(acc, sm);
+    if (threadIdx.x == 0) {
+        partials[(long long)blockIdx.x * 2 * R + 2 * r] = acc[0];
+        partials[(long long)blockIdx.x * 2 * R + 2 * r + 1] = acc[1];
+    }
 }
 
 template <int R>
@@ -183,51 +186,39 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_update_multi(int nb, const 
 template <int R>
 __global__ void __launch_bounds__(kThreads) k_pupdate_multi(long long n, float *__restrict__ p, const float *__restrict__ rv,
                                                           const float *__restrict__ dinv, const MultiState *st) {
-    if (st->all_done) return;
+    const int r = blockIdx.y;
+    if (st->all_done || st->done[r]) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     long long i0, i1;
     range_of(n, lb, gridDim.x, i0, i1);
-    float beta[R];
-    bool run[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        beta[r] = (float)st->beta[r];
-        run[r] = !st->done[r];
-    }
-    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const float di = dinv[i];
-#pragma unroll
-        for (int r = 0; r < R; r++)
-            if (run[r]) {
-                const long long j = r * n + i;
-                p[j] = rv[j] * di + beta[r] * p[j];
-            }
-    }
+    const float beta = (float)st->beta[r];
+    const long long o = (long long)r * n;
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[o + i] = rv[o + i] * dinv[i] + beta * p[o + i];
 }
 
-// p_r = z_r; partials (r.z, r.r) per case.
+// p_r = z_r; partials (r.z, r.r).
 template <int R>
 __global__ void __launch_bounds__(kThreads) k_init_multi(long long n, const float *__restrict__ rv, float *__restrict__ p,
                                                        const float *__restrict__ dinv, double *partials) {
+    const int r = blockIdx.y;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     long long i0, i1;
     range_of(n, lb, gridDim.x, i0, i1);
-    double acc[2 * R];
-#pragma unroll
-    for (int k = 0; k < 2 * R; k++) acc[k] = 0;
+    const long long o = (long long)r * n;
+    double acc[2] = {0, 0};
     for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const float di = dinv[i];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const long long j = r * n + i;
-            const float ri = rv[j];
-            const float zi = ri * di;
-            p[j] = zi;
-            acc[2 * r] += (double)ri * zi;
-            acc[2 * r + 1] += (double)ri * ri;
-        }
+        const float ri = rv[o + i];
+        const float zi = ri * dinv[i];
+        p[o + i] = zi;
+        acc[0] += (double)ri * zi;
+        acc[1] += (double)ri * ri;
     }
-    store_partials_n<2 * R>(acc, partials);
+    __shared__ double sm[(kThreads / 64) * 2];
+    block_sum<2>(acc, sm);
+    if (threadIdx.x == 0) {
+        partials[(long long)blockIdx.x * 2 * R + 2 * r] = acc[0];
+        partials[(long long)blockIdx.x * 2 * R + 2 * r + 1] = acc[1];
+    }
 }
 
 template <int R>
@@ -450,10 +441,10 @@ void enqueue_iteration(bhip_ctx *ctx, MultiData &M) {
                                                   ctx->bval32.p, ctx->n_sell, n, M.p32.p, M.q32.p, M.st.p,
                                                   M.partials.p);
     k_fin_alpha_multi<R><<<1, kFinThreads, 0, s>>>(gs, M.partials.p, M.st.p);
-    k_update_multi<R><<<gv, kThreads, 0, s>>>(n, M.d32.p, M.r32.p, M.p32.p, M.q32.p, ctx->dinv32.p, M.st.p,
-                                              M.partials.p);
+    k_update_multi<R><<<dim3(gv, R), kThreads, 0, s>>>(n, M.d32.p, M.r32.p, M.p32.p, M.q32.p, ctx->dinv32.p,
+                                                       M.st.p, M.partials.p);
     k_fin_update_multi<R><<<1, kFinThreads, 0, s>>>(gv, M.partials.p, M.st.p);
-    k_pupdate_multi<R><<<gv, kThreads, 0, s>>>(n, M.p32.p, M.r32.p, ctx->dinv32.p, M.st.p);
+    k_pupdate_multi<R><<<dim3(gv, R), kThreads, 0, s>>>(n, M.p32.p, M.r32.p, ctx->dinv32.p, M.st.p);
 }
 
 bool graphs_enabled() {
@@ -489,7 +480,7 @@ int inner_solve(bhip_ctx *ctx, MultiData &M, double inner_tol, int maxit, int ch
     hipStream_t s = ctx->stream;
     const long long n = ctx->n_dof;
     const int gv = vec_grid(ctx, n);
-    k_init_multi<R><<<gv, kThreads, 0, s>>>(n, M.r32.p, M.p32.p, ctx->dinv32.p, M.partials.p);
+    k_init_multi<R><<<dim3(gv, R), kThreads, 0, s>>>(n, M.r32.p, M.p32.p, ctx->dinv32.p, M.partials.p);
     k_fin_init_multi<R><<<1, kFinThreads, 0, s>>>(gv, M.partials.p, M.st.p, inner_tol, maxit);
     BHIP_TRY(ctx, hipGetLastError());
     const bool use_graph = graphs_enabled();
