@@ -94,6 +94,15 @@ struct AsmArgs {
     const double *gw, *gdN, *gN;
 };
 
+// Point the element policies at the Gauss tables passed as a restrict
+// kernel argument: read-only and uniform, so the compiler reads them with
+// scalar loads.
+__device__ __forceinline__ void bind_tables(AsmArgs &A, const double *__restrict__ gtab) {
+    A.gw = gtab;
+    A.gdN = gtab + kGaussDN;
+    A.gN = gtab + kGaussN;
+}
+
 __device__ __forceinline__ int group_of(const AsmArgs &A, int e) {
     if (A.elem_group) return A.elem_group[e];
     return A.n_groups == A.n_el ? e : 0;
@@ -315,7 +324,8 @@ struct AsmOut {
 
 template <class P>
 __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const int *n2e_ptr,
-                                                   const int *n2e) {
+                                                   const int *n2e, const double *__restrict__ gtab) {
+    bind_tables(A, gtab);
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= O.n_pairs) return;
     int n = O.pair_row[p], m = O.nbr[p];
@@ -350,11 +360,13 @@ __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const in
 //   - sum over restrained J of Kscat(I,J) * presc[J]         (addMatrix lift)
 //   + fcoef * sum over all J of Kraw(I,J) * u_state[J]       (element forces, heat)
 template <class P>
-__global__ void __launch_bounds__(256) k_gather_rhs(AsmArgs A, AsmOut O, const int *n2e_ptr,
+__global__ void __launch_bounds__(256) k_gather_rhs(AsmArgs A, const double *__restrict__ gtab, AsmOut O,
+                                                    const int *n2e_ptr,
                                                     const int *n2e, const int *nbr_ptr, int n_nodes,
                                                     const int *ndof, const double *presc,
                                                     const double *u_state, double fcoef,
                                                     double *pseudo) {
+    bind_tables(A, gtab);
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     unsigned rn = O.rmask[n];
@@ -409,13 +421,13 @@ template <class P>
 int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, double fcoef) {
     const int B = 256;
     hipStream_t s = ctx->stream;
-    k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p);
+    k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p);
     BHIP_TRY(ctx, hipGetLastError());
     k_restrained_diag<<<grid_for(ctx->n_dof, B), B, 0, s>>>(ctx->n_dof, ctx->rest.p, ctx->rowptr.p,
                                                             ctx->val.p, O.mode == 1 ? ctx->val_aux.p : nullptr);
     BHIP_TRY(ctx, hipGetLastError());
     k_gather_rhs<P><<<grid_for(ctx->n_nodes, B), B, 0, s>>>(
-        A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->nbr_ptr.p, ctx->n_nodes, ctx->ndof.p, ctx->presc.p,
+        A, ctx->gauss.p, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->nbr_ptr.p, ctx->n_nodes, ctx->ndof.p, ctx->presc.p,
         u_state, fcoef, ctx->pseudo.p);
     BHIP_TRY(ctx, hipGetLastError());
     return BHIP_OK;
@@ -561,9 +573,11 @@ __global__ void k_lift_fill(int n_nodes, const unsigned char *rmask, const int *
 }
 
 template <class P>
-__global__ void __launch_bounds__(256) k_lift_blocks(AsmArgs A, const int *n2e_ptr, const int *n2e,
+__global__ void __launch_bounds__(256) k_lift_blocks(AsmArgs A, const double *__restrict__ gtab,
+                                                     const int *n2e_ptr, const int *n2e,
                                                      long long n_lift, const int *lrow, const int *lcol,
                                                      double *lval) {
+    bind_tables(A, gtab);
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_lift) return;
     const int n = lrow[k], m = lcol[k];
@@ -580,9 +594,11 @@ __global__ void __launch_bounds__(256) k_lift_blocks(AsmArgs A, const int *n2e_p
 // Heat (1 DOF per node): entry k holds K(n,m) and (a1 K(n,m) + M(n,m))/dt,
 // the two coefficients the assembly's gather applies to u_s and to the
 // prescribed increment.
-__global__ void __launch_bounds__(256) k_lift_blocks_heat(AsmArgs A, const int *n2e_ptr, const int *n2e,
+__global__ void __launch_bounds__(256) k_lift_blocks_heat(AsmArgs A, const double *__restrict__ gtab,
+                                                          const int *n2e_ptr, const int *n2e,
                                                           long long n_lift, const int *lrow, const int *lcol,
                                                           double a1, double dt, double *lval) {
+    bind_tables(A, gtab);
     long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_lift) return;
     const int n = lrow[k], m = lcol[k];
@@ -689,17 +705,17 @@ int bhip_lift_build(bhip_ctx *ctx) {
         A.flags = nullptr;
         const int g = grid_for(total, B);
         if (heat)
-            k_lift_blocks_heat<<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,
+            k_lift_blocks_heat<<<g, B, 0, s>>>(A, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,
                                                ctx->lift_col.p, ctx->heat_a1, ctx->heat_dt, ctx->lift_val.p);
         else if (quad)
-            k_lift_blocks<ElasQuad4><<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,
+            k_lift_blocks<ElasQuad4><<<g, B, 0, s>>>(A, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,
                                                      ctx->lift_col.p, ctx->lift_val.p);
         else if (kind == BHIP_MAT_CMAT3D)
-            k_lift_blocks<ElasHex8<BHIP_MAT_CMAT3D>><<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total,
+            k_lift_blocks<ElasHex8<BHIP_MAT_CMAT3D>><<<g, B, 0, s>>>(A, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, total,
                                                                      ctx->lift_row.p, ctx->lift_col.p,
                                                                      ctx->lift_val.p);
         else  // isotropic, and the elastic stiffness of the J2 material
-            k_lift_blocks<ElasHex8<BHIP_MAT_ISO_ELASTIC>><<<g, B, 0, s>>>(A, ctx->n2e_ptr.p, ctx->n2e.p, total,
+            k_lift_blocks<ElasHex8<BHIP_MAT_ISO_ELASTIC>><<<g, B, 0, s>>>(A, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, total,
                                                                           ctx->lift_row.p, ctx->lift_col.p,
                                                                           ctx->lift_val.p);
         BHIP_TRY(ctx, hipGetLastError());

@@ -160,7 +160,7 @@ __device__ __forceinline__ int group_of(const int *elem_group, int n_groups, int
 }
 
 // "V": strain increment + return map (getStressIncrement, plastic.cpp:368-465).
-__global__ void k_plastic_update(int n_el, int nip, const double *gtab, const int *conn, const double *xyz,
+__global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ gtab, const int *conn, const double *xyz,
                                  const double *mat, const int *elem_group, int n_groups, const double *du,
                                  double *isv, double *tan) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -226,7 +226,7 @@ __global__ void k_plastic_update(int n_el, int nip, const double *gtab, const in
 
 // "F" + calculateResidualNorm: per node, resultant of B^T dstress over the
 // incident elements (ascending), residual = load - resultant on free DOFs.
-__global__ void k_plastic_residual(int n_nodes, int nip, const double *gtab, const int *n2e_ptr, const int *n2e,
+__global__ void k_plastic_residual(int n_nodes, int nip, const double *__restrict__ gtab, const int *n2e_ptr, const int *n2e,
                                    const int *conn,
                                    const double *xyz, const double *isv, const unsigned char *rmask,
                                    const double *load, double *resid) {

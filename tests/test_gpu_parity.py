@@ -123,6 +123,37 @@ def test_pcg_fp64(hip_ctx_factory, kind):
     assert np.array_equal(x3, ctx.solution())
 
 
+def test_plane_stress_64_full_size(hip_ctx_factory):
+    """The 2D workload at its full size (64 x 64 quad4, 8,450 DOF), plain FP64
+    Jacobi-PCG to a 1e-10 relative residual: pattern bit-exact and 77,630
+    lower entries, K <= 1e-12, solution against the oracle's FP64 PCG at
+    1e-13 to 1e-8, iteration count within 2 of the oracle's at 1e-10, and the
+    solution against a sparse direct solve (scipy) of the same system to 1e-8."""
+    prob = meshgen.plane_stress_quad(64)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    eq = O.assemble_problem(prob)
+    rp_o, ci_o = eq.pattern()
+    rp_g, ci_g = ctx.pattern_lower()
+    assert np.array_equal(rp_o, rp_g) and np.array_equal(ci_o, ci_g)
+    assert ctx.nnz()[0] == 77630
+    ctx.assemble()
+    assert _row_scaled_err(rp_o, ci_o, ctx.values_lower(), eq.values()) <= 1e-12
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    b = eq.rhs()
+    s = ctx.solve(tol=1e-10, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64, maxit=20000)
+    assert s["rel_resid"] <= 1e-10
+    st, x_o, it_o, _ = eq.pcg(b, eps=1e-10, maxit=20000, stop_rule=1)
+    assert st == 0 and abs(s["iters"] - it_o) <= 2
+    st, x_t, _, _ = eq.pcg(b, eps=1e-13, maxit=40000, stop_rule=1)
+    x_g = ctx.solution()
+    assert np.linalg.norm(x_g - x_t) <= 1e-8 * np.linalg.norm(x_t)
+    import scipy.sparse as sp
+    rp, ci, v = ctx.csr()
+    x_d = sp.linalg.spsolve(sp.csr_matrix((v, ci, rp), shape=(prob.n_dof, prob.n_dof)).tocsc(), b)
+    assert np.abs(x_g - x_d).max() <= 1e-8 * np.abs(x_d).max()
+
+
 @pytest.mark.parametrize("kind", ["quad16", "hex6", "hex5_rollers"])
 def test_pcg_fp32_refined(hip_ctx_factory, kind):
     prob = _problem(kind)

@@ -308,11 +308,6 @@ def test_pattern_closed_form_counts():
     assert eq.nnz() == 77630
 
 
-def _nnz_lower_hex_closed_form(n, rest_nodes_mask, nodes_ijk_n):
-    """Free-node adjacency count for structured hex with whole nodes fixed."""
-    raise NotImplementedError
-
-
 def test_structured_hex_counts_small():
     """n^3 hex8, z=0 clamped: full nnz = sum over free nodes of 3*3*(free
     neighbours incl. self) + restrained DOFs; checked against the oracle."""
