@@ -40,6 +40,17 @@ STOP_REL_RESID = 1
 PREC_FP64 = 0
 PREC_FP32_IR = 1
 
+FMT_RCV_LOWER = 1
+FMT_RCV_UPPER = 2
+FMT_CSR_ASCII = 3
+FMT_CSR_BINARY = 4
+FMT_RCV = 5
+TRI_LOWER = 1
+TRI_UPPER = 2
+TRI_FULL = 3
+VEC_LOAD = 1
+VEC_SOLUTION = 2
+
 HEAT_X0_ZERO = 0
 HEAT_X0_PREVIOUS = 1
 HEAT_X0_REFERENCE = 2
@@ -55,7 +66,8 @@ EXPORTS = [
     "bhip_bench_spmv", "bhip_bench_stream", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
     "bhip_server_create", "bhip_server_destroy", "bhip_server_submit", "bhip_server_poll", "bhip_server_wait",
-    "bhip_server_budget", "bhip_request_footprint", "bhip_heat_run",
+    "bhip_server_budget", "bhip_request_footprint", "bhip_heat_run", "bhip_write_matrix", "bhip_write_csr_file",
+    "bhip_set_csr", "bhip_read_matrix", "bhip_write_vector", "bhip_read_vector",
 ]
 
 
@@ -184,6 +196,12 @@ def load_library(path=LIB_PATH):
         "bhip_server_budget": (C.c_double, [vp]),
         "bhip_request_footprint": (C.c_double, [C.c_int, C.c_int, C.c_int]),
         "bhip_heat_run": (C.c_int, [vp, C.c_int, _P_D, C.c_int, C.POINTER(SolveOpts), C.POINTER(HeatStats)]),
+        "bhip_write_matrix": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "bhip_write_csr_file": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_longlong, _P_I, _P_I, _P_D]),
+        "bhip_set_csr": (C.c_int, [vp, C.c_int, C.c_longlong, _P_I, _P_I, _P_D, C.c_int, C.c_int]),
+        "bhip_read_matrix": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "bhip_write_vector": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _P_D]),
+        "bhip_read_vector": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _P_D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -393,6 +411,21 @@ class Context:
         self._chk(self.lib.bhip_heat_run(self.h, n_steps, _d(u), x0_mode, C.byref(o), C.byref(s)))
         return u, s.as_dict()
 
+    # ---- exchange formats
+    def write_matrix(self, path, fmt):
+        self._chk(self.lib.bhip_write_matrix(self.h, str(path).encode(), fmt))
+
+    def set_csr(self, ia, ja, a, base=0, triangle=TRI_FULL):
+        ia, ja, a = _i32(ia), _i32(ja), _f64(a)
+        n = ia.shape[0] - 1
+        self._chk(self.lib.bhip_set_csr(self.h, n, ja.shape[0], ia.ctypes.data_as(_P_I), ja.ctypes.data_as(_P_I),
+                                        _d(a), base, triangle))
+        self.n_dof = n
+
+    def read_matrix(self, path, fmt):
+        self._chk(self.lib.bhip_read_matrix(self.h, str(path).encode(), fmt))
+        self.n_dof = self.lib.bhip_num_dof(self.h)
+
     def spmv(self, x):
         y = np.zeros(self.n_dof)
         self._chk(self.lib.bhip_spmv(self.h, _d(_f64(x)), _d(y)))
@@ -517,3 +550,30 @@ class Server:
         if st != OK:
             raise BhipError(st, d["message"])
         return X, d
+
+
+def write_csr_file(path, fmt, ia, ja, a):
+    """bhip_write_csr_file: raw arrays in a reference file format (no GPU)."""
+    lib = load_library()
+    ia, ja, a = _i32(ia), _i32(ja), _f64(a)
+    st = lib.bhip_write_csr_file(str(path).encode(), fmt, ia.shape[0] - 1, ja.shape[0], ia.ctypes.data_as(_P_I),
+                                 ja.ctypes.data_as(_P_I), _d(a))
+    if st != OK:
+        raise BhipError(st, "bhip_write_csr_file failed")
+
+
+def write_vector(path, kind, v):
+    lib = load_library()
+    v = _f64(v)
+    st = lib.bhip_write_vector(str(path).encode(), kind, v.shape[0], _d(v))
+    if st != OK:
+        raise BhipError(st, "bhip_write_vector failed")
+
+
+def read_vector(path, kind, n):
+    lib = load_library()
+    v = np.zeros(n)
+    st = lib.bhip_read_vector(str(path).encode(), kind, n, _d(v))
+    if st != OK:
+        raise BhipError(st, "bhip_read_vector failed")
+    return v

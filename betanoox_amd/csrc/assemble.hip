@@ -436,6 +436,8 @@ int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, d
 }  // namespace
 
 int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double *u_state) {
+    if (ctx->external_matrix || ctx->n_nodes == 0)
+        return bhip_fail(ctx, BHIP_ERR_STATE, "no mesh: the matrix was loaded from CSR/RCV data or the mesh is not set");
     if (!ctx->pattern_ready) return bhip_fail(ctx, BHIP_ERR_STATE, "pattern not built");
     if (!ctx->have_mat) return bhip_fail(ctx, BHIP_ERR_STATE, "materials not set");
     BHIP_PROPAGATE(bhip_ensure_vectors(ctx));

@@ -126,6 +126,7 @@ struct bhip_ctx {
     int gauss_nnpe = 0, gauss_order = 0, gauss_nip = 0;
     bool assembled = false;
     int asm_kind = 0;
+    bool external_matrix = false;  // matrix loaded from CSR/RCV data, no mesh behind it
     // ---- lift table of the last stiffness assembly: for every node with free
     // DOFs, its neighbours with restrained DOFs and their FP64 coupling blocks
     bool lift_ready = false;

@@ -149,6 +149,7 @@ int bhip_set_mesh(bhip_ctx *ctx, int dim, int n_nodes, const double *xyz, int n_
     ctx->uniform_ndof = uni;
     ctx->pattern_ready = ctx->assembled = ctx->f32_ready = false;
     ctx->have_rest = false;
+    ctx->external_matrix = false;
     hipStream_t s = ctx->stream;
     BHIP_TRY(ctx, ctx->xyz.resize((size_t)n_nodes * dim));
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->xyz.p, xyz, sizeof(double) * n_nodes * dim, hipMemcpyHostToDevice, s));
@@ -174,6 +175,7 @@ int bhip_num_dof(const bhip_ctx *ctx) { return ctx ? ctx->n_dof : BHIP_ERR_ARG; 
 
 int bhip_get_dof_first(bhip_ctx *ctx, int *dof_first) {
     if (!ctx || !dof_first) return BHIP_ERR_ARG;
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
     std::copy(ctx->h_dof_first.begin(), ctx->h_dof_first.end(), dof_first);
     return BHIP_OK;
 }
@@ -195,7 +197,7 @@ extern "C" {
 
 int bhip_set_restraints(bhip_ctx *ctx, const unsigned char *restrained, const double *values) {
     if (!ctx || !restrained) return BHIP_ERR_ARG;
-    if (ctx->n_dof == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
     (void)hipSetDevice(ctx->device);
     std::vector<unsigned char> r(restrained, restrained + ctx->n_dof);
     for (auto &v : r) v = v ? 1 : 0;
@@ -217,6 +219,7 @@ int bhip_set_restraints(bhip_ctx *ctx, const unsigned char *restrained, const do
 
 int bhip_set_prescribed(bhip_ctx *ctx, const double *values) {
     if (!ctx || !values) return BHIP_ERR_ARG;
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->presc.p, values, sizeof(double) * ctx->n_dof, hipMemcpyHostToDevice, ctx->stream));
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return BHIP_OK;
@@ -224,7 +227,7 @@ int bhip_set_prescribed(bhip_ctx *ctx, const double *values) {
 
 int bhip_build_pattern(bhip_ctx *ctx) {
     if (!ctx) return BHIP_ERR_ARG;
-    if (ctx->n_dof == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
     (void)hipSetDevice(ctx->device);
     return bhip_pattern_build(ctx);
 }
@@ -337,7 +340,7 @@ int bhip_get_pseudo_load(bhip_ctx *ctx, double *f) {
 
 int bhip_set_load(bhip_ctx *ctx, const double *load) {
     if (!ctx) return BHIP_ERR_ARG;
-    if (ctx->n_dof == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
     if (load) {
         // addLoadToEqn: a load on a restrained DOF is a fatal error (equation.cpp:862)
         for (int i = 0; i < ctx->n_dof; i++)
@@ -361,7 +364,7 @@ __global__ void k_build_rhs(int n, const unsigned char *rest, const double *pres
 
 int bhip_build_rhs(bhip_ctx *ctx) {
     if (!ctx) return BHIP_ERR_ARG;
-    if (ctx->n_dof == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
     k_build_rhs<<<grid_for(ctx->n_dof, 256), 256, 0, ctx->stream>>>(ctx->n_dof, ctx->rest.p, ctx->presc.p,
                                                                     ctx->load.p, ctx->pseudo.p, ctx->rhs.p);
     BHIP_TRY(ctx, hipGetLastError());

@@ -223,6 +223,46 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
  * accumulated displacement (n_dof); any pointer may be NULL. */
 int bhip_plastic_get_state(bhip_ctx *ctx, double *stress, double *eps_bar, double *u_total);
 
+/* ------------------------------------------------------ exchange formats */
+#define BHIP_FMT_RCV_LOWER   1   /* "1", "L nnz", rows of "\t i j value" (lower, 0-based):
+                                    SymmetricSparseMatrix::printMatrixAddressFormat (sparse.cpp:305-325) */
+#define BHIP_FMT_RCV_UPPER   2   /* "2", upper rows: printMatrixAddressFormatUpper (sparse.cpp:266-290) */
+#define BHIP_FMT_RCV         5   /* reading only: any RCV type 1-3 (SparseMatrixInterface::ReadMatrix,
+                                    sparseMatrixInterface.cpp:51-101) */
+#define BHIP_FMT_CSR_ASCII   3   /* writeCSRMatrixInASCII (SparseMatrixUtility.cpp:11-26) of the PARDISO
+                                    storage (Pardiso.cpp:208-211): upper triangle, 1-based */
+#define BHIP_FMT_CSR_BINARY  4   /* writeCSRMatrixInBinary (SparseMatrixUtility.cpp:28-38), same storage */
+
+#define BHIP_TRI_LOWER       1
+#define BHIP_TRI_UPPER       2
+#define BHIP_TRI_FULL        3
+
+#define BHIP_VEC_LOAD        1   /* LoadVector.txt (equation.cpp:1357-1362): one value per line */
+#define BHIP_VEC_SOLUTION    2   /* Solution.txt (equation.cpp:1378-1383): count, then one value per line */
+
+/* Write the context's matrix in a reference format (the printCSRFormat* /
+ * printMatrixAddressFormat dumps of Equations::solve, equation.cpp:1345-1356). */
+int bhip_write_matrix(bhip_ctx *ctx, const char *path, int format);
+/* Write raw CSR arrays in a reference format (CSR formats: as given; RCV
+ * formats: 0-based rows of the triangle to write).  No GPU needed. */
+int bhip_write_csr_file(const char *path, int format, int n, long long nnz, const int *ia, const int *ja,
+                        const double *a);
+/* Load a matrix into the context instead of assembling one: a triangle
+ * (expanded to the full symmetric matrix) or the full matrix, 0- or 1-based.
+ * Every row needs its diagonal.  The context then solves with bhip_set_rhs +
+ * bhip_solve / bhip_pcg_solve (FP32 inner values in CSR); mesh, assembly and
+ * load calls report BHIP_ERR_STATE until bhip_set_mesh. */
+int bhip_set_csr(bhip_ctx *ctx, int n, long long nnz, const int *ia, const int *ja, const double *a, int base,
+                 int triangle);
+/* Read a matrix file written by the reference (or by bhip_write_matrix) into
+ * the context (readCSRMatrixInASCII/Binary, SparseMatrixUtility.cpp:40-78;
+ * ReadMatrix for RCV); index base and triangle are taken from the file. */
+int bhip_read_matrix(bhip_ctx *ctx, const char *path, int format);
+/* LoadVector.txt / Solution.txt writers and readers (readLoadVector,
+ * SparseMatrixUtility.cpp:79-85).  No GPU needed. */
+int bhip_write_vector(const char *path, int kind, int n, const double *v);
+int bhip_read_vector(const char *path, int kind, int n, double *v);
+
 /* ------------------------------------------------------ transient diffusion */
 #define BHIP_HEAT_X0_ZERO       0   /* initial guess du = 0 */
 #define BHIP_HEAT_X0_PREVIOUS   1   /* du of the previous step (0 on the first) */

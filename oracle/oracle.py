@@ -320,8 +320,59 @@ def ref_lib():
     R._Z9derivHex8PdS_S_ddd.argtypes = [_P_D, _P_D, _P_D, C.c_double, C.c_double, C.c_double]
     R._Z10shapeQuad4Pddd.argtypes = [_P_D, C.c_double, C.c_double]
     R._Z10derivQuad4PdS_dd.argtypes = [_P_D, _P_D, C.c_double, C.c_double]
+    ppi, ppd = C.POINTER(_P_I), C.POINTER(_P_D)
+    # SparseMatrixUtility.cpp:11-85 (int*& / double*& arguments are pointers to pointers)
+    R._Z21writeCSRMatrixInASCIIPcRPiS1_RPdii.argtypes = [C.c_char_p, ppi, ppi, ppd, C.c_int, C.c_int]
+    R._Z22writeCSRMatrixInBinaryPcRPiS1_RPdii.argtypes = [C.c_char_p, ppi, ppi, ppd, C.c_int, C.c_int]
+    R._Z20readCSRMatrixInASCIIPcRPiS1_RPdRi.argtypes = [C.c_char_p, ppi, ppi, ppd, _P_I]
+    R._Z21readCSRMatrixInBinaryPcRPiS1_RPdRiS4_.argtypes = [C.c_char_p, ppi, ppi, ppd, _P_I, _P_I]
+    R._Z14readLoadVectorPcPdi.argtypes = [C.c_char_p, _P_D, C.c_int]
+    for f in (R._Z21writeCSRMatrixInASCIIPcRPiS1_RPdii, R._Z22writeCSRMatrixInBinaryPcRPiS1_RPdii,
+              R._Z20readCSRMatrixInASCIIPcRPiS1_RPdRi, R._Z21readCSRMatrixInBinaryPcRPiS1_RPdRiS4_,
+              R._Z14readLoadVectorPcPdi):
+        f.restype = None
     _ref = R
     return R
+
+
+def ref_write_csr(path, binary, ia, ja, a):
+    """The reference's writeCSRMatrixInASCII / writeCSRMatrixInBinary
+    (SparseMatrixUtility.cpp:11-38) on the given arrays."""
+    R = ref_lib()
+    ia, ja, a = ai32(ia), ai32(ja), f64(a)
+    pia, pja, pa = i32(ia), i32(ja), d(a)
+    fn = R._Z22writeCSRMatrixInBinaryPcRPiS1_RPdii if binary else R._Z21writeCSRMatrixInASCIIPcRPiS1_RPdii
+    fn(str(path).encode(), C.byref(pia), C.byref(pja), C.byref(pa), ia.shape[0] - 1, ja.shape[0])
+
+
+def ref_read_csr(path, binary):
+    """The reference's readCSRMatrixInASCII / readCSRMatrixInBinary
+    (SparseMatrixUtility.cpp:40-78); returns (ia, ja, a) as numpy copies."""
+    R = ref_lib()
+    pia, pja, pa = _P_I(), _P_I(), _P_D()
+    n = C.c_int()
+    if binary:
+        nc = C.c_int()
+        R._Z21readCSRMatrixInBinaryPcRPiS1_RPdRiS4_(str(path).encode(), C.byref(pia), C.byref(pja), C.byref(pa),
+                                                     C.byref(n), C.byref(nc))
+        m = nc.value
+    else:
+        R._Z20readCSRMatrixInASCIIPcRPiS1_RPdRi(str(path).encode(), C.byref(pia), C.byref(pja), C.byref(pa),
+                                                C.byref(n))
+        m = None
+    ia = np.ctypeslib.as_array(pia, shape=(n.value + 1,)).copy()
+    if m is None:
+        m = int(ia[-1] - ia[0])
+    ja = np.ctypeslib.as_array(pja, shape=(m,)).copy()
+    a = np.ctypeslib.as_array(pa, shape=(m,)).copy()
+    return ia, ja, a
+
+
+def ref_read_load_vector(path, n):
+    """The reference's readLoadVector (SparseMatrixUtility.cpp:79-85)."""
+    v = np.zeros(n)
+    ref_lib()._Z14readLoadVectorPcPdi(str(path).encode(), d(v), n)
+    return v
 
 
 # ------------------------------------------------------------ plasticity
