@@ -64,8 +64,11 @@ __device__ __forceinline__ void range_of(long long n, int lb, int nb, long long 
 }
 
 // y_r = K x_r for all R cases (SELL-64 3x3 blocks, lane = node row);
-// partials x_r . y_r.
-template <int R>
+// partials x_r . y_r.  SPLIT = 1: each of the 4 waves of a block owns one
+// slice.  SPLIT = 4 (small matrices, too few slices to fill the GPU): the 4
+// waves share one slice, each taking every fourth block column, and wave 0
+// adds the other waves' row sums in wave order through LDS.
+template <int R, int SPLIT>
 __global__ void __launch_bounds__(kThreads) k_sell3_spmv_multi(int n_nodes, int n_slices, const int *__restrict__ sptr,
                                                              const int *__restrict__ scol,
                                                              const float *__restrict__ sval, long long nsell,
@@ -74,8 +77,8 @@ __global__ void __launch_bounds__(kThreads) k_sell3_spmv_multi(int n_nodes, int 
                                                              double *partials) {
     if (st->all_done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x & 63;
-    const long long s = (long long)lb * (kThreads / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long s = SPLIT == 1 ? (long long)lb * (kThreads / 64) + wave : (long long)lb;
     double dots[R];
 #pragma unroll
     for (int r = 0; r < R; r++) dots[r] = 0;
@@ -86,7 +89,7 @@ __global__ void __launch_bounds__(kThreads) k_sell3_spmv_multi(int n_nodes, int 
 #pragma unroll
         for (int r = 0; r < R; r++) a0[r] = a1[r] = a2[r] = 0.f;
 #pragma unroll 2
-        for (int k = kb + lane; k < ke; k += 64) {
+        for (int k = kb + lane + (SPLIT == 1 ? 0 : 64 * wave); k < ke; k += 64 * SPLIT) {
             const long long m = scol[k];
             const float v0 = sval[0 * nsell + k], v1 = sval[1 * nsell + k], v2 = sval[2 * nsell + k];
             const float v3 = sval[3 * nsell + k], v4 = sval[4 * nsell + k], v5 = sval[5 * nsell + k];
@@ -100,7 +103,26 @@ __global__ void __launch_bounds__(kThreads) k_sell3_spmv_multi(int n_nodes, int 
                 a2[r] += v6 * x0 + v7 * x1 + v8 * x2;
             }
         }
-        if (node < n_nodes) {
+        if (SPLIT > 1) {
+            __shared__ float red[3 * R][kThreads];
+            if (wave > 0)
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    red[3 * r][threadIdx.x] = a0[r];
+                    red[3 * r + 1][threadIdx.x] = a1[r];
+                    red[3 * r + 2][threadIdx.x] = a2[r];
+                }
+            __syncthreads();
+            if (wave == 0)
+                for (int w = 1; w < SPLIT; w++)
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        a0[r] += red[3 * r][64 * w + lane];
+                        a1[r] += red[3 * r + 1][64 * w + lane];
+                        a2[r] += red[3 * r + 2][64 * w + lane];
+                    }
+        }
+        if (node < n_nodes && (SPLIT == 1 || wave == 0)) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const long long j = r * ld + 3 * node;
@@ -114,13 +136,27 @@ __global__ void __launch_bounds__(kThreads) k_sell3_spmv_multi(int n_nodes, int 
     store_partials_n<R>(dots, partials);
 }
 
+// Finalize kernels: thread r < R owns case r.  Its state is loaded at the
+// start so that the loads overlap the partial sums; the block sum lands in
+// thread 0 and is passed to the case threads through LDS.
 template <int R>
 __global__ void __launch_bounds__(kFinThreads) k_fin_alpha_multi(int nb, const double *partials, MultiState *st) {
     if (st->all_done) return;
+    const int c = threadIdx.x;
+    int done = 1;
+    double rz = 0;
+    if (c < R) {
+        done = st->done[c];
+        rz = st->rz[c];
+    }
     double t[R];
     fin_sum_n<R>(partials, nb, t);
+    __shared__ double tot[R];
     if (threadIdx.x == 0)
-        for (int r = 0; r < R; r++) st->alpha[r] = st->done[r] ? 0.0 : st->rz[r] / t[r];
+#pragma unroll
+        for (int r = 0; r < R; r++) tot[r] = t[r];
+    __syncthreads();
+    if (c < R) st->alpha[c] = done ? 0.0 : rz / tot[c];
 }
 
 // Vector kernels run on a (blocks, R) grid: block (b, r) streams case r over
@@ -160,25 +196,41 @@ __global__ void __launch_bounds__(kThreads) k_update_multi(long long n, float *_
 template <int R>
 __global__ void __launch_bounds__(kFinThreads) k_fin_update_multi(int nb, const double *partials, MultiState *st) {
     if (st->all_done) return;
+    const int c = threadIdx.x;
+    int done = 1, iter = 0;
+    double rz = 0, rr0 = 1, tol = 0;
+    int maxit = 0;
+    if (c < R) {
+        done = st->done[c];
+        rz = st->rz[c];
+        rr0 = st->rr0[c];
+        iter = st->iter[c];
+        tol = st->tol;
+        maxit = st->maxit;
+    }
     double t[2 * R];
     fin_sum_n<2 * R>(partials, nb, t);
+    __shared__ double tot[2 * R];
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 2 * R; k++) tot[k] = t[k];
+    __syncthreads();
+    if (c < R && !done) {
+        const double rz_new = tot[2 * c];
+        st->beta[c] = rz_new / rz;
+        st->rz[c] = rz_new;
+        st->rr[c] = tot[2 * c + 1];
+        st->iter[c] = iter + 1;
+        const bool conv = sqrt(tot[2 * c + 1] / rr0) <= tol;
+        if (conv) st->converged[c] = 1;
+        if (conv || iter + 1 >= maxit) done = 1;
+        st->done[c] = done;
+    }
+    // the case threads are lanes 0..R-1 of wave 0
+    const unsigned long long ball = __ballot(c < R && done);
     if (threadIdx.x == 0) {
-        int all = 1;
-        for (int r = 0; r < R; r++) {
-            if (!st->done[r]) {
-                const double rz_new = t[2 * r];
-                st->beta[r] = rz_new / st->rz[r];
-                st->rz[r] = rz_new;
-                st->rr[r] = t[2 * r + 1];
-                st->iter[r] += 1;
-                const bool conv = sqrt(t[2 * r + 1] / st->rr0[r]) <= st->tol;
-                if (conv) st->converged[r] = 1;
-                if (conv || st->iter[r] >= st->maxit) st->done[r] = 1;
-            }
-            all &= st->done[r];
-        }
         st->iters += 1;
-        st->all_done = all;
+        st->all_done = (ball & ((1ull << R) - 1)) == ((1ull << R) - 1);
     }
 }
 
@@ -327,7 +379,14 @@ int vec_grid(bhip_ctx *ctx, long long n) {
     return (int)std::max<long long>(1, std::min(want, cap));
 }
 
+// Matrices with fewer slices than this run the SPLIT = 4 SpMV (one slice per
+// block): with one wave per slice they would leave most CUs idle.
+constexpr int kSplitSlices = 2048;
+
+bool sell_split(bhip_ctx *ctx) { return ctx->n_slices < kSplitSlices; }
+
 int sell_grid(bhip_ctx *ctx) {
+    if (sell_split(ctx)) return ctx->n_slices > 0 ? ctx->n_slices : 1;
     long long g = (ctx->n_slices + kThreads / 64 - 1) / (kThreads / 64);
     return (int)(g > 0 ? g : 1);
 }
@@ -437,9 +496,14 @@ void enqueue_iteration(bhip_ctx *ctx, MultiData &M) {
     hipStream_t s = ctx->stream;
     const long long n = ctx->n_dof;
     const int gs = sell_grid(ctx), gv = vec_grid(ctx, n);
-    k_sell3_spmv_multi<R><<<gs, kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p,
-                                                  ctx->bval32.p, ctx->n_sell, n, M.p32.p, M.q32.p, M.st.p,
-                                                  M.partials.p);
+    if (sell_split(ctx))
+        k_sell3_spmv_multi<R, 4><<<gs, kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
+                                                         ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, n, M.p32.p,
+                                                         M.q32.p, M.st.p, M.partials.p);
+    else
+        k_sell3_spmv_multi<R, 1><<<gs, kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
+                                                         ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, n, M.p32.p,
+                                                         M.q32.p, M.st.p, M.partials.p);
     k_fin_alpha_multi<R><<<1, kFinThreads, 0, s>>>(gs, M.partials.p, M.st.p);
     k_update_multi<R><<<dim3(gv, R), kThreads, 0, s>>>(n, M.d32.p, M.r32.p, M.p32.p, M.q32.p, ctx->dinv32.p,
                                                        M.st.p, M.partials.p);

@@ -223,43 +223,48 @@ __global__ void __launch_bounds__(kThreads) k_pcg_init(long long n, const T *__r
 }
 
 // ------------------------------------------------------ finalize kernels
+// The state a finalize kernel needs is read before the partial sums so that
+// its load latency overlaps theirs.
 __global__ void __launch_bounds__(kFinThreads) k_fin_alpha(int nb, const double *partials, PcgState *st) {
     if (st->done) return;
+    const double rz = st->rz;
     double t[1];
     fin_sum<1>(partials, nb, t);
     if (threadIdx.x == 0) {
         st->pAp = t[0];
-        st->alpha = st->rz / t[0];
+        st->alpha = rz / t[0];
     }
 }
 
 // beta, stopping rule (sparse.cpp:202-223), iteration count.
 __global__ void __launch_bounds__(kFinThreads) k_fin_update(int nb, const double *partials, PcgState *st) {
     if (st->done) return;
+    const double rz = st->rz, tol = st->tol, bb = st->bb, rr0 = st->rr0;
+    const int stop_rule = st->stop_rule, iter = st->iter + 1, maxit = st->maxit;
     double t[3];
     fin_sum<3>(partials, nb, t);
     if (threadIdx.x == 0) {
         double rz_new = t[0];
-        st->beta = rz_new / st->rz;
+        st->beta = rz_new / rz;
         st->rz = rz_new;
         st->rr = t[1];
         st->xx = t[2];
-        st->iter += 1;
+        st->iter = iter;
         double nrm;
         bool conv;
-        if (st->stop_rule == BHIP_STOP_REF_R_OVER_X) {
+        if (stop_rule == BHIP_STOP_REF_R_OVER_X) {
             nrm = sqrt(t[1] / t[2]);
-            conv = nrm < st->tol;
-        } else if (st->stop_rule == BHIP_STOP_REL_RESID) {
-            nrm = sqrt(t[1] / st->bb);
-            conv = nrm <= st->tol;
+            conv = nrm < tol;
+        } else if (stop_rule == BHIP_STOP_REL_RESID) {
+            nrm = sqrt(t[1] / bb);
+            conv = nrm <= tol;
         } else {  // inner solve: relative reduction of |r|
-            nrm = sqrt(t[1] / st->rr0);
-            conv = nrm <= st->tol;
+            nrm = sqrt(t[1] / rr0);
+            conv = nrm <= tol;
         }
         st->norm = nrm;
         if (conv) st->converged = 1;
-        if (conv || st->iter >= st->maxit) st->done = 1;
+        if (conv || iter >= maxit) st->done = 1;
     }
 }
 
