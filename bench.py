@@ -161,6 +161,8 @@ def heat_leg(n, steps, fp64_steps):
         t0 = time.perf_counter()
         ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[1.0, prob.extra["dt"]], u_state=u0)
         out["assembly_ms"] = 1e3 * (time.perf_counter() - t0)
+        chosen, table = ctx.tune_spmv(reps=10)
+        out["spmv_tuning"] = {"chosen": table[chosen], "shape0_ms": table[0]["ms"]}
         o32 = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
                                 inner_tol=1e-3, maxit=40000)
         ctx.heat_run(2, u0, capi.HEAT_X0_PREVIOUS, opts=o32)  # warm-up (graphs, tables)
@@ -269,6 +271,11 @@ def main():
         }
         log("fp64 PCG: %s" % json.dumps(result_fp64))
 
+    # SpMV tile shape: the autotuner times every shape on this matrix and keeps the fastest
+    chosen, table = ctx.tune_spmv(reps=10)
+    tuning = {"chosen": table[chosen], "shape0_ms": table[0]["ms"], "shapes_timed": len(table)}
+    log("spmv tuning: %s" % json.dumps(tuning))
+
     opts = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
                              inner_tol=args.inner_tol, maxit=40000)
     for w in range(args.warmup):
@@ -290,9 +297,11 @@ def main():
     ms32, by32 = ctx.bench_spmv(fp32=True, reps=50)
     achieved = by32 / ms32 / 1e6
     _, stream_gbps = ctx.bench_stream(1 << 30, 20)
+    sh = table[chosen]
+    kname = "k_sell_spmv<3, 1, %d, %d, %d>" % (sh["slices_per_block"], sh["unroll"], sh["nontemporal"])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.n, "k_sell_spmv<3, 1>"),
-                "kernel": "k_sell_spmv<3, 1> (FP32 3x3-block sliced-ELL SpMV with p.Ap partials)",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.n, kname),
+                "kernel": kname + " (FP32 3x3-block sliced-ELL SpMV with p.Ap partials)",
                 "bytes_per_launch": by32, "ms_per_launch": ms32}
 
     line = {
@@ -320,6 +329,7 @@ def main():
         "plain_fp64": result_fp64,
         "assembly": {"ms": 1e3 * min(t_asm), "elements_per_s": prob.n_el / min(t_asm)},
         "spmv_fp32_GBps": achieved,
+        "spmv_tuning": tuning,
         "stream_copy_GBps": stream_gbps,
     }
     if result_fp64:

@@ -68,6 +68,7 @@ EXPORTS = [
     "bhip_server_create", "bhip_server_destroy", "bhip_server_submit", "bhip_server_poll", "bhip_server_wait",
     "bhip_server_budget", "bhip_request_footprint", "bhip_heat_run", "bhip_write_matrix", "bhip_write_csr_file",
     "bhip_set_csr", "bhip_read_matrix", "bhip_write_vector", "bhip_read_vector",
+    "bhip_tune_spmv", "bhip_set_spmv_shape", "bhip_get_spmv_shape", "bhip_spmv_shape_describe",
 ]
 
 
@@ -127,6 +128,14 @@ class PlasticStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SpmvShape(C.Structure):
+    _fields_ = [("id", C.c_int), ("slices_per_block", C.c_int), ("unroll", C.c_int), ("nontemporal", C.c_int),
+                ("ms", C.c_double), ("gbps", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BhipError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("status %d: %s" % (status, msg))
@@ -181,6 +190,10 @@ def load_library(path=LIB_PATH):
         "bhip_solve_cases": (C.c_int, [vp, C.c_int, _P_D, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),
         "bhip_bench_stream": (C.c_int, [vp, C.c_longlong, C.c_int, _P_D, _P_D]),
+        "bhip_tune_spmv": (C.c_int, [vp, C.c_int, C.POINTER(SpmvShape), C.c_int, _P_I, _P_I]),
+        "bhip_set_spmv_shape": (C.c_int, [vp, C.c_int]),
+        "bhip_get_spmv_shape": (C.c_int, [vp]),
+        "bhip_spmv_shape_describe": (C.c_int, [C.c_int, C.POINTER(SpmvShape)]),
         "bhip_plastic_init": (C.c_int, [vp]),
         "bhip_plastic_update": (C.c_int, [vp, _P_D]),
         "bhip_plastic_residual": (C.c_int, [vp, _P_D, _P_D]),
@@ -474,6 +487,20 @@ class Context:
         self._chk(self.lib.bhip_bench_spmv(self.h, int(fp32), reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def tune_spmv(self, reps=10):
+        """bhip_tune_spmv: time every SELL tile shape on the assembled matrix,
+        keep the fastest.  Returns (chosen id or -1, list of shape dicts)."""
+        table = (SpmvShape * 64)()
+        n, chosen = C.c_int(), C.c_int()
+        self._chk(self.lib.bhip_tune_spmv(self.h, reps, table, 64, C.byref(n), C.byref(chosen)))
+        return chosen.value, [table[i].as_dict() for i in range(n.value)]
+
+    def set_spmv_shape(self, shape_id):
+        self._chk(self.lib.bhip_set_spmv_shape(self.h, int(shape_id)))
+
+    def spmv_shape(self):
+        return self.lib.bhip_get_spmv_shape(self.h)
+
 
 class Server:
     """Resident request loop (bhip_server_*): hex8 elasticity requests with
@@ -550,6 +577,16 @@ class Server:
         if st != OK:
             raise BhipError(st, d["message"])
         return X, d
+
+
+def spmv_shapes():
+    """Every SELL tile shape the library knows (bhip_spmv_shape_describe)."""
+    lib = load_library()
+    out = []
+    info = SpmvShape()
+    while lib.bhip_spmv_shape_describe(len(out), C.byref(info)) == 0:
+        out.append(info.as_dict())
+    return out
 
 
 def write_csr_file(path, fmt, ia, ja, a):

@@ -113,6 +113,9 @@ struct bhip_ctx {
     long long n_sell = 0;    // padded block count
     dvec<int> sell_ptr, sell_col;
     dvec<float> bval32;      // SELL block values, SoA: B*B x n_sell
+    int sell_shape = 0;      // tile shape id of the FP32 SELL SpMV
+    bool sell_shape_set = false;  // chosen by bhip_tune_spmv / bhip_set_spmv_shape (else the
+                                  // tuned default table picks it when the SELL layout is built)
     // ---- materials / integration
     int mat_kind = 0, n_groups = 0, mat_stride = 0;
     dvec<double> mat;
@@ -224,6 +227,12 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *opts, bhip_stats *stat
 int bhip_spmv_dev(bhip_ctx *ctx, const double *x, double *y);
 int bhip_export_lower(bhip_ctx *ctx, int *rowptr, int *colind, double *val);
 int bhip_bench_spmv_impl(bhip_ctx *ctx, int fp32, int reps, double *ms, double *bytes);
+int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max_entries, int *n_entries,
+                        int *chosen);
+int bhip_spmv_shape_info(int id, bhip_spmv_shape *info);
+int bhip_set_spmv_shape_impl(bhip_ctx *ctx, int id);
+// 1 if the context's SELL shape uses streaming loads (the multi-case SpMV follows it)
+bool bhip_sell_streams(const bhip_ctx *ctx);
 void bhip_invalidate_graphs(bhip_ctx *ctx);
 void bhip_plastic_release(bhip_ctx *ctx);
 void bhip_multi_invalidate(bhip_ctx *ctx);

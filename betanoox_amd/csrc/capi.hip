@@ -437,6 +437,26 @@ int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *b
     return bhip_bench_spmv_impl(ctx, fp32, reps, ms_avg, bytes);
 }
 
+int bhip_tune_spmv(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max_entries, int *n_entries,
+                   int *chosen) {
+    if (!ctx || !n_entries || !chosen || (max_entries > 0 && !table)) return BHIP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    return bhip_tune_spmv_impl(ctx, reps, table, max_entries, n_entries, chosen);
+}
+
+int bhip_set_spmv_shape(bhip_ctx *ctx, int id) {
+    if (!ctx) return BHIP_ERR_ARG;
+    return bhip_set_spmv_shape_impl(ctx, id);
+}
+
+int bhip_get_spmv_shape(const bhip_ctx *ctx) { return ctx ? ctx->sell_shape : BHIP_ERR_ARG; }
+
+int bhip_spmv_shape_describe(int id, bhip_spmv_shape *info) {
+    if (!info) return BHIP_ERR_ARG;
+    memset(info, 0, sizeof(*info));
+    return bhip_spmv_shape_info(id, info);
+}
+
 }  // extern "C"
 
 namespace {

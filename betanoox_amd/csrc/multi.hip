@@ -67,8 +67,16 @@ __device__ __forceinline__ void range_of(long long n, int lb, int nb, long long 
 // partials x_r . y_r.  SPLIT = 1: each of the 4 waves of a block owns one
 // slice.  SPLIT = 4 (small matrices, too few slices to fill the GPU): the 4
 // waves share one slice, each taking every fourth block column, and wave 0
-// adds the other waves' row sums in wave order through LDS.
-template <int R, int SPLIT>
+// adds the other waves' row sums in wave order through LDS.  NT: streaming
+// (nontemporal) loads of block values and columns, used with the single-case
+// shapes that stream (matrices larger than the last-level cache).
+template <int NT, class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <int R, int SPLIT, int NT>
 __global__ void __launch_bounds__(kThreads) k_sell3_spmv_multi(int n_nodes, int n_slices, const int *__restrict__ sptr,
                                                              const int *__restrict__ scol,
                                                              const float *__restrict__ sval, long long nsell,
@@ -90,10 +98,12 @@ __global__ void __launch_bounds__(kThreads) k_sell3_spmv_multi(int n_nodes, int 
         for (int r = 0; r < R; r++) a0[r] = a1[r] = a2[r] = 0.f;
 #pragma unroll 2
         for (int k = kb + lane + (SPLIT == 1 ? 0 : 64 * wave); k < ke; k += 64 * SPLIT) {
-            const long long m = scol[k];
-            const float v0 = sval[0 * nsell + k], v1 = sval[1 * nsell + k], v2 = sval[2 * nsell + k];
-            const float v3 = sval[3 * nsell + k], v4 = sval[4 * nsell + k], v5 = sval[5 * nsell + k];
-            const float v6 = sval[6 * nsell + k], v7 = sval[7 * nsell + k], v8 = sval[8 * nsell + k];
+            const long long m = ld_stream<NT>(scol + k);
+            const float v0 = ld_stream<NT>(sval + 0 * nsell + k), v1 = ld_stream<NT>(sval + 1 * nsell + k);
+            const float v2 = ld_stream<NT>(sval + 2 * nsell + k), v3 = ld_stream<NT>(sval + 3 * nsell + k);
+            const float v4 = ld_stream<NT>(sval + 4 * nsell + k), v5 = ld_stream<NT>(sval + 5 * nsell + k);
+            const float v6 = ld_stream<NT>(sval + 6 * nsell + k), v7 = ld_stream<NT>(sval + 7 * nsell + k);
+            const float v8 = ld_stream<NT>(sval + 8 * nsell + k);
             const float *xm = x + 3 * m;
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -496,14 +506,19 @@ void enqueue_iteration(bhip_ctx *ctx, MultiData &M) {
     hipStream_t s = ctx->stream;
     const long long n = ctx->n_dof;
     const int gs = sell_grid(ctx), gv = vec_grid(ctx, n);
-    if (sell_split(ctx))
-        k_sell3_spmv_multi<R, 4><<<gs, kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
-                                                         ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, n, M.p32.p,
-                                                         M.q32.p, M.st.p, M.partials.p);
-    else
-        k_sell3_spmv_multi<R, 1><<<gs, kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,
-                                                         ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, n, M.p32.p,
-                                                         M.q32.p, M.st.p, M.partials.p);
+#define BHIP_MULTI_SPMV(SPLIT, NT)                                                                           \
+    k_sell3_spmv_multi<R, SPLIT, NT><<<gs, kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,   \
+                                                             ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, n, \
+                                                             M.p32.p, M.q32.p, M.st.p, M.partials.p)
+    const bool nt = bhip_sell_streams(ctx);
+    if (sell_split(ctx)) {
+        if (nt) BHIP_MULTI_SPMV(4, 1);
+        else BHIP_MULTI_SPMV(4, 0);
+    } else {
+        if (nt) BHIP_MULTI_SPMV(1, 1);
+        else BHIP_MULTI_SPMV(1, 0);
+    }
+#undef BHIP_MULTI_SPMV
     k_fin_alpha_multi<R><<<1, kFinThreads, 0, s>>>(gs, M.partials.p, M.st.p);
     k_update_multi<R><<<dim3(gv, R), kThreads, 0, s>>>(n, M.d32.p, M.r32.p, M.p32.p, M.q32.p, ctx->dinv32.p,
                                                        M.st.p, M.partials.p);

@@ -302,6 +302,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     ctx->pattern_ready = true;
     ctx->assembled = false;
     ctx->f32_ready = false;
+    ctx->sell_shape_set = false;  // a tuned shape belongs to the pattern it was measured on
     ctx->val32.release();
     ctx->bval32.release();
     ctx->sell_ptr.release();

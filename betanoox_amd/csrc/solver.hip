@@ -20,9 +20,12 @@
 // FP32 values in B x B node blocks laid out as sliced ELL (SELL-64, one lane
 // per node row) when every node carries the same number B <= 3 of DOFs, and
 // FP32 CSR otherwise.
+#include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include "bhip_internal.hpp"
+#include "tuned_shapes.hpp"
 
 namespace {
 
@@ -112,9 +115,23 @@ __global__ void __launch_bounds__(kThreads) k_csr_spmv(int n, const int *__restr
 // entry of the B x B block) and column indices are laid out so that the 64
 // lanes of a wave read 64 consecutive entries and consecutive nodes gather
 // neighbouring x entries.  No cross-lane reduction; padded blocks carry zeros
-// and the node's own column.  4 slices per block.
-template <int B, int MODE>
-__global__ void __launch_bounds__(kThreads) k_sell_spmv(int n_nodes, int n_slices,
+// and the node's own column.
+//
+// Tile shape (chosen per matrix by bhip_tune_spmv): SPB slices per workgroup
+// (one wave each, SPB = 4, 8 or 16), UNR block columns per loop trip, NT
+// streaming (nontemporal) loads of block values and columns, which are read
+// once per SpMV.  Every shape produces the same bits: each lane sums its block
+// columns in order, and the p.q partials are always one per group of four
+// consecutive slices (wave sums added in wave order), whatever the workgroup
+// size, so the finalize sums the same records in the same order.
+template <int NT, class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <int B, int MODE, int SPB, int UNR, int NT>
+__global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slices,
                                                       const int *__restrict__ sptr,
                                                       const int *__restrict__ scol,
                                                       const float *__restrict__ sval, long long nsell,
@@ -123,8 +140,8 @@ __global__ void __launch_bounds__(kThreads) k_sell_spmv(int n_nodes, int n_slice
                                                       double *partials) {
     if (MODE == 1 && st->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x & 63;
-    const long long s = (long long)lb * (kThreads / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long s = (long long)lb * SPB + wave;
     double acc = 0;
     if (s < n_slices) {
         const long long node = s * 64 + lane;
@@ -132,17 +149,17 @@ __global__ void __launch_bounds__(kThreads) k_sell_spmv(int n_nodes, int n_slice
         float a[B];
 #pragma unroll
         for (int c = 0; c < B; c++) a[c] = 0.f;
-#pragma unroll 4
+#pragma unroll UNR
         for (int k = kb + lane; k < ke; k += 64) {
-            const long long m = scol[k];
+            const long long m = ld_stream<NT>(scol + k);
             float xv[B];
 #pragma unroll
             for (int d = 0; d < B; d++) xv[d] = x[B * m + d];
 #pragma unroll
             for (int c = 0; c < B; c++) {
-                float t = sval[(c * B) * nsell + k] * xv[0];
+                float t = ld_stream<NT>(sval + (c * B) * nsell + k) * xv[0];
 #pragma unroll
-                for (int d = 1; d < B; d++) t += sval[(c * B + d) * nsell + k] * xv[d];
+                for (int d = 1; d < B; d++) t += ld_stream<NT>(sval + (c * B + d) * nsell + k) * xv[d];
                 a[c] += t;
             }
         }
@@ -155,8 +172,17 @@ __global__ void __launch_bounds__(kThreads) k_sell_spmv(int n_nodes, int n_slice
         }
     }
     if (MODE == 1) {
-        double v[1] = {acc};
-        store_partials<1>(v, partials);
+        // one partial per group of four slices: wave sums added in wave order
+        __shared__ double sm[SPB];
+        acc = wave_sum(acc);
+        if (lane == 0) sm[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x < SPB / 4) {
+            double t = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) t += sm[4 * threadIdx.x + w];
+            partials[(long long)lb * (SPB / 4) + threadIdx.x] = t;
+        }
     }
 }
 
@@ -413,8 +439,9 @@ int csr_grid(bhip_ctx *ctx) {
     return (int)(g > 0 ? g : 1);
 }
 
-int sell_grid(bhip_ctx *ctx) {
-    long long g = (ctx->n_slices + kThreads / 64 - 1) / (kThreads / 64);
+// p.q partial records of the SELL SpMV: one per group of four slices.
+int sell_groups(bhip_ctx *ctx) {
+    long long g = (ctx->n_slices + 3) / 4;
     return (int)(g > 0 ? g : 1);
 }
 
@@ -437,20 +464,43 @@ void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *
 #undef BHIP_CSR_CASE
 }
 
-void launch_sell(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgState *st) {
-    hipStream_t s = ctx->stream;
-#define BHIP_SELL(BB, MM)                                                                                      \
-    k_sell_spmv<BB, MM><<<sell_grid(ctx), kThreads, 0, s>>>(ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p,     \
-                                                            ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, \
-                                                            st, ctx->partials.p)
-    if (ctx->sell_b == 3) {
-        if (mode == 1) BHIP_SELL(3, 1); else BHIP_SELL(3, 0);
-    } else if (ctx->sell_b == 2) {
-        if (mode == 1) BHIP_SELL(2, 1); else BHIP_SELL(2, 0);
-    } else {
-        if (mode == 1) BHIP_SELL(1, 1); else BHIP_SELL(1, 0);
+template <int B, int MODE, int SPB, int UNR, int NT>
+void launch_sell_t(bhip_ctx *ctx, const float *x, float *y, const PcgState *st) {
+    long long g = (ctx->n_slices + SPB - 1) / SPB;
+    k_sell_spmv<B, MODE, SPB, UNR, NT><<<(int)(g > 0 ? g : 1), 64 * SPB, 0, ctx->stream>>>(
+        ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
+        ctx->partials.p);
+}
+
+// Tile shapes in shape-id order (bhip_tune_spmv reports and sets the id).
+constexpr int kNumSellShapes = 18;
+#define BHIP_SELL_SHAPES(X)                                                                               \
+    X(0, 4, 4, 0) X(1, 4, 2, 0) X(2, 4, 8, 0) X(3, 8, 4, 0) X(4, 8, 2, 0) X(5, 8, 8, 0) X(6, 16, 4, 0)    \
+    X(7, 16, 2, 0) X(8, 16, 8, 0) X(9, 4, 4, 1) X(10, 4, 2, 1) X(11, 4, 8, 1) X(12, 8, 4, 1)             \
+    X(13, 8, 2, 1) X(14, 8, 8, 1) X(15, 16, 4, 1) X(16, 16, 2, 1) X(17, 16, 8, 1)
+
+template <int B>
+void launch_sell_b(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgState *st) {
+    if (mode == 0) {
+        launch_sell_t<B, 0, 4, 4, 0>(ctx, x, y, st);
+        return;
     }
-#undef BHIP_SELL
+    switch (ctx->sell_shape) {
+#define BHIP_SHAPE_CASE(ID, SPB, UNR, NT) \
+    case ID:                              \
+        launch_sell_t<B, 1, SPB, UNR, NT>(ctx, x, y, st); \
+        break;
+        BHIP_SELL_SHAPES(BHIP_SHAPE_CASE)
+#undef BHIP_SHAPE_CASE
+        default:
+            launch_sell_t<B, 1, 4, 4, 0>(ctx, x, y, st);
+    }
+}
+
+void launch_sell(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgState *st) {
+    if (ctx->sell_b == 3) launch_sell_b<3>(ctx, mode, x, y, st);
+    else if (ctx->sell_b == 2) launch_sell_b<2>(ctx, mode, x, y, st);
+    else launch_sell_b<1>(ctx, mode, x, y, st);
 }
 
 // Algorithmic bytes of one SpMV (values + indices + row pointers + x + y);
@@ -468,7 +518,8 @@ double spmv_bytes(bhip_ctx *ctx, bool fp32, bool bsr) {
 int ensure_partials(bhip_ctx *ctx) {
     size_t need = (size_t)kRedBlocks * 3;
     need = std::max(need, (size_t)csr_grid(ctx) * 2);
-    if (ctx->n_slices > 0) need = std::max(need, (size_t)sell_grid(ctx));
+    // SELL: the widest workgroup (16 slices) writes up to 3 unused group records
+    if (ctx->n_slices > 0) need = std::max(need, (size_t)sell_groups(ctx) + 4);
     if (ctx->partials.n < need) {
         bhip_invalidate_graphs(ctx);
         BHIP_TRY(ctx, ctx->partials.resize(need));
@@ -488,6 +539,13 @@ int prepare_diag(bhip_ctx *ctx) {
     if (hf & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "negative diagonal term in the global matrix");
     if (hf & 1) return bhip_fail(ctx, BHIP_ERR_ZERO_DIAG, "zero diagonal term in the global matrix");
     return BHIP_OK;
+}
+
+// Default shape of a new SELL layout from the tuned table.
+int tuned_default_shape(int b, long long n_slices) {
+    for (const TunedSellRow &row : kTunedSell)
+        if (row.b == b && n_slices <= row.max_slices) return row.shape;
+    return 0;
 }
 
 int build_sell(bhip_ctx *ctx) {
@@ -519,6 +577,7 @@ int build_sell(bhip_ctx *ctx) {
     k_sell_pad<<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
     BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
     BHIP_TRY(ctx, hipGetLastError());
+    if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
     return BHIP_OK;
 }
 
@@ -573,7 +632,7 @@ void enqueue_iteration(bhip_ctx *ctx, bool fp32) {
         int gs;
         if (ctx->use_bsr) {
             launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
-            gs = sell_grid(ctx);
+            gs = sell_groups(ctx);
         } else {
             launch_csr<float, float, 1>(ctx, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
             gs = csr_grid(ctx);
@@ -821,4 +880,94 @@ int bhip_bench_spmv_impl(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, doub
     *ms_avg = ms / reps;
     *bytes = spmv_bytes(ctx, fp32 != 0, bsr);
     return BHIP_OK;
+}
+
+int bhip_spmv_shape_info(int id, bhip_spmv_shape *info) {
+    switch (id) {
+#define BHIP_SHAPE_INFO(ID, SPB, UNR, NT) \
+    case ID:                              \
+        info->id = ID;                    \
+        info->slices_per_block = SPB;     \
+        info->unroll = UNR;               \
+        info->nontemporal = NT;           \
+        return BHIP_OK;
+        BHIP_SELL_SHAPES(BHIP_SHAPE_INFO)
+#undef BHIP_SHAPE_INFO
+    }
+    return BHIP_ERR_ARG;
+}
+
+// Times every SELL tile shape on the resident matrix.  Each shape gets one
+// warm-up launch, then three rounds over all shapes of `reps` launches each
+// (interleaved, so that clock ramps and neighbours affect all shapes alike);
+// a shape's time is its best round.  The fastest shape is kept for later
+// solves.  Shapes do not change results (see k_sell_spmv), so tuning never
+// changes a solve's bits.
+int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max_entries, int *n_entries,
+                        int *chosen) {
+    if (!ctx->assembled) return bhip_fail(ctx, BHIP_ERR_STATE, "matrix not assembled");
+    if (reps <= 0) reps = 10;
+    *n_entries = 0;
+    *chosen = -1;
+    BHIP_PROPAGATE(prepare_diag(ctx));
+    BHIP_PROPAGATE(prepare_f32(ctx));
+    BHIP_PROPAGATE(ensure_partials(ctx));
+    if (!ctx->use_bsr) return BHIP_OK;  // FP32 CSR: no shapes
+    hipStream_t s = ctx->stream;
+    PcgState *st = ctx->state.p;
+    const long long n = ctx->n_dof;
+    k_to_f32<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->dinv.p, ctx->p32.p);
+    k_state_init<<<1, 1, 0, s>>>(st, 1.0, 1, 0);
+    const int ns = kNumSellShapes;
+    std::vector<double> best(ns, 1e300);
+    const int keep = ctx->sell_shape;
+    for (int id = 0; id < ns; id++) {
+        ctx->sell_shape = id;
+        launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
+    }
+    for (int round = 0; round < 3; round++)
+        for (int id = 0; id < ns; id++) {
+            ctx->sell_shape = id;
+            BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+            for (int k = 0; k < reps; k++) launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
+            BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+            BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) {
+                ctx->sell_shape = keep;
+                BHIP_TRY(ctx, e);
+            }
+            float ms = 0;
+            BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            best[id] = std::min(best[id], (double)ms / reps);
+        }
+    int pick = 0;
+    for (int id = 1; id < ns; id++)
+        if (best[id] < best[pick]) pick = id;
+    const double by = spmv_bytes(ctx, true, true);
+    for (int id = 0; id < ns && id < max_entries; id++) {
+        bhip_spmv_shape_info(id, &table[id]);
+        table[id].ms = best[id];
+        table[id].gbps = by / best[id] / 1e6;
+    }
+    *n_entries = std::min(ns, std::max(max_entries, 0));
+    *chosen = pick;
+    if (pick != keep) bhip_invalidate_graphs(ctx);
+    ctx->sell_shape = pick;
+    ctx->sell_shape_set = true;
+    return BHIP_OK;
+}
+
+int bhip_set_spmv_shape_impl(bhip_ctx *ctx, int id) {
+    bhip_spmv_shape info;
+    if (bhip_spmv_shape_info(id, &info) != BHIP_OK) return bhip_fail(ctx, BHIP_ERR_ARG, "unknown SpMV shape id");
+    if (id != ctx->sell_shape) bhip_invalidate_graphs(ctx);
+    ctx->sell_shape = id;
+    ctx->sell_shape_set = true;
+    return BHIP_OK;
+}
+
+bool bhip_sell_streams(const bhip_ctx *ctx) {
+    bhip_spmv_shape info;
+    return bhip_spmv_shape_info(ctx->sell_shape, &info) == BHIP_OK && info.nontemporal != 0;
 }

@@ -15,7 +15,11 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
   10 backward-Euler steps                 (unless --no-solve)
 
+The SELL SpMV runs in the shape the library's tuned table gives the matrix,
+or in --shape ID (see bhip_tune_spmv); the shape is printed.
+
 usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat] [--n N] [--reps 10] [--no-solve]
+                                              [--shape ID]
 """
 import argparse
 import json
@@ -32,6 +36,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--shape", type=int, default=-1)
     a = ap.parse_args(argv)
     out = {"workload": a.workload}
     with capi.Context(0) as ctx:
@@ -44,8 +49,10 @@ def main(argv=None):
             ctx.build_rhs()
             ms, gbps = ctx.bench_stream(1 << 30, a.reps)
             out["stream_copy"] = {"bytes_read_plus_written": 2 * (1 << 30), "ms": ms, "GBps": gbps}
+            if a.shape >= 0:
+                ctx.set_spmv_shape(a.shape)
             ms, by = ctx.bench_spmv(fp32=True, reps=a.reps)
-            out["sell3_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
+            out["sell3_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6, "shape": ctx.spmv_shape()}
             ms, by = ctx.bench_spmv(fp32=False, reps=a.reps)
             out["csr_fp64"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
             if not a.no_solve:
@@ -56,6 +63,8 @@ def main(argv=None):
             prob = meshgen.rve_request(n, 7)
             setup_problem(ctx, prob)
             ctx.assemble()
+            if a.shape >= 0:
+                ctx.set_spmv_shape(a.shape)
             P = np.stack(prob.extra["cases"])
             o = capi.Context.opts(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
             X, s = ctx.solve_cases(P, opts=o)
@@ -66,8 +75,10 @@ def main(argv=None):
             setup_problem(ctx, prob)
             u0 = prob.extra["u0"]
             ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[1.0, prob.extra["dt"]], u_state=u0)
+            if a.shape >= 0:
+                ctx.set_spmv_shape(a.shape)
             ms, by = ctx.bench_spmv(fp32=True, reps=a.reps)
-            out["sell1_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
+            out["sell1_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6, "shape": ctx.spmv_shape()}
             ms, by = ctx.bench_spmv(fp32=False, reps=a.reps)
             out["csr_fp64"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
             if not a.no_solve:

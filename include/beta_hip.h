@@ -181,6 +181,31 @@ int bhip_solve_cases(bhip_ctx *ctx, int nrhs, const double *presc, const double 
 /* Time `reps` FP64 (fp32 = 0) or FP32 (fp32 = 1) SpMVs on resident vectors;
  * returns the mean kernel time in ms (HIP events). */
 int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes);
+/* Tile shapes of the FP32 blocked (SELL-64) SpMV of the inner solve.  No
+ * counterpart in the reference (its SpMV is the serial vectorProduct,
+ * sparse.cpp:452-478).  A shape changes speed only: every shape sums each row
+ * and the p.Ap partials in the same order, so solves give the same bits
+ * whichever shape runs. */
+typedef struct bhip_spmv_shape {
+    int    id;               /* shape id (0: the default shape) */
+    int    slices_per_block; /* 64-node slices (one wave each) per workgroup: 4, 8 or 16 */
+    int    unroll;           /* block columns per loop trip */
+    int    nontemporal;      /* 1: streaming loads of block values and columns */
+    double ms;               /* best mean kernel time of the tuning rounds (HIP events) */
+    double gbps;             /* algorithmic bytes / ms */
+} bhip_spmv_shape;
+/* Autotuner: times every shape on the context's assembled matrix (three
+ * interleaved rounds of `reps` launches per shape), keeps the fastest for the
+ * context's later solves and reports the table (up to max_entries rows).
+ * n_entries = 0 and chosen = -1 when the matrix has no blocked FP32 form
+ * (DOFs per node not uniform or above 3). */
+int bhip_tune_spmv(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max_entries, int *n_entries,
+                   int *chosen);
+/* Select a shape by id (a stored tuning result); BHIP_ERR_ARG for an unknown id. */
+int bhip_set_spmv_shape(bhip_ctx *ctx, int id);
+int bhip_get_spmv_shape(const bhip_ctx *ctx);
+/* Shape parameters of an id (ms and gbps zero); BHIP_ERR_ARG past the last id. */
+int bhip_spmv_shape_describe(int id, bhip_spmv_shape *info);
 /* Diagnostic without a reference counterpart: a plain device copy of `bytes`
  * bytes (read + write), timed over `reps` launches; reports the achieved
  * read+write GB/s (STREAM-copy convention) of this GPU. */

@@ -48,3 +48,14 @@ def test_create_without_gpu_reports_status():
     st = lib.bhip_create(0, C.byref(h))
     assert st in (capi.ERR_HIP, capi.ERR_ARG)
     assert not h.value
+
+
+def test_spmv_shape_table():
+    """The tile shapes the tuner chooses from (no GPU needed)."""
+    from betanoox_amd import capi
+    shapes = capi.spmv_shapes()
+    assert len(shapes) == 18
+    assert [s["id"] for s in shapes] == list(range(18))
+    assert shapes[0]["slices_per_block"] == 4 and shapes[0]["unroll"] == 4 and shapes[0]["nontemporal"] == 0
+    triples = {(s["slices_per_block"], s["unroll"], s["nontemporal"]) for s in shapes}
+    assert triples == {(a, b, c) for a in (4, 8, 16) for b in (2, 4, 8) for c in (0, 1)}

@@ -1,0 +1,88 @@
+"""SpMV tile-shape autotuner (bhip_tune_spmv) through the C ABI.
+
+A tile shape changes the speed of the FP32 blocked SpMV, never its results:
+every shape sums each node row and the p.Ap partials in the same order, so a
+solve gives the same bits whichever shape runs (checked bit for bit here on
+meshes whose slice count is not a multiple of the workgroup sizes, for 1, 2
+and 3 DOFs per node).  The tuner reports every shape, keeps the fastest, and a
+solve after tuning equals the solve before it bit for bit.
+"""
+import numpy as np
+import pytest
+
+from betanoox_amd import capi, meshgen, setup_problem
+
+pytestmark = pytest.mark.gpu
+
+N_SHAPES = 18
+
+
+def _problem(kind):
+    if kind == "hex11":        # 1728 nodes: 27 slices
+        return meshgen.elastic_cube(11), capi.ASM_STIFFNESS, None
+    if kind == "quad40":       # 1681 nodes, 2 DOFs per node: 27 slices
+        return meshgen.plane_stress_quad(40), capi.ASM_STIFFNESS, None
+    if kind == "heat13":       # 2744 nodes, 1 DOF per node: 43 slices
+        p = meshgen.diffusion_cube(13)
+        return p, capi.ASM_HEAT_TRANSIENT, p
+    raise KeyError(kind)
+
+
+def _assembled(factory, kind):
+    prob, what, heat = _problem(kind)
+    print("case", kind, prob.n_dof, flush=True)
+    ctx = setup_problem(factory(), prob)
+    if heat is not None:
+        ctx.assemble(what, coef=[1.0, heat.extra["dt"]], u_state=heat.extra["u0"])
+        rng = np.random.default_rng(7)
+        b = rng.standard_normal(prob.n_dof)
+        ctx.set_rhs(b)
+    else:
+        ctx.assemble(what)
+        ctx.set_load(prob.load)
+        ctx.build_rhs()
+    return prob, ctx
+
+
+def _solve(ctx):
+    st = ctx.solve(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=20000)
+    return st, ctx.solution()
+
+
+@pytest.mark.parametrize("kind", ["hex11", "quad40", "heat13"])
+def test_every_shape_same_bits(hip_ctx_factory, kind):
+    prob, ctx = _assembled(hip_ctx_factory, kind)
+    assert ctx.spmv_shape() == 0
+    st0, x0 = _solve(ctx)
+    assert st0["rel_resid"] <= 1e-8
+    for sid in range(1, N_SHAPES):
+        print("shape", sid, flush=True)
+        ctx.set_spmv_shape(sid)
+        assert ctx.spmv_shape() == sid
+        st, x = _solve(ctx)
+        assert st["iters"] == st0["iters"] and st["outer_iters"] == st0["outer_iters"], sid
+        assert np.array_equal(x.view(np.uint64), x0.view(np.uint64)), sid
+
+
+def test_tune_keeps_fastest_and_bits(hip_ctx_factory):
+    prob, ctx = _assembled(hip_ctx_factory, "hex11")
+    st0, x0 = _solve(ctx)
+    chosen, table = ctx.tune_spmv(reps=5)
+    assert len(table) == N_SHAPES
+    assert [t["id"] for t in table] == list(range(N_SHAPES))
+    ms = np.array([t["ms"] for t in table])
+    assert np.all(ms > 0) and np.all(np.isfinite(ms))
+    assert chosen == int(np.argmin(ms))
+    assert ctx.spmv_shape() == chosen
+    st, x = _solve(ctx)
+    assert st["iters"] == st0["iters"]
+    assert np.array_equal(x.view(np.uint64), x0.view(np.uint64))
+
+
+def test_shape_argument_errors(hip_ctx_factory):
+    prob, ctx = _assembled(hip_ctx_factory, "hex11")
+    with pytest.raises(capi.BhipError):
+        ctx.set_spmv_shape(N_SHAPES)
+    with pytest.raises(capi.BhipError):
+        ctx.set_spmv_shape(-1)
+    assert ctx.spmv_shape() == 0
