@@ -68,6 +68,7 @@ struct PcgState {
     int converged;
     unsigned cnt[4];  // last-block counters (reset by the last block)
     int pad;
+    double rzb[2];    // r.z double-buffered for the fused iteration (read [par], written [par ^ 1])
 };
 
 constexpr int kMaxNnpe = 8;

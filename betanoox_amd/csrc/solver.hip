@@ -8,8 +8,16 @@
 //   3. x += alpha p, r -= alpha q, z = r/diag, block partials of r.z, r.r, x.x
 //   4. finalize (one block): beta, stopping rule, iteration count
 //   5. p = z + beta p
-// Reductions are deterministic: blocks store partials, a single-block kernel
-// sums them in block order.  (A fence-and-counter "last block reduces"
+// On the FP32 blocked (SELL) inner solve the two finalize kernels are folded
+// into the vector kernels, three kernels per iteration:
+//   1. q = K p, partials of p.q (one per group of four slices)
+//   2. every block sums the p.q partials itself, alpha; x, r update, z,
+//      partials of r.z, r.r, x.x
+//   3. every block sums those partials itself, beta, stopping rule (block 0
+//      writes the state); p = z + beta p
+// Reductions are deterministic: blocks store partials, and the partials are
+// summed in block order (by a single-block kernel, or by every block of the
+// consuming kernel, which then all obtain the same bits).  (A fence-and-counter "last block reduces"
 // scheme costs an L2 write-back per block on gfx950 and halves the SpMV
 // bandwidth.)  Kernels read the solver scalars from device memory and return
 // at once when the solve is done, so iterations are enqueued in chunks
@@ -52,6 +60,39 @@ __device__ __forceinline__ void fin_sum(const double *partials, int nb, double (
 #pragma unroll
         for (int k = 0; k < NV; k++) tot[k] += partials[(long long)b * NV + k];
     block_sum<NV>(tot, sm);
+}
+
+// Fixed-order sum of nb partial records by one block, result in every thread.
+// Thread t adds records t, t + blockDim.x, ... in that order; the loads of
+// eight records are issued before their sums so that they overlap (the
+// records come from L2 and a thread may own dozens of them).
+template <int NV>
+__device__ __forceinline__ void sum_records_all(const double *partials, int nb, double (&tot)[NV]) {
+    __shared__ double sm[16 * 4];
+    __shared__ double res[NV];
+    constexpr int U = 8;
+#pragma unroll
+    for (int k = 0; k < NV; k++) tot[k] = 0;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += U * blockDim.x) {
+        double v[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int b = b0 + u * blockDim.x;
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[u][k] = b < nb ? partials[(long long)b * NV + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < NV; k++) tot[k] += v[u][k];
+    }
+    block_sum<NV>(tot, sm);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) res[k] = tot[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) tot[k] = res[k];
 }
 
 // Contiguous range of logical block lb out of nb over n items.
@@ -228,6 +269,91 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate(long long n, T *__rest
     for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = r[i] * dinv[i] + beta * p[i];
 }
 
+// Fused form of k_fin_alpha + k_pcg_update: every block sums the nb_in p.q
+// partial records of the SpMV (the same records in the same order in every
+// block), alpha = rz / pAp; then x += alpha p, r -= alpha q, z = r*dinv and
+// partials r.z, r.r, x.x into pout (disjoint from pin).
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_pcg_update_fa(long long n, int par, int nb_in,
+                                                          const double *__restrict__ pin, T *__restrict__ x,
+                                                          T *__restrict__ r, const T *__restrict__ p,
+                                                          const T *__restrict__ q, const T *__restrict__ dinv,
+                                                          PcgState *st, double *__restrict__ pout) {
+    if (st->done) return;
+    const double rz = st->rzb[par];
+    double pap[1];
+    sum_records_all<1>(pin, nb_in, pap);
+    const double alpha_d = rz / pap[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pAp = pap[0];
+        st->alpha = alpha_d;
+    }
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    long long i0, i1;
+    block_range(n, lb, gridDim.x, i0, i1);
+    const T alpha = (T)alpha_d;
+    double rzn = 0, rr = 0, xx = 0;
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        T xi = x[i] + alpha * p[i];
+        T ri = r[i] - alpha * q[i];
+        T zi = ri * dinv[i];
+        x[i] = xi;
+        r[i] = ri;
+        rzn += (double)ri * (double)zi;
+        rr += (double)ri * (double)ri;
+        xx += (double)xi * (double)xi;
+    }
+    double v[3] = {rzn, rr, xx};
+    store_partials<3>(v, pout);
+}
+
+// Fused form of k_fin_update + k_pcg_pupdate: every block sums the nb_in
+// (r.z, r.r, x.x) records, beta = rz_new / rz; block 0 applies the stopping
+// rule and writes the state (rz_new into rzb[par ^ 1], which no block of this
+// kernel reads); then p = r*dinv + beta p.  A block that starts after block 0
+// marked the solve done skips its p update, which is no longer used.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_pcg_pupdate_fb(long long n, int par, int nb_in,
+                                                           const double *__restrict__ pin, T *__restrict__ p,
+                                                           const T *__restrict__ r, const T *__restrict__ dinv,
+                                                           PcgState *st) {
+    if (st->done) return;
+    const double rz = st->rzb[par];
+    double t[3];
+    sum_records_all<3>(pin, nb_in, t);
+    const double beta_d = t[0] / rz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double tol = st->tol, bb = st->bb, rr0 = st->rr0;
+        const int stop_rule = st->stop_rule, iter = st->iter + 1, maxit = st->maxit;
+        st->beta = beta_d;
+        st->rz = t[0];
+        st->rzb[par ^ 1] = t[0];
+        st->rr = t[1];
+        st->xx = t[2];
+        st->iter = iter;
+        double nrm;
+        bool conv;
+        if (stop_rule == BHIP_STOP_REF_R_OVER_X) {
+            nrm = sqrt(t[1] / t[2]);
+            conv = nrm < tol;
+        } else if (stop_rule == BHIP_STOP_REL_RESID) {
+            nrm = sqrt(t[1] / bb);
+            conv = nrm <= tol;
+        } else {  // inner solve: relative reduction of |r|
+            nrm = sqrt(t[1] / rr0);
+            conv = nrm <= tol;
+        }
+        st->norm = nrm;
+        if (conv) st->converged = 1;
+        if (conv || iter >= maxit) st->done = 1;
+    }
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    long long i0, i1;
+    block_range(n, lb, gridDim.x, i0, i1);
+    const T beta = (T)beta_d;
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = r[i] * dinv[i] + beta * p[i];
+}
+
 // Initial: z = r*dinv, p = z; partials r.z, r.r, x.x.
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_pcg_init(long long n, const T *__restrict__ x,
@@ -299,6 +425,7 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_init(int nb, const double *
     fin_sum<3>(partials, nb, t);
     if (threadIdx.x == 0) {
         st->rz = t[0];
+        st->rzb[0] = t[0];
         st->rr = t[1];
         st->rr0 = t[1];
         st->xx = t[2];
@@ -513,13 +640,18 @@ double spmv_bytes(bhip_ctx *ctx, bool fp32, bool bsr) {
     return (double)ctx->nnz * (sv + 4) + (ctx->n_dof + 1.0) * 4 + 2.0 * ctx->n_dof * sv;
 }
 
+// Offset of the vector-kernel records of the fused iteration, after the
+// SpMV's group records (the widest workgroup, 16 slices, writes up to 3 unused
+// group records past sell_groups).
+size_t fused_offset(bhip_ctx *ctx) { return ((size_t)sell_groups(ctx) + 4 + 63) / 64 * 64; }
+
 // Partials large enough for every kernel of the solve (3 per block for the
-// vector kernels, 2 per block for the CSR residual).
+// vector kernels, 2 per block for the CSR residual, the SpMV's group records
+// plus 3 per vector block for the fused iteration).
 int ensure_partials(bhip_ctx *ctx) {
     size_t need = (size_t)kRedBlocks * 3;
     need = std::max(need, (size_t)csr_grid(ctx) * 2);
-    // SELL: the widest workgroup (16 slices) writes up to 3 unused group records
-    if (ctx->n_slices > 0) need = std::max(need, (size_t)sell_groups(ctx) + 4);
+    if (ctx->n_slices > 0) need = std::max(need, fused_offset(ctx) + (size_t)kRedBlocks * 3);
     if (ctx->partials.n < need) {
         bhip_invalidate_graphs(ctx);
         BHIP_TRY(ctx, ctx->partials.resize(need));
@@ -615,8 +747,9 @@ int prepare_f32(bhip_ctx *ctx) {
     return BHIP_OK;
 }
 
-// Enqueue one PCG iteration (five kernels).
-void enqueue_iteration(bhip_ctx *ctx, bool fp32) {
+// Enqueue one PCG iteration: three kernels on the FP32 blocked form (par =
+// parity of the iteration, which selects the r.z buffer), five otherwise.
+void enqueue_iteration(bhip_ctx *ctx, bool fp32, int par) {
     hipStream_t s = ctx->stream;
     PcgState *st = ctx->state.p;
     const long long n = ctx->n_dof;
@@ -629,14 +762,18 @@ void enqueue_iteration(bhip_ctx *ctx, bool fp32) {
         k_fin_update<<<1, kFinThreads, 0, s>>>(gv, ctx->partials.p, st);
         k_pcg_pupdate<double><<<gv, kThreads, 0, s>>>(n, ctx->p.p, ctx->r.p, ctx->dinv.p, st);
     } else {
-        int gs;
         if (ctx->use_bsr) {
+            double *p2 = ctx->partials.p + fused_offset(ctx);
             launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
-            gs = sell_groups(ctx);
-        } else {
-            launch_csr<float, float, 1>(ctx, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
-            gs = csr_grid(ctx);
+            k_pcg_update_fa<float><<<gv, kThreads, 0, s>>>(n, par, sell_groups(ctx), ctx->partials.p, ctx->d32.p,
+                                                           ctx->r32.p, ctx->p32.p, ctx->q32.p, ctx->dinv32.p, st,
+                                                           p2);
+            k_pcg_pupdate_fb<float><<<gv, kThreads, 0, s>>>(n, par, gv, p2, ctx->p32.p, ctx->r32.p, ctx->dinv32.p,
+                                                            st);
+            return;
         }
+        launch_csr<float, float, 1>(ctx, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
+        const int gs = csr_grid(ctx);
         k_fin_alpha<<<1, kFinThreads, 0, s>>>(gs, ctx->partials.p, st);
         k_pcg_update<float><<<gv, kThreads, 0, s>>>(n, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->q32.p,
                                                     ctx->dinv32.p, st, ctx->partials.p);
@@ -664,7 +801,7 @@ int get_graph(bhip_ctx *ctx, bool fp32, int chunk, hipGraphExec_t *out) {
     }
     hipGraph_t graph;
     BHIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < chunk; k++) enqueue_iteration(ctx, fp32);
+    for (int k = 0; k < chunk; k++) enqueue_iteration(ctx, fp32, k & 1);
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     BHIP_TRY(ctx, e);
     e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
@@ -694,7 +831,7 @@ int run_iterations(bhip_ctx *ctx, bool fp32, int check_every, int *spmv_launches
         if (use_graph) {
             BHIP_TRY(ctx, hipGraphLaunch(g, s));
         } else {
-            for (int k = 0; k < check_every; k++) enqueue_iteration(ctx, fp32);
+            for (int k = 0; k < check_every; k++) enqueue_iteration(ctx, fp32, k & 1);
         }
         BHIP_TRY(ctx, hipGetLastError());
         if (spmv_launches) *spmv_launches += check_every;
@@ -759,6 +896,8 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
     if (!ctx->assembled) return bhip_fail(ctx, BHIP_ERR_STATE, "matrix not assembled");
     bhip_solve_opts opts = *o;
     if (opts.check_every <= 0) opts.check_every = 32;
+    // iterations alternate the r.z buffer, so every chunk holds an even count
+    if (opts.check_every & 1) opts.check_every++;
     if (opts.maxit <= 0) opts.maxit = 10000;
     if (!(opts.tol > 0)) return bhip_fail(ctx, BHIP_ERR_ARG, "tol must be positive");
     hipStream_t s = ctx->stream;
