@@ -83,7 +83,7 @@ def cpu_baseline(sample_n, budget_s):
     }
 
 
-def served_stream(count, slots, rate_frac, probe_count, seed=None):
+def served_stream(count, slots, rate_frac, probe_count, seed=None, max_pack=0):
     """Workload 5 on a bounded prefix of the seeded 4,096-request stream: RVE
     requests (n = round(exp(U(ln 16, ln 96))), 5-40 inclusions, six affine
     load cases each), solved by the resident server to 1e-8.  A closed-loop
@@ -105,7 +105,7 @@ def served_stream(count, slots, rate_frac, probe_count, seed=None):
     out = {"requests": count, "of_stream": total, "slots": slots, "cases_per_request": 6, "tol": 1e-8,
            "n_min": int(ns[:count].min()), "n_max": int(ns[:count].max()),
            "dof_total": int(sum(r.n_dof for r in reqs))}
-    with capi.Server(0, max_slots=slots, tol=1e-8) as srv:
+    with capi.Server(0, max_slots=slots, tol=1e-8, max_pack=max_pack) as srv:
         t0 = time.perf_counter()
         ids = [srv.submit(r) for r in probe]
         pst = [srv.wait(i)[1] for i in ids]
@@ -143,6 +143,11 @@ def served_stream(count, slots, rate_frac, probe_count, seed=None):
         "dof_iter_per_s": dof_iter / span, "wall_s": span,
         "worst_rel_resid": max(r["rel_resid"] for r in res),
         "pattern_reuse": sum(r["reused_pattern"] for r in res),
+        "max_pack": max_pack if max_pack > 0 else 8,
+        "packed_requests": sum(1 for r in res if r["pack_size"] > 1),
+        "mean_pack_size": float(np.mean([r["pack_size"] for r in res])),
+        "setup_p50_ms": float(np.percentile([r["setup_ms"] for r in res], 50)),
+        "solve_p50_ms": float(np.percentile([r["solve_ms"] for r in res], 50)),
     })
     return out
 
@@ -226,6 +231,7 @@ def main():
     ap.add_argument("--served-slots", type=int, default=4)
     ap.add_argument("--served-load", type=float, default=0.7, help="arrival rate / probed capacity")
     ap.add_argument("--served-probe", type=int, default=32)
+    ap.add_argument("--served-pack", type=int, default=0, help="requests per packed solve (0: library default 8)")
     ap.add_argument("--heat-n", type=int, default=128, help="diffusion cube size (0: skip)")
     ap.add_argument("--heat-steps", type=int, default=100)
     ap.add_argument("--heat-fp64-steps", type=int, default=10)
@@ -343,7 +349,8 @@ def main():
         line["plastic"] = plastic_leg(args.plastic_n, args.plastic_tol, 300)
         log("plastic: %s" % json.dumps(line["plastic"]))
     if args.served > 0:
-        line["served"] = served_stream(args.served, args.served_slots, args.served_load, args.served_probe)
+        line["served"] = served_stream(args.served, args.served_slots, args.served_load, args.served_probe,
+                                       max_pack=args.served_pack)
         log("served: %s" % json.dumps(line["served"]))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_budget)

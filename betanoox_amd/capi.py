@@ -69,6 +69,7 @@ EXPORTS = [
     "bhip_server_budget", "bhip_request_footprint", "bhip_heat_run", "bhip_write_matrix", "bhip_write_csr_file",
     "bhip_set_csr", "bhip_read_matrix", "bhip_write_vector", "bhip_read_vector",
     "bhip_tune_spmv", "bhip_set_spmv_shape", "bhip_get_spmv_shape", "bhip_spmv_shape_describe",
+    "bhip_pack_create", "bhip_pack_destroy", "bhip_pack_solve_cases",
 ]
 
 
@@ -89,7 +90,7 @@ class Stats(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("max_slots", C.c_int), ("hbm_budget", C.c_double), ("tol", C.c_double),
-                ("inner_tol", C.c_double), ("maxit", C.c_int), ("precision", C.c_int)]
+                ("inner_tol", C.c_double), ("maxit", C.c_int), ("precision", C.c_int), ("max_pack", C.c_int)]
 
 
 class Request(C.Structure):
@@ -104,7 +105,8 @@ class RequestStats(C.Structure):
     _fields_ = [("status", C.c_int), ("slot", C.c_int), ("reused_pattern", C.c_int), ("iters", C.c_int),
                 ("outer_iters", C.c_int), ("rel_resid", C.c_double), ("queue_ms", C.c_double),
                 ("service_ms", C.c_double), ("latency_ms", C.c_double), ("start_ms", C.c_double),
-                ("end_ms", C.c_double), ("message", C.c_char * 256)]
+                ("end_ms", C.c_double), ("pack_size", C.c_int), ("setup_ms", C.c_double),
+                ("solve_ms", C.c_double), ("message", C.c_char * 256)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "message"}
@@ -194,6 +196,10 @@ def load_library(path=LIB_PATH):
         "bhip_set_spmv_shape": (C.c_int, [vp, C.c_int]),
         "bhip_get_spmv_shape": (C.c_int, [vp]),
         "bhip_spmv_shape_describe": (C.c_int, [C.c_int, C.POINTER(SpmvShape)]),
+        "bhip_pack_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "bhip_pack_destroy": (None, [vp]),
+        "bhip_pack_solve_cases": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(_P_D), C.POINTER(_P_D),
+                                            C.POINTER(_P_D), C.POINTER(SolveOpts), C.POINTER(Stats), _P_I]),
         "bhip_plastic_init": (C.c_int, [vp]),
         "bhip_plastic_update": (C.c_int, [vp, _P_D]),
         "bhip_plastic_residual": (C.c_int, [vp, _P_D, _P_D]),
@@ -505,12 +511,13 @@ class Context:
 class Server:
     """Resident request loop (bhip_server_*): hex8 elasticity requests with
     several prescribed-displacement load cases, solved by up to `max_slots`
-    contexts at once on one GPU."""
+    workers at once on one GPU; each worker packs up to `max_pack` small
+    requests into one batched solve."""
 
     def __init__(self, device=0, max_slots=4, hbm_budget=0.0, tol=1e-8, inner_tol=1e-3, maxit=40000,
-                 precision=PREC_FP32_IR):
+                 precision=PREC_FP32_IR, max_pack=0):
         self.lib = load_library()
-        o = ServerOpts(max_slots, hbm_budget, tol, inner_tol, maxit, precision)
+        o = ServerOpts(max_slots, hbm_budget, tol, inner_tol, maxit, precision, max_pack)
         h = C.c_void_p()
         st = self.lib.bhip_server_create(device, C.byref(o), C.byref(h))
         if st != OK:
@@ -577,6 +584,57 @@ class Server:
         if st != OK:
             raise BhipError(st, d["message"])
         return X, d
+
+
+class Pack:
+    """Packed solver (bhip_pack_*): the load cases of several assembled
+    Contexts solved together in one batched FP32-inner PCG."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.bhip_pack_create(device, C.byref(h))
+        if st != OK:
+            raise BhipError(st, "bhip_pack_create failed (is a GPU visible?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.bhip_pack_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def solve_cases(self, ctxs, prescs, loads=None, opts=None, **kw):
+        """ctxs: list of Context; prescs: list of (nrhs, n_dof) arrays (same
+        nrhs).  Returns (list of X, list of stats dicts, list of status)."""
+        o = opts if opts is not None else Context.opts(**kw)
+        n = len(ctxs)
+        P = [np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64) for a in prescs]
+        nrhs = P[0].shape[0]
+        X = [np.zeros_like(a) for a in P]
+        L = None
+        if loads is not None:
+            L = [None if l is None else _f64(l) for l in loads]
+        handles = (C.c_void_p * n)(*[c.h for c in ctxs])
+        pp = (_P_D * n)(*[_d(a) for a in P])
+        xp = (_P_D * n)(*[_d(a) for a in X])
+        lp = None if L is None else (_P_D * n)(*[_d(l) for l in L])
+        stats = (Stats * n)()
+        status = np.zeros(n, dtype=np.int32)
+        self.lib.bhip_pack_solve_cases(self.h, n, handles, nrhs, pp, lp, xp, C.byref(o), stats,
+                                       status.ctypes.data_as(_P_I))
+        return X, [s.as_dict() for s in stats], [int(v) for v in status]
 
 
 def spmv_shapes():

@@ -238,6 +238,36 @@ void bhip_invalidate_graphs(bhip_ctx *ctx);
 void bhip_plastic_release(bhip_ctx *ctx);
 void bhip_multi_invalidate(bhip_ctx *ctx);
 void bhip_multi_release(bhip_ctx *ctx);
+
+// ---- multi-case solve (multi.hip) and the packed solver (pack.hip)
+struct PackEngine;
+// Buffers of the multi-case solve, one set per context.
+struct MultiData {
+    int R = 0;        // cases the buffers hold
+    long long n = 0;  // DOFs per case
+    dvec<double> b64, x64, r64, scale, rn;  // case-major, R x n (scale, rn: R)
+    dvec<double> save;                     // RHS and solution of the context (case-by-case path)
+    dvec<float> r32, p32, q32, d32;        // FP32 inner vectors, case-major, R x n
+    dvec<double> partials;                 // FP64 residual partials
+    dvec<double> resid;                    // 2R: |b_r - K x_r|^2, |b_r|^2
+    double *h_resid = nullptr;             // pinned mirror of resid
+    PackEngine *pack = nullptr;            // packed solver of this context alone (its stream)
+    void release_buffers();
+};
+int bhip_multi_prepare(bhip_ctx *ctx, int R, bool fused);
+void bhip_multi_enqueue_resid(bhip_ctx *ctx, int R, hipStream_t s, double *out);
+void bhip_multi_enqueue_pack32(bhip_ctx *ctx, int R, hipStream_t s);
+void bhip_multi_enqueue_axpy(bhip_ctx *ctx, int R, hipStream_t s);
+int bhip_multi_worst_resid(bhip_ctx *ctx, int R, double *worst);
+int bhip_cases_begin(bhip_ctx *ctx, int nrhs, const double *presc, const double *load, const bhip_solve_opts *opts,
+                     bool *fused);
+int bhip_cases_finish(bhip_ctx *ctx, int nrhs, double *x, bhip_stats *S);
+// stream = nullptr: the engine creates its own (non-blocking) stream
+int bhip_pack_engine_create(int device, hipStream_t stream, PackEngine **out);
+void bhip_pack_engine_destroy(PackEngine *pe);
+hipStream_t bhip_pack_engine_stream(PackEngine *pe);
+int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n, int R, const bhip_solve_opts &o, bhip_stats *stats,
+                  int *status);
 // Gauss tables of (nnpe, order) in ctx->gauss (uploaded when they change):
 // w = gauss, dN = gauss + kGaussDN, N = gauss + kGaussN.
 constexpr int kGaussDN = kMaxIp;

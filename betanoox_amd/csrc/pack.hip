@@ -1,0 +1,800 @@
+// pack.hip -- packed batched inner PCG: the load cases of several assembled
+// systems solved together by one persistent set of kernels.
+//
+// Reference workflow: the column loop of MultiFieldMacroElement::CalculateK /
+// solveForStiffnessMatrixColumn (MultiFieldMacroElement.cpp:135-407), where
+// every load case of a macro element is solved with the same assembled K
+// (the prescribed values re-lifted through K, Equations::
+// updatePseudoLoadVectorUsingGK, equation.cpp:343-381).  The reference runs one
+// model at a time; here a pack holds several models (systems), each with R
+// load cases, and their FP32 inner Jacobi-PCGs advance together.
+//
+// Design
+//  - A system is an assembled context with 3 DOFs per node and its FP32
+//    SELL-64 3x3-block matrix; its vectors are case-major (case r, DOF i at
+//    r * n + i).  Every case keeps its own CG recurrence and stopping test.
+//  - Work items: an SpMV item is one 64-node slice of a small system (four
+//    waves share the slice) or four slices of a large one (one wave each); a
+//    vector item is one DOF range of one case of one system.  Item tables map
+//    items to systems.
+//  - Persistent kernels: a fixed grid walks the items (grid stride).  The
+//    kernels take only the address of a device-side pack descriptor and the
+//    iteration parity, so one captured HIP graph of a chunk of iterations
+//    serves every pack composition; the host rewrites the descriptor between
+//    inner solves.
+//  - Three kernels per iteration: (1) q = K p for all cases, one record of p.q
+//    per SpMV item and case; (2) every vector item sums its system's p.q
+//    records itself (fixed order), alpha, x and r update, z, records of r.z
+//    and r.r; (3) every vector item sums its system's (r.z, r.r) records,
+//    beta, p = z + beta p; the first item of a case writes its state (stop
+//    test, iteration count; r.z double-buffered by parity).
+//  - Determinism: a system's records, their order and its item decomposition
+//    depend on that system alone, so its results are the same bits in any
+//    pack, alone or beside others.
+//  - Outer loop: FP64 residual b - K x per system on its FP64 CSR
+//    (multi.hip), scaling into the FP32 inner right-hand side, inner packed
+//    solve of the systems not yet converged, x += |r| d.  A system leaves the
+//    pack when all its cases meet the tolerance.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "bhip_internal.hpp"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxR = 8;
+constexpr int kChunk = 32;             // iterations per captured graph (even: r.z parity)
+constexpr int kSplitSlices = 2048;     // systems with fewer slices run four waves per slice
+constexpr double kStreamBytes = 240e6;  // packs whose SELL arrays exceed the 256 MB last-level cache stream
+
+struct PackState {
+    double rzb[2][kMaxR];  // r.z by iteration parity
+    double rr[kMaxR], rr0[kMaxR];
+    int done[kMaxR], iter[kMaxR], converged[kMaxR];
+    int pad;
+};
+
+struct PackSysDesc {
+    const int *sptr, *scol;
+    const float *sval;
+    long long nsell;
+    const float *dinv;
+    float *r, *p, *q, *d;  // case-major R x n
+    long long n;           // DOFs
+    int n_nodes, n_slices;
+    int split;             // 1: one slice per SpMV item, 0: four slices per item
+    int n_sitems, sitem0;  // SpMV items of the system, first item
+    int n_vitems, vitem0;  // vector items per case, first item
+    int maxit;             // inner iteration cap of this solve
+    double tol;            // inner relative residual reduction
+    long long rec_s;       // p.q records: R x n_sitems, case-major
+    long long rec_v;       // (r.z, r.r) records: R x n_vitems pairs
+    PackState *st;
+};
+
+struct PackHdr {
+    int n_sys, R, n_sitems, n_vitems;  // n_vitems: vector items per case over all systems
+    const PackSysDesc *sys;
+    const int *sitem_sys, *vitem_sys;
+    double *rec;
+};
+
+template <int NT, class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+// Fixed-order sum of nb records of NV doubles by one block, result in every
+// thread (thread t adds records t, t + 256, ...; eight loads in flight).
+template <int NV>
+__device__ __forceinline__ void sum_records_all(const double *rec, int nb, double (&tot)[NV]) {
+    __shared__ double sm[16 * NV];
+    __shared__ double res[NV];
+    constexpr int U = 8;
+#pragma unroll
+    for (int k = 0; k < NV; k++) tot[k] = 0;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += U * blockDim.x) {
+        double v[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int b = b0 + u * blockDim.x;
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[u][k] = b < nb ? rec[(long long)b * NV + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < NV; k++) tot[k] += v[u][k];
+    }
+    block_sum<NV>(tot, sm);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) res[k] = tot[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) tot[k] = res[k];
+}
+
+__device__ __forceinline__ void item_range(long long n, int lb, int nb, long long &b, long long &e) {
+    long long per = (n + nb - 1) / nb;
+    b = (long long)lb * per;
+    e = b + per;
+    if (e > n) e = n;
+    if (b > n) b = n;
+}
+
+// q_r = K p_r for every case of every live system; record of p_r.q_r per
+// item.  Small systems: the four waves share the item's slice, each taking
+// every fourth block column, and wave 0 adds the others' row sums in wave
+// order through LDS.
+template <int R, int NT>
+__global__ void __launch_bounds__(kThreads) k_pk_spmv(const PackHdr *__restrict__ hdr) {
+    __shared__ float red[3 * R][kThreads];
+    __shared__ double sm[16 * R];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n_items = hdr->n_sitems;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const PackSysDesc &D = hdr->sys[hdr->sitem_sys[item]];
+        const PackState *st = D.st;
+        bool live = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) live |= st->done[r] == 0;
+        if (!live) continue;
+        const int local = item - D.sitem0;
+        const int split = D.split;
+        const long long slice = split ? (long long)local : (long long)local * 4 + wave;
+        const long long node = slice * 64 + lane;
+        const long long ld = D.n;
+        double dots[R];
+        float a0[R], a1[R], a2[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            dots[r] = 0;
+            a0[r] = a1[r] = a2[r] = 0.f;
+        }
+        if (slice < D.n_slices) {
+            const int *scol = D.scol;
+            const float *sval = D.sval;
+            const float *x = D.p;
+            const long long nsell = D.nsell;
+            const int kb = D.sptr[slice], ke = D.sptr[slice + 1];
+            const int kstep = split ? 256 : 64;
+#pragma unroll 2
+            for (int k = kb + lane + (split ? 64 * wave : 0); k < ke; k += kstep) {
+                const long long m = ld_stream<NT>(scol + k);
+                const float v0 = ld_stream<NT>(sval + 0 * nsell + k), v1 = ld_stream<NT>(sval + 1 * nsell + k);
+                const float v2 = ld_stream<NT>(sval + 2 * nsell + k), v3 = ld_stream<NT>(sval + 3 * nsell + k);
+                const float v4 = ld_stream<NT>(sval + 4 * nsell + k), v5 = ld_stream<NT>(sval + 5 * nsell + k);
+                const float v6 = ld_stream<NT>(sval + 6 * nsell + k), v7 = ld_stream<NT>(sval + 7 * nsell + k);
+                const float v8 = ld_stream<NT>(sval + 8 * nsell + k);
+                const float *xm = x + 3 * m;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const float x0 = xm[r * ld], x1 = xm[r * ld + 1], x2 = xm[r * ld + 2];
+                    a0[r] += v0 * x0 + v1 * x1 + v2 * x2;
+                    a1[r] += v3 * x0 + v4 * x1 + v5 * x2;
+                    a2[r] += v6 * x0 + v7 * x1 + v8 * x2;
+                }
+            }
+        }
+        if (split) {
+            if (wave > 0)
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    red[3 * r][threadIdx.x] = a0[r];
+                    red[3 * r + 1][threadIdx.x] = a1[r];
+                    red[3 * r + 2][threadIdx.x] = a2[r];
+                }
+            __syncthreads();
+            if (wave == 0)
+                for (int w = 1; w < 4; w++)
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        a0[r] += red[3 * r][64 * w + lane];
+                        a1[r] += red[3 * r + 1][64 * w + lane];
+                        a2[r] += red[3 * r + 2][64 * w + lane];
+                    }
+        }
+        if (slice < D.n_slices && node < D.n_nodes && (!split || wave == 0)) {
+            const float *x = D.p;
+            float *y = D.q;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const long long j = r * ld + 3 * node;
+                y[j] = a0[r];
+                y[j + 1] = a1[r];
+                y[j + 2] = a2[r];
+                dots[r] = (double)x[j] * a0[r] + (double)x[j + 1] * a1[r] + (double)x[j + 2] * a2[r];
+            }
+        }
+        block_sum<R>(dots, sm);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int r = 0; r < R; r++) hdr->rec[D.rec_s + (long long)r * D.n_sitems + local] = dots[r];
+        __syncthreads();  // LDS is reused by the next item
+    }
+}
+
+// Per (case, vector item): alpha from the system's p.q records; x += alpha p,
+// r -= alpha q; records (r.z, r.r).
+template <int R>
+__global__ void __launch_bounds__(kThreads) k_pk_update(const PackHdr *__restrict__ hdr, int par) {
+    __shared__ double sm[16 * 2];
+    const int nv = hdr->n_vitems;
+    for (int item = blockIdx.x; item < nv * R; item += gridDim.x) {
+        const int r = item / nv, vi = item - r * nv;
+        const PackSysDesc &D = hdr->sys[hdr->vitem_sys[vi]];
+        const PackState *st = D.st;
+        if (st->done[r]) continue;
+        const double rz = st->rzb[par][r];
+        double pap[1];
+        sum_records_all<1>(hdr->rec + D.rec_s + (long long)r * D.n_sitems, D.n_sitems, pap);
+        const float alpha = (float)(rz / pap[0]);
+        const int lv = vi - D.vitem0;
+        long long i0, i1;
+        item_range(D.n, lv, D.n_vitems, i0, i1);
+        const long long o = (long long)r * D.n;
+        float *__restrict__ x = D.d;
+        float *__restrict__ rv = D.r;
+        const float *__restrict__ p = D.p;
+        const float *__restrict__ q = D.q;
+        const float *__restrict__ dinv = D.dinv;
+        double acc[2] = {0, 0};
+        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            const float ri = rv[o + i] - alpha * q[o + i];
+            x[o + i] = x[o + i] + alpha * p[o + i];
+            rv[o + i] = ri;
+            acc[0] += (double)ri * (ri * dinv[i]);
+            acc[1] += (double)ri * ri;
+        }
+        block_sum<2>(acc, sm);
+        if (threadIdx.x == 0) {
+            double *rec = hdr->rec + D.rec_v + ((long long)r * D.n_vitems + lv) * 2;
+            rec[0] = acc[0];
+            rec[1] = acc[1];
+        }
+        __syncthreads();
+    }
+}
+
+// Per (case, vector item): beta from the system's (r.z, r.r) records; the
+// first item of the case applies the stopping test and writes the state (a
+// later item of a finished case may skip its p update, which is then unused);
+// p = z + beta p.
+template <int R>
+__global__ void __launch_bounds__(kThreads) k_pk_pupdate(const PackHdr *__restrict__ hdr, int par) {
+    const int nv = hdr->n_vitems;
+    for (int item = blockIdx.x; item < nv * R; item += gridDim.x) {
+        const int r = item / nv, vi = item - r * nv;
+        const PackSysDesc &D = hdr->sys[hdr->vitem_sys[vi]];
+        PackState *st = D.st;
+        if (st->done[r]) continue;
+        const double rz = st->rzb[par][r];
+        double t[2];
+        sum_records_all<2>(hdr->rec + D.rec_v + (long long)r * D.n_vitems * 2, D.n_vitems, t);
+        const int lv = vi - D.vitem0;
+        if (lv == 0 && threadIdx.x == 0) {
+            const int it = st->iter[r] + 1;
+            const bool conv = sqrt(t[1] / st->rr0[r]) <= D.tol;
+            st->rzb[par ^ 1][r] = t[0];
+            st->rr[r] = t[1];
+            st->iter[r] = it;
+            if (conv) st->converged[r] = 1;
+            if (conv || it >= D.maxit) st->done[r] = 1;
+        }
+        const float beta = (float)(t[0] / rz);
+        long long i0, i1;
+        item_range(D.n, lv, D.n_vitems, i0, i1);
+        const long long o = (long long)r * D.n;
+        float *__restrict__ p = D.p;
+        const float *__restrict__ rv = D.r;
+        const float *__restrict__ dinv = D.dinv;
+        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[o + i] = rv[o + i] * dinv[i] + beta * p[o + i];
+    }
+}
+
+// Start of an inner solve: p = z = r dinv; records (r.z, r.r).
+template <int R>
+__global__ void __launch_bounds__(kThreads) k_pk_init(const PackHdr *__restrict__ hdr) {
+    __shared__ double sm[16 * 2];
+    const int nv = hdr->n_vitems;
+    for (int item = blockIdx.x; item < nv * R; item += gridDim.x) {
+        const int r = item / nv, vi = item - r * nv;
+        const PackSysDesc &D = hdr->sys[hdr->vitem_sys[vi]];
+        const int lv = vi - D.vitem0;
+        long long i0, i1;
+        item_range(D.n, lv, D.n_vitems, i0, i1);
+        const long long o = (long long)r * D.n;
+        double acc[2] = {0, 0};
+        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            const float ri = D.r[o + i];
+            const float zi = ri * D.dinv[i];
+            D.p[o + i] = zi;
+            acc[0] += (double)ri * zi;
+            acc[1] += (double)ri * ri;
+        }
+        block_sum<2>(acc, sm);
+        if (threadIdx.x == 0) {
+            double *rec = hdr->rec + D.rec_v + ((long long)r * D.n_vitems + lv) * 2;
+            rec[0] = acc[0];
+            rec[1] = acc[1];
+        }
+        __syncthreads();
+    }
+}
+
+// One block per (system, case): initial state from the init records.  A case
+// with a zero inner right-hand side (already converged) is solved by d = 0.
+template <int R>
+__global__ void __launch_bounds__(kThreads) k_pk_init_state(const PackHdr *__restrict__ hdr) {
+    for (int item = blockIdx.x; item < hdr->n_sys * R; item += gridDim.x) {
+        const int s = item / R, r = item - s * R;
+        const PackSysDesc &D = hdr->sys[s];
+        double t[2];
+        sum_records_all<2>(hdr->rec + D.rec_v + (long long)r * D.n_vitems * 2, D.n_vitems, t);
+        if (threadIdx.x == 0) {
+            PackState *st = D.st;
+            st->rzb[0][r] = t[0];
+            st->rzb[1][r] = 0;
+            st->rr[r] = t[1];
+            st->rr0[r] = t[1];
+            st->iter[r] = 0;
+            st->converged[r] = t[1] == 0.0;
+            st->done[r] = t[1] == 0.0;
+        }
+    }
+}
+
+template <int R>
+void enqueue_pk_iteration(int grid, hipStream_t s, const PackHdr *hdr, int par, bool nt) {
+    if (nt) k_pk_spmv<R, 1><<<grid, kThreads, 0, s>>>(hdr);
+    else k_pk_spmv<R, 0><<<grid, kThreads, 0, s>>>(hdr);
+    k_pk_update<R><<<grid, kThreads, 0, s>>>(hdr, par);
+    k_pk_pupdate<R><<<grid, kThreads, 0, s>>>(hdr, par);
+}
+
+void enqueue_iteration(int R, int grid, hipStream_t s, const PackHdr *hdr, int par, bool nt) {
+    switch (R) {
+        case 1: enqueue_pk_iteration<1>(grid, s, hdr, par, nt); break;
+        case 2: enqueue_pk_iteration<2>(grid, s, hdr, par, nt); break;
+        case 3: enqueue_pk_iteration<3>(grid, s, hdr, par, nt); break;
+        case 4: enqueue_pk_iteration<4>(grid, s, hdr, par, nt); break;
+        case 5: enqueue_pk_iteration<5>(grid, s, hdr, par, nt); break;
+        case 6: enqueue_pk_iteration<6>(grid, s, hdr, par, nt); break;
+        case 7: enqueue_pk_iteration<7>(grid, s, hdr, par, nt); break;
+        default: enqueue_pk_iteration<8>(grid, s, hdr, par, nt); break;
+    }
+}
+
+void enqueue_init(int R, int grid, hipStream_t s, const PackHdr *hdr) {
+#define BHIP_PK_INIT(RR)                                        \
+    case RR:                                                    \
+        k_pk_init<RR><<<grid, kThreads, 0, s>>>(hdr);           \
+        k_pk_init_state<RR><<<grid, kThreads, 0, s>>>(hdr);     \
+        break;
+    switch (R) {
+        BHIP_PK_INIT(1)
+        BHIP_PK_INIT(2)
+        BHIP_PK_INIT(3)
+        BHIP_PK_INIT(4)
+        BHIP_PK_INIT(5)
+        BHIP_PK_INIT(6)
+        BHIP_PK_INIT(7)
+        default:
+            k_pk_init<8><<<grid, kThreads, 0, s>>>(hdr);
+            k_pk_init_state<8><<<grid, kThreads, 0, s>>>(hdr);
+    }
+#undef BHIP_PK_INIT
+}
+
+bool graphs_enabled() {
+    const char *e = getenv("BHIP_NO_GRAPH");
+    return !(e && e[0] == '1');
+}
+
+// Per-system item counts (they depend on the system alone).
+int sys_split(const bhip_ctx *c) { return c->n_slices < kSplitSlices ? 1 : 0; }
+int sys_sitems(const bhip_ctx *c) { return sys_split(c) ? c->n_slices : (c->n_slices + 3) / 4; }
+int sys_vitems(const bhip_ctx *c) {
+    const long long want = ((long long)c->n_dof + kThreads * 8 - 1) / (kThreads * 8);
+    return (int)std::max<long long>(1, std::min<long long>(want, kRedBlocks));
+}
+
+}  // namespace
+
+struct PackEngine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int grid = 2048;
+    // descriptor block on the device: header | system descriptors | item tables
+    dvec<unsigned char> dbuf;
+    unsigned char *hbuf = nullptr;  // pinned staging of the descriptor block
+    size_t hcap = 0;
+    dvec<double> rec;               // records of the pack
+    dvec<PackState> st;             // one per system
+    PackState *h_st = nullptr;
+    size_t h_st_cap = 0;
+    dvec<double> resid;             // 2R per system: |b - K x|^2, |b|^2
+    double *h_resid = nullptr;
+    size_t h_resid_cap = 0;
+    double *h_scale = nullptr;      // per system: R scales then R norms
+    size_t h_scale_cap = 0;
+    hipGraphExec_t graph[2 * (kMaxR + 1)] = {};
+};
+
+namespace {
+
+void pk_graphs_reset(PackEngine *pe) {
+    for (auto &g : pe->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Report a HIP failure on every context of the pack.
+int pk_fail(bhip_ctx *const *ctxs, int n, hipError_t e, const char *what) {
+    const int code = e == hipErrorOutOfMemory ? BHIP_ERR_OOM : BHIP_ERR_HIP;
+    for (int i = 0; i < n; i++) bhip_fail(ctxs[i], code, std::string(what) + ": " + hipGetErrorString(e));
+    return code;
+}
+
+#define PK_TRY(call)                                             \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return pk_fail(ctxs, n_all, e_, #call); \
+    } while (0)
+
+template <class T>
+hipError_t pinned_reserve(T *&p, size_t &cap, size_t count) {
+    if (count <= cap && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipHostMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) cap = count;
+    return e;
+}
+
+}  // namespace
+
+int bhip_pack_engine_create(int device, hipStream_t stream, PackEngine **out) {
+    *out = nullptr;
+    PackEngine *pe = new PackEngine();
+    pe->device = device;
+    if (stream) {
+        pe->stream = stream;
+    } else {
+        if (hipStreamCreateWithFlags(&pe->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete pe;
+            return BHIP_ERR_HIP;
+        }
+        pe->own_stream = true;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) pe->grid = std::max(64, prop.multiProcessorCount * 8);
+    *out = pe;
+    return BHIP_OK;
+}
+
+void bhip_pack_engine_destroy(PackEngine *pe) {
+    if (!pe) return;
+    if (pe->stream) (void)hipStreamSynchronize(pe->stream);
+    pk_graphs_reset(pe);
+    pe->dbuf.release();
+    pe->rec.release();
+    pe->st.release();
+    pe->resid.release();
+    if (pe->hbuf) (void)hipHostFree(pe->hbuf);
+    if (pe->h_st) (void)hipHostFree(pe->h_st);
+    if (pe->h_resid) (void)hipHostFree(pe->h_resid);
+    if (pe->h_scale) (void)hipHostFree(pe->h_scale);
+    if (pe->own_stream) (void)hipStreamDestroy(pe->stream);
+    delete pe;
+}
+
+hipStream_t bhip_pack_engine_stream(PackEngine *pe) { return pe->stream; }
+
+// Solve the R load cases of n_all assembled contexts (FP32 SELL 3x3 form ready,
+// right-hand sides in multi->b64, x in multi->x64 zeroed) to opts.tol with
+// FP32-inner Jacobi-PCG and FP64 refinement, every system to its own
+// tolerance.  stats[i] and status[i] per context.  The contexts' streams must
+// be idle; all work runs on the engine's stream.
+int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const bhip_solve_opts &o,
+                  bhip_stats *stats, int *status) {
+    if (n_all <= 0) return BHIP_OK;
+    hipStream_t s = pe->stream;
+    const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-3;
+    const int max_outer = o.max_outer > 0 ? o.max_outer : 50;
+    const int maxit = o.maxit > 0 ? o.maxit : 10000;
+    // streaming loads when the pack's SELL arrays exceed the last-level cache
+    // (a single system follows its context's SpMV shape)
+    bool nt;
+    if (n_all == 1) {
+        nt = bhip_sell_streams(ctxs[0]);
+    } else {
+        double by = 0;
+        for (int i = 0; i < n_all; i++) by += (double)ctxs[i]->n_sell * 40.0;
+        nt = by > kStreamBytes;
+    }
+    std::vector<int> total(n_all, 0), outer(n_all, 0), active(n_all, 1), ok(n_all, 0);
+    for (int i = 0; i < n_all; i++) {
+        memset(&stats[i], 0, sizeof(bhip_stats));
+        status[i] = BHIP_OK;
+    }
+    PK_TRY(pe->resid.resize(std::max<size_t>(pe->resid.n, (size_t)n_all * 2 * R)));
+    PK_TRY(pinned_reserve(pe->h_resid, pe->h_resid_cap, (size_t)n_all * 2 * R));
+    PK_TRY(pinned_reserve(pe->h_scale, pe->h_scale_cap, (size_t)n_all * 2 * kMaxR));
+    for (int round = 0;; round++) {
+        // FP64 residuals of the active systems
+        for (int i = 0; i < n_all; i++)
+            if (active[i]) bhip_multi_enqueue_resid(ctxs[i], R, s, pe->resid.p + (size_t)i * 2 * R);
+        PK_TRY(hipGetLastError());
+        PK_TRY(hipMemcpyAsync(pe->h_resid, pe->resid.p, sizeof(double) * n_all * 2 * R, hipMemcpyDeviceToHost, s));
+        PK_TRY(hipStreamSynchronize(s));
+        std::vector<int> run;
+        for (int i = 0; i < n_all; i++) {
+            if (!active[i]) continue;
+            stats[i].spmv_launches += 1;
+            const double *rb = pe->h_resid + (size_t)i * 2 * R;
+            double *sc = pe->h_scale + (size_t)i * 2 * kMaxR;
+            bool all = true;
+            for (int r = 0; r < R; r++) {
+                const double rr = rb[2 * r], bb = rb[2 * r + 1];
+                const double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+                const bool conv = rel <= o.tol || rr == 0.0;
+                const double rn = conv ? 0.0 : sqrt(rr);
+                sc[r] = conv ? 0.0 : 1.0 / rn;  // a converged case gets a zero inner RHS
+                sc[kMaxR + r] = rn;
+                all &= conv;
+            }
+            if (all) {
+                ok[i] = 1;
+                active[i] = 0;
+                continue;
+            }
+            if (outer[i] == max_outer || total[i] >= maxit) {
+                active[i] = 0;
+                continue;
+            }
+            run.push_back(i);
+        }
+        if (run.empty()) break;
+        // inner right-hand sides r32 = r64 / |r64|, d = 0
+        for (int i : run) {
+            MultiData &M = *ctxs[i]->multi;
+            const double *sc = pe->h_scale + (size_t)i * 2 * kMaxR;
+            PK_TRY(hipMemcpyAsync(M.scale.p, sc, sizeof(double) * R, hipMemcpyHostToDevice, s));
+            PK_TRY(hipMemcpyAsync(M.rn.p, sc + kMaxR, sizeof(double) * R, hipMemcpyHostToDevice, s));
+            bhip_multi_enqueue_pack32(ctxs[i], R, s);
+        }
+        // descriptor of the systems in this inner solve
+        const int ns = (int)run.size();
+        int n_s = 0, n_v = 0;
+        long long n_rec = 0;
+        std::vector<PackSysDesc> desc(ns);
+        for (int k = 0; k < ns; k++) {
+            bhip_ctx *c = ctxs[run[k]];
+            MultiData &M = *c->multi;
+            PackSysDesc &D = desc[k];
+            D.sptr = c->sell_ptr.p;
+            D.scol = c->sell_col.p;
+            D.sval = c->bval32.p;
+            D.nsell = c->n_sell;
+            D.dinv = c->dinv32.p;
+            D.r = M.r32.p;
+            D.p = M.p32.p;
+            D.q = M.q32.p;
+            D.d = M.d32.p;
+            D.n = c->n_dof;
+            D.n_nodes = c->n_nodes;
+            D.n_slices = c->n_slices;
+            D.split = sys_split(c);
+            D.n_sitems = sys_sitems(c);
+            D.sitem0 = n_s;
+            D.n_vitems = sys_vitems(c);
+            D.vitem0 = n_v;
+            D.maxit = maxit - total[run[k]];
+            D.tol = inner_tol;
+            D.rec_s = n_rec;
+            n_rec += (long long)R * D.n_sitems;
+            D.rec_v = n_rec;
+            n_rec += (long long)R * D.n_vitems * 2;
+            n_s += D.n_sitems;
+            n_v += D.n_vitems;
+        }
+        if (pe->rec.n < (size_t)n_rec) PK_TRY(pe->rec.resize((size_t)n_rec));
+        if (pe->st.n < (size_t)ns) PK_TRY(pe->st.resize((size_t)std::max(ns, 16)));
+        PK_TRY(pinned_reserve(pe->h_st, pe->h_st_cap, (size_t)ns));
+        for (int k = 0; k < ns; k++) desc[k].st = pe->st.p + k;
+        const size_t off_sys = (sizeof(PackHdr) + 255) / 256 * 256;
+        const size_t off_s = off_sys + ((sizeof(PackSysDesc) * ns + 255) / 256 * 256);
+        const size_t off_v = off_s + ((sizeof(int) * n_s + 255) / 256 * 256);
+        const size_t bytes = off_v + sizeof(int) * (size_t)n_v;
+        if (pe->dbuf.n < bytes) {
+            // the graphs hold the descriptor's address
+            pk_graphs_reset(pe);
+            PK_TRY(pe->dbuf.resize(std::max(bytes, pe->dbuf.n * 2)));
+        }
+        PK_TRY(pinned_reserve(pe->hbuf, pe->hcap, bytes));
+        unsigned char *dev = pe->dbuf.p;
+        PackHdr hh;
+        hh.n_sys = ns;
+        hh.R = R;
+        hh.n_sitems = n_s;
+        hh.n_vitems = n_v;
+        hh.sys = reinterpret_cast<const PackSysDesc *>(dev + off_sys);
+        hh.sitem_sys = reinterpret_cast<const int *>(dev + off_s);
+        hh.vitem_sys = reinterpret_cast<const int *>(dev + off_v);
+        hh.rec = pe->rec.p;
+        memcpy(pe->hbuf, &hh, sizeof(hh));
+        memcpy(pe->hbuf + off_sys, desc.data(), sizeof(PackSysDesc) * ns);
+        int *hs = reinterpret_cast<int *>(pe->hbuf + off_s);
+        int *hv = reinterpret_cast<int *>(pe->hbuf + off_v);
+        for (int k = 0; k < ns; k++) {
+            for (int j = 0; j < desc[k].n_sitems; j++) hs[desc[k].sitem0 + j] = k;
+            for (int j = 0; j < desc[k].n_vitems; j++) hv[desc[k].vitem0 + j] = k;
+        }
+        PK_TRY(hipMemcpyAsync(dev, pe->hbuf, bytes, hipMemcpyHostToDevice, s));
+        const PackHdr *hdr = reinterpret_cast<const PackHdr *>(dev);
+        enqueue_init(R, pe->grid, s, hdr);
+        PK_TRY(hipGetLastError());
+        // chunks of kChunk iterations until every case of every system is done
+        hipGraphExec_t g = nullptr;
+        if (graphs_enabled()) {
+            hipGraphExec_t &slot = pe->graph[2 * R + (nt ? 1 : 0)];
+            if (!slot) {
+                hipGraph_t graph;
+                PK_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt);
+                hipError_t e = hipStreamEndCapture(s, &graph);
+                PK_TRY(e);
+                e = hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (e != hipSuccess) slot = nullptr;
+                PK_TRY(e);
+            }
+            g = slot;
+        }
+        int max_left = 0;
+        for (int k = 0; k < ns; k++) max_left = std::max(max_left, desc[k].maxit);
+        const long long max_chunks = (long long)max_left / kChunk + 2;
+        for (long long c = 0;; c++) {
+            PK_TRY(hipMemcpyAsync(pe->h_st, pe->st.p, sizeof(PackState) * ns, hipMemcpyDeviceToHost, s));
+            PK_TRY(hipStreamSynchronize(s));
+            bool all = true;
+            for (int k = 0; k < ns && all; k++)
+                for (int r = 0; r < R; r++) all &= pe->h_st[k].done[r] != 0;
+            if (all) break;
+            if (c > max_chunks) {
+                for (int i = 0; i < n_all; i++)
+                    bhip_fail(ctxs[i], BHIP_ERR_STATE, "packed PCG did not finish (internal error)");
+                return BHIP_ERR_STATE;
+            }
+            if (g) {
+                PK_TRY(hipGraphLaunch(g, s));
+            } else {
+                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt);
+            }
+            PK_TRY(hipGetLastError());
+            for (int k = 0; k < ns; k++) stats[run[k]].spmv_launches += kChunk;
+        }
+        // x += |r| d per system; iterations of this inner solve = longest case
+        for (int k = 0; k < ns; k++) {
+            const int i = run[k];
+            int it = 0;
+            for (int r = 0; r < R; r++) it = std::max(it, pe->h_st[k].iter[r]);
+            total[i] += it;
+            outer[i] += 1;
+            bhip_multi_enqueue_axpy(ctxs[i], R, s);
+        }
+        PK_TRY(hipGetLastError());
+    }
+    PK_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < n_all; i++) {
+        stats[i].iters = total[i];
+        stats[i].outer_iters = outer[i];
+        status[i] = ok[i] ? BHIP_OK : BHIP_ERR_NOT_CONVERGED;
+    }
+    return BHIP_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+struct bhip_pack {
+    int device = 0;
+    PackEngine *pe = nullptr;
+};
+
+extern "C" {
+
+int bhip_pack_create(int device, bhip_pack **out) {
+    if (!out) return BHIP_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return BHIP_ERR_HIP;
+    (void)hipSetDevice(device);
+    bhip_pack *pk = new bhip_pack();
+    pk->device = device;
+    if (bhip_pack_engine_create(device, nullptr, &pk->pe) != BHIP_OK) {
+        delete pk;
+        return BHIP_ERR_HIP;
+    }
+    *out = pk;
+    return BHIP_OK;
+}
+
+void bhip_pack_destroy(bhip_pack *pk) {
+    if (!pk) return;
+    bhip_pack_engine_destroy(pk->pe);
+    delete pk;
+}
+
+int bhip_pack_solve_cases(bhip_pack *pk, int n, bhip_ctx *const *ctxs, int nrhs, const double *const *presc,
+                          const double *const *load, double *const *x, const bhip_solve_opts *opts,
+                          bhip_stats *stats, int *status) {
+    if (!pk || n < 1 || !ctxs || !presc || !x || !opts || !status) return BHIP_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || !presc[i] || !x[i]) return BHIP_ERR_ARG;
+        if (ctxs[i]->device != pk->device) return bhip_fail(ctxs[i], BHIP_ERR_ARG, "context on another device");
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return bhip_fail(ctxs[i], BHIP_ERR_ARG, "a context appears twice in the pack");
+    }
+    (void)hipSetDevice(pk->device);
+    std::vector<bhip_stats> S(n);
+    std::vector<bhip_ctx *> run;
+    std::vector<int> idx;
+    for (int i = 0; i < n; i++) {
+        memset(&S[i], 0, sizeof(bhip_stats));
+        bool fused = false;
+        status[i] = bhip_cases_begin(ctxs[i], nrhs, presc[i], load ? load[i] : nullptr, opts, &fused);
+        if (status[i] == BHIP_OK && !fused)
+            status[i] = bhip_fail(ctxs[i], BHIP_ERR_UNSUPPORTED,
+                                  "packed solves need FP32_IR and 3 DOFs on every node (use bhip_solve_cases)");
+        if (status[i] != BHIP_OK) continue;
+        run.push_back(ctxs[i]);
+        idx.push_back(i);
+    }
+    if (!run.empty()) {
+        std::vector<bhip_stats> Sr(run.size());
+        std::vector<int> st(run.size(), BHIP_OK);
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess) return BHIP_ERR_HIP;
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return BHIP_ERR_HIP;
+        }
+        hipStream_t s = bhip_pack_engine_stream(pk->pe);
+        (void)hipEventRecord(e0, s);
+        const int e = bhip_pack_run(pk->pe, run.data(), (int)run.size(), nrhs, *opts, Sr.data(), st.data());
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (size_t j = 0; j < run.size(); j++) {
+            const int i = idx[j];
+            S[i] = Sr[j];
+            S[i].ms_total = ms;
+            int sj = e != BHIP_OK ? e : BHIP_OK;
+            if (sj == BHIP_OK) sj = bhip_cases_finish(run[j], nrhs, x[i], &S[i]);
+            if (sj == BHIP_OK && st[j] != BHIP_OK)
+                sj = bhip_fail(run[j], st[j], "multi-case solve not converged within the iteration caps");
+            status[i] = sj;
+        }
+    }
+    int first = BHIP_OK;
+    for (int i = 0; i < n; i++) {
+        if (stats) stats[i] = S[i];
+        if (first == BHIP_OK && status[i] != BHIP_OK) first = status[i];
+    }
+    return first;
+}
+
+}  // extern "C"

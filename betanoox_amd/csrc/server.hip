@@ -1,5 +1,6 @@
 // server.hip -- resident request loop for batches of load cases on many
-// meshes (host code; the device work goes through the contexts' C ABI).
+// meshes (host code; the device work goes through the contexts' C ABI and the
+// packed solver).
 //
 // Reference workflow served here: a macro-element / RVE homogenisation
 // (MultiFieldMacroElement::CalculateK, MultiFieldMacroElement.cpp:302-407) --
@@ -8,24 +9,32 @@
 // `max_slots` of them resident on one GPU at once.
 //
 // Design
-//  - Slots: each slot is a worker thread with its own context (own HIP
-//    stream, own device buffers, own captured graphs), so the work of
-//    different requests overlaps on the device and no state is shared.
-//  - Continuous batching: workers take the next queued request as soon as
+//  - Slots: each slot is a worker thread with a pool of contexts (seats, one
+//    per request of a pack; own HIP streams, device buffers) and a packed
+//    solver with its own stream, so the work of different slots overlaps on
+//    the device and no state is shared between slots.
+//  - Packing: a slot takes the request at the head of the queue and, when it
+//    is small (fewer than 2048 node slices, at most 8 load cases), also the
+//    small requests queued right behind it with the same number of cases, up
+//    to max_pack and the HBM budget.  It assembles each on its own seat, then
+//    solves all their cases in one packed batched PCG (pack.hip): one
+//    persistent kernel set, graph-captured chunks, every request keeping its
+//    own recurrences and stopping tests.
+//  - Continuous batching: workers take the next queued requests as soon as
 //    they are free; a request that arrives while others run starts as soon as
 //    a slot and device memory allow, without waiting for the batch to drain.
 //  - Admission by HBM footprint: every request carries an estimate of the
-//    device bytes its context needs; a slot charges the estimate of the
-//    request it holds (its buffers stay resident for reuse after the request
-//    ends) and a request is started only while the charges of all slots fit
+//    device bytes its context needs; a slot charges the estimates of the
+//    requests it holds (its buffers stay resident for reuse after the requests
+//    end) and requests are started only while the charges of all slots fit
 //    the budget.  Idle slots give their buffers back when a request does not
 //    fit otherwise.  Requests start in arrival order.
-//  - Reuse: a slot whose last request had the same connectivity and
+//  - Reuse: a seat whose last request had the same connectivity and
 //    restraints keeps its pattern (node graph, CSR, SELL layout) and only
 //    re-uploads coordinates, materials and values.
-//  - Determinism: every kernel of a solve runs on the slot's own stream with
-//    fixed-order reductions, so a request's result does not depend on what
-//    else runs beside it.
+//  - Determinism: every kernel of a solve runs with fixed-order reductions
+//    confined to its own request, so a request's result does not depend on
+//    what else runs beside it or shares its pack.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -59,16 +68,21 @@ struct Request {
     Clock::time_point t_submit, t_start, t_end;
 };
 
-struct Slot {
+struct Seat {
     bhip_ctx *ctx = nullptr;
-    std::thread th;
-    bool busy = false;
-    double charge = 0;  // device bytes attributed to this slot
     // topology of the last request (pattern reuse)
     bool have_topo = false;
     int n_nodes = 0, n_el = 0;
     std::vector<int> conn;
     std::vector<unsigned char> rest;
+};
+
+struct Slot {
+    std::thread th;
+    bool busy = false;
+    double charge = 0;  // device bytes attributed to this slot
+    PackEngine *pe = nullptr;
+    std::vector<Seat> seats;
 };
 
 }  // namespace
@@ -100,41 +114,19 @@ double estimate_bytes(long long n_nodes, long long n_el, int cases) {
     return (double)n_nodes * (4700.0 + 120.0 * r) + 1024.0 * (double)n_el + 64.0 * (1 << 20);
 }
 
-bool topo_same(const Slot &s, const Request &r) {
+bool topo_same(const Seat &s, const Request &r) {
     return s.have_topo && s.n_nodes == r.n_nodes && s.n_el == r.n_el && s.conn == r.conn && s.rest == r.rest;
 }
 
-int run_request(bhip_server *srv, Slot &slot, Request &r) {
-    int st = BHIP_OK;
-    if (!slot.ctx) {
-        st = bhip_create(srv->device, &slot.ctx);
-        if (st != BHIP_OK) return st;
-        slot.have_topo = false;
-    }
-    bhip_ctx *ctx = slot.ctx;
-    const long long n_dof = 3LL * r.n_nodes;
-    r.st.reused_pattern = 0;
-    if (topo_same(slot, r)) {
-        st = bhip_update_coords(ctx, r.xyz.data());
-        r.st.reused_pattern = 1;
-    } else {
-        slot.have_topo = false;
-        st = bhip_set_mesh(ctx, 3, r.n_nodes, r.xyz.data(), r.n_el, 8, r.conn.data(), nullptr, 3);
-        if (st == BHIP_OK) st = bhip_set_restraints(ctx, r.rest.data(), nullptr);
-        if (st == BHIP_OK) st = bhip_build_pattern(ctx);
-        if (st == BHIP_OK) {
-            slot.have_topo = true;
-            slot.n_nodes = r.n_nodes;
-            slot.n_el = r.n_el;
-            slot.conn = r.conn;
-            slot.rest = r.rest;
-        }
-    }
-    if (st == BHIP_OK)
-        st = bhip_set_materials(ctx, r.mat_kind, r.n_groups, r.mat.data(), r.group.empty() ? nullptr : r.group.data());
-    if (st == BHIP_OK) st = bhip_set_integration(ctx, r.order, 1.0);
-    if (st == BHIP_OK) st = bhip_assemble(ctx, BHIP_ASM_STIFFNESS, nullptr, nullptr);
-    r.x.assign((size_t)r.n_cases * n_dof, 0.0);
+// Requests that go into packs: fewer node slices than the packed solver's
+// split threshold, at most 8 cases (one case group).
+constexpr long long kPackMaxNodes = 2048LL * 64;
+bool packable(const bhip_server *srv, const Request &r) {
+    return srv->opts.max_pack > 1 && srv->opts.precision == BHIP_PREC_FP32_IR && r.n_cases <= 8 &&
+           r.n_nodes < kPackMaxNodes;
+}
+
+bhip_solve_opts solve_opts(const bhip_server *srv) {
     bhip_solve_opts o{};
     o.tol = srv->opts.tol;
     o.maxit = srv->opts.maxit;
@@ -144,23 +136,128 @@ int run_request(bhip_server *srv, Slot &slot, Request &r) {
     o.max_outer = 0;
     o.check_every = 0;
     o.use_x0 = 0;
-    for (int c0 = 0; c0 < r.n_cases && st == BHIP_OK; c0 += 8) {
-        const int nc = std::min(8, r.n_cases - c0);
-        bhip_stats s{};
-        st = bhip_solve_cases(ctx, nc, r.presc.data() + (size_t)c0 * n_dof, r.load.empty() ? nullptr : r.load.data(),
-                              r.x.data() + (size_t)c0 * n_dof, &o, &s);
-        r.st.iters += s.iters;
-        r.st.outer_iters = std::max(r.st.outer_iters, s.outer_iters);
-        r.st.rel_resid = std::max(r.st.rel_resid, s.rel_resid);
+    return o;
+}
+
+// Mesh, pattern (reused when the seat's last request had the same topology),
+// materials and stiffness assembly of one request on its seat.
+int setup_request(bhip_server *srv, Seat &seat, Request &r) {
+    int st = BHIP_OK;
+    if (!seat.ctx) {
+        st = bhip_create(srv->device, &seat.ctx);
+        if (st != BHIP_OK) return st;
+        seat.have_topo = false;
     }
-    if (st != BHIP_OK) {
-        std::strncpy(r.st.message, bhip_last_error(ctx), sizeof(r.st.message) - 1);
-        // a failed request leaves the context in an unknown state: start afresh
-        bhip_destroy(slot.ctx);
-        slot.ctx = nullptr;
-        slot.have_topo = false;
+    bhip_ctx *ctx = seat.ctx;
+    r.st.reused_pattern = 0;
+    if (topo_same(seat, r)) {
+        st = bhip_update_coords(ctx, r.xyz.data());
+        r.st.reused_pattern = 1;
+    } else {
+        seat.have_topo = false;
+        st = bhip_set_mesh(ctx, 3, r.n_nodes, r.xyz.data(), r.n_el, 8, r.conn.data(), nullptr, 3);
+        if (st == BHIP_OK) st = bhip_set_restraints(ctx, r.rest.data(), nullptr);
+        if (st == BHIP_OK) st = bhip_build_pattern(ctx);
+        if (st == BHIP_OK) {
+            seat.have_topo = true;
+            seat.n_nodes = r.n_nodes;
+            seat.n_el = r.n_el;
+            seat.conn = r.conn;
+            seat.rest = r.rest;
+        }
     }
+    if (st == BHIP_OK)
+        st = bhip_set_materials(ctx, r.mat_kind, r.n_groups, r.mat.data(), r.group.empty() ? nullptr : r.group.data());
+    if (st == BHIP_OK) st = bhip_set_integration(ctx, r.order, 1.0);
+    if (st == BHIP_OK) st = bhip_assemble(ctx, BHIP_ASM_STIFFNESS, nullptr, nullptr);
+    r.x.assign((size_t)r.n_cases * 3 * (size_t)r.n_nodes, 0.0);
     return st;
+}
+
+// A failed request leaves its seat's context in an unknown state: start afresh.
+void fail_seat(Seat &seat, Request &r, int st) {
+    r.st.status = st;
+    if (seat.ctx) {
+        std::strncpy(r.st.message, bhip_last_error(seat.ctx), sizeof(r.st.message) - 1);
+        bhip_destroy(seat.ctx);
+    }
+    seat.ctx = nullptr;
+    seat.have_topo = false;
+}
+
+// Runs the requests of one slot's pack: set-up on each seat, one packed solve
+// of all their cases (or, for a request that does not pack, its case groups
+// through bhip_solve_cases), then residuals and solutions per request.
+void run_pack(bhip_server *srv, Slot &slot, std::vector<Request *> &pk) {
+    const int np = (int)pk.size();
+    if ((int)slot.seats.size() < np) slot.seats.resize(np);
+    const bhip_solve_opts o = solve_opts(srv);
+    std::vector<bhip_ctx *> ctxs;
+    std::vector<int> idx;
+    for (int k = 0; k < np; k++) {
+        Request &r = *pk[k];
+        r.st.status = BHIP_OK;
+        r.st.pack_size = np;
+        const auto t0 = Clock::now();
+        int st = setup_request(srv, slot.seats[k], r);
+        if (st != BHIP_OK) {
+            fail_seat(slot.seats[k], r, st);
+            continue;
+        }
+        bhip_ctx *ctx = slot.seats[k].ctx;
+        const long long n_dof = 3LL * r.n_nodes;
+        if (np == 1 && !packable(srv, r)) {
+            // alone: case groups of up to 8 through bhip_solve_cases
+            const auto t1 = Clock::now();
+            r.st.setup_ms = ms_between(t0, t1);
+            for (int c0 = 0; c0 < r.n_cases && st == BHIP_OK; c0 += 8) {
+                const int nc = std::min(8, r.n_cases - c0);
+                bhip_stats s{};
+                st = bhip_solve_cases(ctx, nc, r.presc.data() + (size_t)c0 * n_dof,
+                                      r.load.empty() ? nullptr : r.load.data(), r.x.data() + (size_t)c0 * n_dof, &o, &s);
+                r.st.iters += s.iters;
+                r.st.outer_iters = std::max(r.st.outer_iters, s.outer_iters);
+                r.st.rel_resid = std::max(r.st.rel_resid, s.rel_resid);
+            }
+            r.st.solve_ms = ms_between(t1, Clock::now());
+            if (st != BHIP_OK) fail_seat(slot.seats[k], r, st);
+            continue;
+        }
+        bool fused = false;
+        st = bhip_cases_begin(ctx, r.n_cases, r.presc.data(), r.load.empty() ? nullptr : r.load.data(), &o, &fused);
+        if (st == BHIP_OK && !fused) st = bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "request cannot join a pack");
+        r.st.setup_ms = ms_between(t0, Clock::now());
+        if (st != BHIP_OK) {
+            fail_seat(slot.seats[k], r, st);
+            continue;
+        }
+        ctxs.push_back(ctx);
+        idx.push_back(k);
+    }
+    if (ctxs.empty()) return;
+    const int n = (int)ctxs.size();
+    const int R = pk[idx[0]]->n_cases;
+    std::vector<bhip_stats> S(n);
+    std::vector<int> status(n, BHIP_OK);
+    const auto t1 = Clock::now();
+    const int e = bhip_pack_run(slot.pe, ctxs.data(), n, R, o, S.data(), status.data());
+    const double solve_ms = ms_between(t1, Clock::now());
+    for (int j = 0; j < n; j++) {
+        Request &r = *pk[idx[j]];
+        Seat &seat = slot.seats[idx[j]];
+        int st = e != BHIP_OK ? e : BHIP_OK;
+        r.st.solve_ms = solve_ms;
+        r.st.iters = S[j].iters;
+        r.st.outer_iters = S[j].outer_iters;
+        if (st == BHIP_OK) {
+            bhip_stats fin{};
+            st = bhip_cases_finish(ctxs[j], R, r.x.data(), &fin);
+            r.st.rel_resid = fin.rel_resid;
+        }
+        if (st == BHIP_OK && status[j] != BHIP_OK)
+            st = bhip_fail(ctxs[j], status[j], "multi-case solve not converged within the iteration caps");
+        if (st != BHIP_OK) fail_seat(seat, r, st);
+    }
 }
 
 // Charges of every slot but `self`.
@@ -171,18 +268,24 @@ double others_charge(bhip_server *srv, size_t self) {
     return t;
 }
 
-// Under the lock: can slot `self` start `r` now?  Frees idle slots' buffers
-// when that makes the request fit.
-bool admit(bhip_server *srv, size_t self, const Request &r) {
-    if (others_charge(srv, self) + r.footprint <= srv->budget) return true;
+void release_slot(Slot &s) {
+    for (auto &seat : s.seats) {
+        if (seat.ctx) bhip_destroy(seat.ctx);
+        seat.ctx = nullptr;
+        seat.have_topo = false;
+    }
+    s.charge = 0;
+}
+
+// Under the lock: can slot `self` hold `bytes` more (beyond `held`) now?
+// Frees idle slots' buffers when that makes the request fit.
+bool admit(bhip_server *srv, size_t self, double held, double bytes) {
+    if (others_charge(srv, self) + held + bytes <= srv->budget) return true;
     for (size_t i = 0; i < srv->slots.size(); i++) {
         Slot &s = srv->slots[i];
-        if (i == self || s.busy || !s.ctx) continue;
-        bhip_destroy(s.ctx);
-        s.ctx = nullptr;
-        s.have_topo = false;
-        s.charge = 0;
-        if (others_charge(srv, self) + r.footprint <= srv->budget) return true;
+        if (i == self || s.busy || s.charge == 0) continue;
+        release_slot(s);
+        if (others_charge(srv, self) + held + bytes <= srv->budget) return true;
     }
     return false;
 }
@@ -190,35 +293,60 @@ bool admit(bhip_server *srv, size_t self, const Request &r) {
 void worker(bhip_server *srv, size_t self) {
     (void)hipSetDevice(srv->device);
     for (;;) {
-        Request *r = nullptr;
+        std::vector<Request *> pk;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
-            srv->cv_work.wait(lk, [&] { return srv->stop || (!srv->queue.empty() && admit(srv, self, *srv->queue.front())); });
+            srv->cv_work.wait(lk, [&] {
+                return srv->stop || (!srv->queue.empty() && admit(srv, self, 0.0, srv->queue.front()->footprint));
+            });
             if (srv->stop) return;
-            r = srv->queue.front();
+            Request *head = srv->queue.front();
             srv->queue.pop_front();
+            pk.push_back(head);
+            double held = head->footprint;
+            if (packable(srv, *head)) {
+                while (!srv->queue.empty() && (int)pk.size() < srv->opts.max_pack) {
+                    Request *nx = srv->queue.front();
+                    if (!packable(srv, *nx) || nx->n_cases != head->n_cases || !admit(srv, self, held, nx->footprint))
+                        break;
+                    srv->queue.pop_front();
+                    pk.push_back(nx);
+                    held += nx->footprint;
+                }
+            }
             Slot &s = srv->slots[self];
             s.busy = true;
-            s.charge = std::max(s.charge, r->footprint);  // old buffers may persist while resizing
-            r->state = 1;
-            r->t_start = Clock::now();
-            r->st.slot = (int)self;
+            s.charge = std::max(s.charge, held);  // old buffers may persist while resizing
+            const auto now = Clock::now();
+            for (Request *r : pk) {
+                r->state = 1;
+                r->t_start = now;
+                r->st.slot = (int)self;
+            }
         }
-        const int st = run_request(srv, srv->slots[self], *r);
+        run_pack(srv, srv->slots[self], pk);
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             Slot &s = srv->slots[self];
             s.busy = false;
-            s.charge = s.ctx ? r->footprint : 0;
-            r->t_end = Clock::now();
-            r->st.status = st;
-            r->st.queue_ms = ms_between(r->t_submit, r->t_start);
-            r->st.service_ms = ms_between(r->t_start, r->t_end);
-            r->st.latency_ms = ms_between(r->t_submit, r->t_end);
-            r->st.start_ms = ms_between(srv->t0, r->t_start);
-            r->st.end_ms = ms_between(srv->t0, r->t_end);
-            r->state = 2;
-            srv->completed++;
+            double held = 0;
+            for (size_t k = 0; k < pk.size(); k++)
+                if (k < s.seats.size() && s.seats[k].ctx) held += pk[k]->footprint;
+            // seats beyond this pack keep their buffers from earlier packs
+            for (size_t k = pk.size(); k < s.seats.size(); k++)
+                if (s.seats[k].ctx) held += bhip_request_footprint(s.seats[k].n_nodes, s.seats[k].n_el, 8);
+            s.charge = held;
+            const auto now = Clock::now();
+            for (Request *r : pk) {
+                r->t_end = now;
+                r->st.queue_ms = ms_between(r->t_submit, r->t_start);
+                r->st.service_ms = ms_between(r->t_start, r->t_end);
+                r->st.latency_ms = ms_between(r->t_submit, r->t_end);
+                r->st.start_ms = ms_between(srv->t0, r->t_start);
+                r->st.end_ms = ms_between(srv->t0, r->t_end);
+                r->state = 2;
+                srv->completed++;
+            }
         }
         srv->cv_done.notify_all();
         srv->cv_work.notify_all();
@@ -244,6 +372,8 @@ int bhip_server_create(int device, const bhip_server_opts *opts, bhip_server **o
     if (o.maxit <= 0) o.maxit = 40000;
     if (!(o.inner_tol > 0)) o.inner_tol = 1e-3;
     if (o.precision != BHIP_PREC_FP64) o.precision = BHIP_PREC_FP32_IR;
+    if (o.max_pack <= 0) o.max_pack = 8;
+    if (o.max_pack > 64) o.max_pack = 64;
     srv->opts = o;
     if (o.hbm_budget > 0) {
         srv->budget = o.hbm_budget;
@@ -257,6 +387,14 @@ int bhip_server_create(int device, const bhip_server_opts *opts, bhip_server **o
         srv->budget = 0.8 * (double)fr;
     }
     srv->slots.resize(o.max_slots);
+    (void)hipSetDevice(device);
+    for (int i = 0; i < o.max_slots; i++) {
+        if (bhip_pack_engine_create(device, nullptr, &srv->slots[i].pe) != BHIP_OK) {
+            for (int j = 0; j < i; j++) bhip_pack_engine_destroy(srv->slots[j].pe);
+            delete srv;
+            return BHIP_ERR_HIP;
+        }
+    }
     for (int i = 0; i < o.max_slots; i++) srv->slots[i].th = std::thread(worker, srv, (size_t)i);
     *out = srv;
     return BHIP_OK;
@@ -271,8 +409,10 @@ void bhip_server_destroy(bhip_server *srv) {
     srv->cv_work.notify_all();
     for (auto &s : srv->slots)
         if (s.th.joinable()) s.th.join();
-    for (auto &s : srv->slots)
-        if (s.ctx) bhip_destroy(s.ctx);
+    for (auto &s : srv->slots) {
+        release_slot(s);
+        bhip_pack_engine_destroy(s.pe);
+    }
     for (auto &kv : srv->reqs) delete kv.second;
     delete srv;
 }

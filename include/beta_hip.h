@@ -178,6 +178,24 @@ int bhip_spmv(bhip_ctx *ctx, const double *x, double *y);
  * The context's own prescribed values, load and RHS are left unchanged. */
 int bhip_solve_cases(bhip_ctx *ctx, int nrhs, const double *presc, const double *load, double *x,
                      const bhip_solve_opts *opts, bhip_stats *stats);
+/* Packed solve: bhip_solve_cases for n assembled contexts at once.  The
+ * reference solves one model's cases at a time (the CalculateK column loop,
+ * MultiFieldMacroElement.cpp:135-407); here the FP32 inner PCGs of all n
+ * contexts' cases advance together in one persistent set of kernels
+ * (graph-captured chunks), each case keeping its own recurrence and stopping
+ * test, every context to its own tolerance.  A context's results are the
+ * same bits as bhip_solve_cases gives it alone.  Every context needs
+ * BHIP_PREC_FP32_IR, 3 DOFs per node and a BHIP_ASM_STIFFNESS assembly, and
+ * all share nrhs (1..8).  presc[i], load[i] (load or load[i] may be NULL) and
+ * x[i] are laid out as in bhip_solve_cases; status[i] and stats[i] (stats may
+ * be NULL) per context; returns the first failing status or BHIP_OK.  The
+ * contexts must not be used by another thread meanwhile. */
+typedef struct bhip_pack bhip_pack;
+int  bhip_pack_create(int device, bhip_pack **out);
+void bhip_pack_destroy(bhip_pack *pk);
+int  bhip_pack_solve_cases(bhip_pack *pk, int n, bhip_ctx *const *ctxs, int nrhs, const double *const *presc,
+                           const double *const *load, double *const *x, const bhip_solve_opts *opts,
+                           bhip_stats *stats, int *status);
 /* Time `reps` FP64 (fp32 = 0) or FP32 (fp32 = 1) SpMVs on resident vectors;
  * returns the mean kernel time in ms (HIP events). */
 int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes);
@@ -319,12 +337,13 @@ int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode, const bhip
  * (MultiFieldMacroElement::CalculateK, MultiFieldMacroElement.cpp:302-407):
  * assemble a hex8 elasticity mesh, then solve it for several prescribed
  * boundary-displacement load cases (bhip_solve_cases).  The server keeps up to
- * max_slots requests resident on one GPU, each in its own context and HIP
- * stream, starts queued requests in arrival order as soon as a slot is free
- * and the requests' estimated device bytes fit the budget (continuous
- * batching), and reuses a slot's pattern when the next request has the same
- * connectivity and restraints.  A request's result does not depend on the
- * other requests in flight. */
+ * max_slots workers resident on one GPU, each with its own contexts and HIP
+ * streams, starts queued requests in arrival order as soon as a worker is
+ * free and the requests' estimated device bytes fit the budget (continuous
+ * batching), packs small requests queued together into one packed solve
+ * (bhip_pack_solve_cases), and reuses a context's pattern when its next
+ * request has the same connectivity and restraints.  A request's result does
+ * not depend on the other requests in flight or in its pack. */
 typedef struct bhip_server bhip_server;
 
 typedef struct bhip_server_opts {
@@ -334,6 +353,8 @@ typedef struct bhip_server_opts {
     double inner_tol;    /* FP32_IR inner residual reduction (0: 1e-3) */
     int    maxit;        /* inner iteration cap per solve (0: 40000) */
     int    precision;    /* BHIP_PREC_FP32_IR (default) or BHIP_PREC_FP64 */
+    int    max_pack;     /* small requests solved together in one packed batched PCG
+                            (0: 8; 1: no packing; at most 64) */
 } bhip_server_opts;
 
 /* One hex8 3D elasticity request (3 DOFs per node).  Every array is copied
@@ -365,6 +386,9 @@ typedef struct bhip_request_stats {
     double service_ms;       /* start -> done */
     double latency_ms;       /* submit -> done */
     double start_ms, end_ms; /* start and end, ms since the server was created */
+    int    pack_size;        /* requests solved in the same packed solve (1: alone) */
+    double setup_ms;         /* mesh, pattern, assembly and right-hand sides of this request */
+    double solve_ms;         /* the (packed) solve it took part in */
     char   message[256];     /* error text when status != BHIP_OK */
 } bhip_request_stats;
 

@@ -7,6 +7,8 @@ Checks:
     context with the same options (same kernels, same reduction order), and
     within 1e-8 true relative residual
   - a request gives the same bits alone and in a mix of concurrent requests
+  - requests packed into one batched solve (several packs of varying
+    composition) give the same bits as their private solves
   - a slot reuses its pattern for a request of the same topology, with the
     same bits as a fresh build
   - admission: with a budget that holds one request at a time, no two
@@ -37,7 +39,7 @@ def test_served_equals_direct_and_mix(hip_ctx_factory):
     probs = [meshgen.rve_request(n, 100 + n) for n in (6, 9, 12, 7, 10, 8)]
     print("requests", [p.n_dof for p in probs], flush=True)
     ref = [_direct(hip_ctx_factory, p) for p in probs]
-    with capi.Server(0, max_slots=4) as srv:
+    with capi.Server(0, max_slots=4, max_pack=1) as srv:
         # alone
         rid = srv.submit(probs[2])
         X_alone, s_alone = srv.wait(rid)
@@ -54,6 +56,29 @@ def test_served_equals_direct_and_mix(hip_ctx_factory):
     assert len({s["slot"] for _, s in out}) > 1
     spans = sorted((s["start_ms"], s["end_ms"]) for _, s in out)
     assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+
+
+def test_packed_requests_same_bits(hip_ctx_factory):
+    """Requests queued together are solved in packs; whatever the packs'
+    composition, every result equals the private solve bit for bit."""
+    probs = [meshgen.rve_request(n, 600 + n) for n in (6, 9, 12, 7, 10, 8, 11)]
+    ref = [_direct(hip_ctx_factory, p) for p in probs]
+    with capi.Server(0, max_slots=1, max_pack=8) as srv:
+        ids = [srv.submit(p) for p in probs]
+        out = [srv.wait(i) for i in ids]
+    sizes = [s["pack_size"] for _, s in out]
+    print("pack sizes", sizes, flush=True)
+    assert max(sizes) > 1
+    for (X, s), (Xr, sr) in zip(out, ref):
+        assert s["status"] == capi.OK and s["rel_resid"] <= TOL
+        assert s["iters"] == sr["iters"]
+        assert np.array_equal(X, Xr)
+    with capi.Server(0, max_slots=3, max_pack=4) as srv:
+        ids = [srv.submit(p) for p in probs[::-1]]
+        out2 = [srv.wait(i) for i in ids]
+    for (X, s), (Xr, _) in zip(out2, ref[::-1]):
+        assert s["status"] == capi.OK
+        assert np.array_equal(X, Xr)
 
 
 def test_pattern_reuse_same_bits(hip_ctx_factory):
