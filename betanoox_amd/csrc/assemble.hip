@@ -669,7 +669,7 @@ int bhip_lift_build(bhip_ctx *ctx) {
     hipStream_t s = ctx->stream;
     const int B = 256;
     const int nn = ctx->n_nodes;
-    dvec<long long> cnt;
+    dvec<long long> &cnt = ctx->tmp_l1;
     BHIP_TRY(ctx, cnt.resize((size_t)nn + 1));
     BHIP_TRY(ctx, ctx->lift_ptr.resize((size_t)nn + 1));
     k_lift_count<<<grid_for(nn + 1, B), B, 0, s>>>(nn, ctx->ndof.p, ctx->node_rmask.p, ctx->nbr_ptr.p, ctx->nbr.p,
@@ -723,7 +723,6 @@ int bhip_lift_build(bhip_ctx *ctx) {
         BHIP_TRY(ctx, hipGetLastError());
     }
     BHIP_TRY(ctx, hipStreamSynchronize(s));
-    cnt.release();
     ctx->lift_ready = true;
     return BHIP_OK;
 }

@@ -29,22 +29,33 @@ int bhip_fail(bhip_ctx *ctx, int status, const std::string &msg);
     } while (0)
 
 // ----------------------------------------------------------- device buffer
+// Device array whose allocation only grows: resizing to fewer elements keeps
+// the allocation (n shrinks, cap stays), so a context reused for meshes of
+// varying size does not free and reallocate.  hipFree waits for the device,
+// which would stall the work other contexts have in flight on other streams.
 template <class T>
 struct dvec {
     T *p = nullptr;
-    size_t n = 0;
+    size_t n = 0;    // elements in use
+    size_t cap = 0;  // elements allocated
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
-        n = 0;
+        n = cap = 0;
     }
-    // (Re)allocate to exactly count elements; contents undefined.
+    // n = count (0 releases); contents undefined.
     hipError_t resize(size_t count) {
-        if (count == n && p) return hipSuccess;
+        if (count == 0) {
+            release();
+            return hipSuccess;
+        }
+        if (p && count <= cap) {
+            n = count;
+            return hipSuccess;
+        }
         release();
-        if (count == 0) return hipSuccess;
         hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-        if (e == hipSuccess) n = count;
+        if (e == hipSuccess) n = cap = count;
         else p = nullptr;
         return e;
     }
@@ -114,6 +125,7 @@ struct bhip_ctx {
     long long n_sell = 0;    // padded block count
     dvec<int> sell_ptr, sell_col;
     dvec<float> bval32;      // SELL block values, SoA: B*B x n_sell
+    bool sell_layout_ok = false;  // SELL pointers/columns built for the current pattern
     int sell_shape = 0;      // tile shape id of the FP32 SELL SpMV
     bool sell_shape_set = false;  // chosen by bhip_tune_spmv / bhip_set_spmv_shape (else the
                                   // tuned default table picks it when the SELL layout is built)
@@ -157,6 +169,11 @@ struct bhip_ctx {
     struct PlasticData *plastic = nullptr;
     // ---- multi-case solve buffers and graph (multi.hip)
     struct MultiData *multi = nullptr;
+    // ---- scratch of pattern, layout and lift-table builds (grow-only, reused)
+    dvec<char> work;                   // hipcub temporary storage
+    dvec<int> tmp_i;                   // per-node counters
+    dvec<unsigned long long> tmp_k1, tmp_k2;  // node-pair keys
+    dvec<long long> tmp_l1, tmp_l2;    // per-DOF lengths and scans
 };
 
 // Blocking copy ordered on the context's stream.  Context streams are

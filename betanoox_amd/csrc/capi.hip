@@ -83,6 +83,12 @@ void bhip_destroy(bhip_ctx *ctx) {
     for (auto *v : vi) v->release();
     dvec<float> *v32[] = {&ctx->val32, &ctx->bval32, &ctx->r32, &ctx->p32, &ctx->q32, &ctx->d32, &ctx->dinv32};
     for (auto *v : v32) v->release();
+    ctx->work.release();
+    ctx->tmp_i.release();
+    ctx->tmp_k1.release();
+    ctx->tmp_k2.release();
+    ctx->tmp_l1.release();
+    ctx->tmp_l2.release();
     ctx->rest.release();
     ctx->node_rmask.release();
     ctx->lift_ptr.release();

@@ -345,6 +345,7 @@ int bhip_set_csr(bhip_ctx *ctx, int n, long long nnz, const int *ia, const int *
     ctx->sell_col.release();
     ctx->bval32.release();
     ctx->val_aux.release();
+    ctx->sell_layout_ok = false;
     ctx->n_pairs = 0;
     ctx->nnz = (long long)ci.size();
     ctx->nnz_lower = nl;

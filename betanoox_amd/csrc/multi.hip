@@ -169,11 +169,9 @@ int bhip_multi_prepare(bhip_ctx *ctx, int R, bool fused) {
     }
     MultiData *M = ctx->multi;
     const long long n = ctx->n_dof;
-    if (M->n != n || M->R < R) {
-        M->release_buffers();
-        M->n = n;
-        M->R = R;
-    }
+    // grow-only buffers: a new size keeps the allocations that are large enough
+    M->n = n;
+    M->R = std::max(M->R, R);
     const size_t nr = (size_t)n * M->R;
     BHIP_TRY(ctx, M->b64.resize(nr));
     BHIP_TRY(ctx, M->x64.resize(nr));

@@ -14,6 +14,8 @@
 // last in each row).  Because DOFs are numbered node-major with cumulative
 // first DOFs (node.cpp:206-214), node order == DOF order and the expanded
 // rows come out sorted.
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "bhip_internal.hpp"
@@ -171,11 +173,9 @@ template <class T>
 int exclusive_scan(bhip_ctx *ctx, const T *in, T *out, long long n) {
     size_t tb = 0;
     BHIP_TRY(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, ctx->stream));
-    dvec<char> tmp;
-    BHIP_TRY(ctx, tmp.resize(tb > 0 ? tb : 1));
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, (int)n, ctx->stream);
+    BHIP_TRY(ctx, ctx->work.resize(tb > 0 ? tb : 1));
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(ctx->work.p, tb, in, out, (int)n, ctx->stream);
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    tmp.release();
     BHIP_TRY(ctx, e);
     return BHIP_OK;
 }
@@ -201,7 +201,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
 
     // ---- node -> element incidence
     long long total = (long long)ctx->n_el * ctx->nnpe;
-    dvec<int> cnt;
+    dvec<int> &cnt = ctx->tmp_i;
     BHIP_TRY(ctx, cnt.resize(nn + 1));
     BHIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, sizeof(int) * (nn + 1), s));
     k_count_incidence<<<grid_for(total, B), B, 0, s>>>(ctx->conn.p, total, cnt.p);
@@ -217,7 +217,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
 
     // ---- node adjacency: sort + unique the (node_a, node_b) keys
     long long nkeys = total * ctx->nnpe;
-    dvec<unsigned long long> keys, keys2;
+    dvec<unsigned long long> &keys = ctx->tmp_k1, &keys2 = ctx->tmp_k2;
     BHIP_TRY(ctx, keys.resize(nkeys > 0 ? nkeys : 1));
     BHIP_TRY(ctx, keys2.resize(nkeys > 0 ? nkeys : 1));
     k_pair_keys<<<grid_for(nkeys, B), B, 0, s>>>(ctx->conn.p, ctx->n_el, ctx->nnpe, keys.p);
@@ -228,25 +228,23 @@ int bhip_pattern_build(bhip_ctx *ctx) {
         size_t tb = 0;
         BHIP_TRY(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.p, keys2.p, (int)nkeys, 0,
                                                        32 + nbits, s));
-        dvec<char> tmp;
-        BHIP_TRY(ctx, tmp.resize(tb > 0 ? tb : 1));
-        BHIP_TRY(ctx, hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys.p, keys2.p, (int)nkeys, 0,
+        BHIP_TRY(ctx, ctx->work.resize(tb > 0 ? tb : 1));
+        BHIP_TRY(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->work.p, tb, keys.p, keys2.p, (int)nkeys, 0,
                                                        32 + nbits, s));
         BHIP_TRY(ctx, hipStreamSynchronize(s));
     }
-    dvec<int> nsel;
-    BHIP_TRY(ctx, nsel.resize(1));
+    // the unique count lands in the per-DOF scratch, which is not in use yet
+    BHIP_TRY(ctx, ctx->tmp_l1.resize(std::max<size_t>(ctx->tmp_l1.n, 1)));
+    int *nsel = reinterpret_cast<int *>(ctx->tmp_l1.p);
     {
         size_t tb = 0;
-        BHIP_TRY(ctx, hipcub::DeviceSelect::Unique(nullptr, tb, keys2.p, keys.p, nsel.p, (int)nkeys, s));
-        dvec<char> tmp;
-        BHIP_TRY(ctx, tmp.resize(tb > 0 ? tb : 1));
-        BHIP_TRY(ctx, hipcub::DeviceSelect::Unique(tmp.p, tb, keys2.p, keys.p, nsel.p, (int)nkeys, s));
+        BHIP_TRY(ctx, hipcub::DeviceSelect::Unique(nullptr, tb, keys2.p, keys.p, nsel, (int)nkeys, s));
+        BHIP_TRY(ctx, ctx->work.resize(tb > 0 ? tb : 1));
+        BHIP_TRY(ctx, hipcub::DeviceSelect::Unique(ctx->work.p, tb, keys2.p, keys.p, nsel, (int)nkeys, s));
         BHIP_TRY(ctx, hipStreamSynchronize(s));
     }
-    keys2.release();
     int h_nsel = 0;
-    BHIP_TRY(ctx, bhip_copy(ctx, &h_nsel, nsel.p, sizeof(int), hipMemcpyDeviceToHost));
+    BHIP_TRY(ctx, bhip_copy(ctx, &h_nsel, nsel, sizeof(int), hipMemcpyDeviceToHost));
     ctx->n_pairs = h_nsel;
     long long np = ctx->n_pairs;
     BHIP_TRY(ctx, ctx->pair_row.resize(np > 0 ? np : 1));
@@ -255,14 +253,12 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, sizeof(int) * (nn + 1), s));
     k_split_keys<<<grid_for(np, B), B, 0, s>>>(keys.p, np, ctx->pair_row.p, ctx->nbr.p, cnt.p);
     BHIP_TRY(ctx, hipGetLastError());
-    keys.release();
     BHIP_TRY(ctx, ctx->nbr_ptr.resize(nn + 1));
     BHIP_PROPAGATE(exclusive_scan(ctx, cnt.p, ctx->nbr_ptr.p, nn + 1));
-    cnt.release();
 
     // ---- DOF rows
     const int nd = ctx->n_dof;
-    dvec<long long> rowlen, rowscan;
+    dvec<long long> &rowlen = ctx->tmp_l1, &rowscan = ctx->tmp_l2;
     BHIP_TRY(ctx, rowlen.resize(nd + 1));
     BHIP_TRY(ctx, rowscan.resize(nd + 1));
     BHIP_TRY(ctx, hipMemsetAsync(rowlen.p, 0, sizeof(long long) * (nd + 1), s));
@@ -303,11 +299,9 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     ctx->assembled = false;
     ctx->f32_ready = false;
     ctx->sell_shape_set = false;  // a tuned shape belongs to the pattern it was measured on
-    ctx->val32.release();
-    ctx->bval32.release();
-    ctx->sell_ptr.release();
-    ctx->sell_col.release();
-    ctx->val_aux.release();
+    // the FP32 layouts belong to the old pattern; their buffers are kept for reuse
+    ctx->sell_layout_ok = false;
+    ctx->val32.n = 0;
     return BHIP_OK;
 }
 

@@ -70,6 +70,8 @@ struct Request {
 
 struct Seat {
     bhip_ctx *ctx = nullptr;
+    double hw = 0;  // device bytes the context holds: its buffers only grow, so the
+                    // largest footprint of the requests it has run
     // topology of the last request (pattern reuse)
     bool have_topo = false;
     int n_nodes = 0, n_el = 0;
@@ -147,7 +149,9 @@ int setup_request(bhip_server *srv, Seat &seat, Request &r) {
         st = bhip_create(srv->device, &seat.ctx);
         if (st != BHIP_OK) return st;
         seat.have_topo = false;
+        seat.hw = 0;
     }
+    seat.hw = std::max(seat.hw, r.footprint);
     bhip_ctx *ctx = seat.ctx;
     r.st.reused_pattern = 0;
     if (topo_same(seat, r)) {
@@ -183,6 +187,7 @@ void fail_seat(Seat &seat, Request &r, int st) {
     }
     seat.ctx = nullptr;
     seat.have_topo = false;
+    seat.hw = 0;
 }
 
 // Runs the requests of one slot's pack: set-up on each seat, one packed solve
@@ -268,26 +273,47 @@ double others_charge(bhip_server *srv, size_t self) {
     return t;
 }
 
-void release_slot(Slot &s) {
-    for (auto &seat : s.seats) {
-        if (seat.ctx) bhip_destroy(seat.ctx);
-        seat.ctx = nullptr;
-        seat.have_topo = false;
+// Device bytes slot `s` holds once a pack with footprints fp runs on its
+// first fp.size() seats (the other seats keep what they hold).
+double slot_charge(const Slot &s, const std::vector<double> &fp) {
+    double t = 0;
+    const size_t ns = std::max(s.seats.size(), fp.size());
+    for (size_t k = 0; k < ns; k++) {
+        const double hw = k < s.seats.size() && s.seats[k].ctx ? s.seats[k].hw : 0.0;
+        const double f = k < fp.size() ? fp[k] : 0.0;
+        t += std::max(hw, f);
     }
+    return t;
+}
+
+void release_seat(Seat &seat) {
+    if (seat.ctx) bhip_destroy(seat.ctx);
+    seat.ctx = nullptr;
+    seat.have_topo = false;
+    seat.hw = 0;
+}
+
+void release_slot(Slot &s) {
+    for (auto &seat : s.seats) release_seat(seat);
     s.charge = 0;
 }
 
-// Under the lock: can slot `self` hold `bytes` more (beyond `held`) now?
-// Frees idle slots' buffers when that makes the request fit.
-bool admit(bhip_server *srv, size_t self, double held, double bytes) {
-    if (others_charge(srv, self) + held + bytes <= srv->budget) return true;
+// Under the lock: can idle slot `self` start a pack with footprints fp now?
+// Frees idle slots' buffers, then its own seats beyond the pack, when that
+// makes the pack fit.
+bool admit(bhip_server *srv, size_t self, const std::vector<double> &fp) {
+    Slot &me = srv->slots[self];
+    auto fits = [&] { return others_charge(srv, self) + slot_charge(me, fp) <= srv->budget; };
+    if (fits()) return true;
     for (size_t i = 0; i < srv->slots.size(); i++) {
         Slot &s = srv->slots[i];
         if (i == self || s.busy || s.charge == 0) continue;
         release_slot(s);
-        if (others_charge(srv, self) + held + bytes <= srv->budget) return true;
+        if (fits()) return true;
     }
-    return false;
+    for (size_t k = fp.size(); k < me.seats.size(); k++) release_seat(me.seats[k]);
+    me.charge = slot_charge(me, {});
+    return fits();
 }
 
 void worker(bhip_server *srv, size_t self) {
@@ -296,27 +322,33 @@ void worker(bhip_server *srv, size_t self) {
         std::vector<Request *> pk;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
+            std::vector<double> fp;
             srv->cv_work.wait(lk, [&] {
-                return srv->stop || (!srv->queue.empty() && admit(srv, self, 0.0, srv->queue.front()->footprint));
+                if (srv->stop) return true;
+                if (srv->queue.empty()) return false;
+                fp.assign(1, srv->queue.front()->footprint);
+                return admit(srv, self, fp);
             });
             if (srv->stop) return;
             Request *head = srv->queue.front();
             srv->queue.pop_front();
             pk.push_back(head);
-            double held = head->footprint;
             if (packable(srv, *head)) {
                 while (!srv->queue.empty() && (int)pk.size() < srv->opts.max_pack) {
                     Request *nx = srv->queue.front();
-                    if (!packable(srv, *nx) || nx->n_cases != head->n_cases || !admit(srv, self, held, nx->footprint))
+                    if (!packable(srv, *nx) || nx->n_cases != head->n_cases) break;
+                    fp.push_back(nx->footprint);
+                    if (!admit(srv, self, fp)) {
+                        fp.pop_back();
                         break;
+                    }
                     srv->queue.pop_front();
                     pk.push_back(nx);
-                    held += nx->footprint;
                 }
             }
             Slot &s = srv->slots[self];
             s.busy = true;
-            s.charge = std::max(s.charge, held);  // old buffers may persist while resizing
+            s.charge = slot_charge(s, fp);
             const auto now = Clock::now();
             for (Request *r : pk) {
                 r->state = 1;
@@ -329,13 +361,7 @@ void worker(bhip_server *srv, size_t self) {
             std::lock_guard<std::mutex> lk(srv->mu);
             Slot &s = srv->slots[self];
             s.busy = false;
-            double held = 0;
-            for (size_t k = 0; k < pk.size(); k++)
-                if (k < s.seats.size() && s.seats[k].ctx) held += pk[k]->footprint;
-            // seats beyond this pack keep their buffers from earlier packs
-            for (size_t k = pk.size(); k < s.seats.size(); k++)
-                if (s.seats[k].ctx) held += bhip_request_footprint(s.seats[k].n_nodes, s.seats[k].n_el, 8);
-            s.charge = held;
+            s.charge = slot_charge(s, {});
             const auto now = Clock::now();
             for (Request *r : pk) {
                 r->t_end = now;

@@ -684,7 +684,7 @@ int build_sell(bhip_ctx *ctx) {
     hipStream_t s = ctx->stream;
     ctx->n_slices = (ctx->n_nodes + 63) / 64;
     const int ns = ctx->n_slices;
-    dvec<int> width;
+    dvec<int> &width = ctx->tmp_i;
     BHIP_TRY(ctx, width.resize(ns + 1));
     BHIP_TRY(ctx, hipMemsetAsync(width.p, 0, sizeof(int) * (ns + 1), s));
     k_sell_width<<<grid_for(ns, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, width.p);
@@ -710,6 +710,7 @@ int build_sell(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
     BHIP_TRY(ctx, hipGetLastError());
     if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
+    ctx->sell_layout_ok = true;
     return BHIP_OK;
 }
 
@@ -719,7 +720,8 @@ int prepare_f32(bhip_ctx *ctx) {
     const int nb = ctx->max_ndof;
     ctx->use_bsr = (ctx->uniform_ndof && nb >= 1 && nb <= 3 && (long long)ctx->n_dof == (long long)nb * ctx->n_nodes);
     if (ctx->use_bsr) {
-        if (ctx->sell_b != nb || ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_col.p) {
+        if (ctx->sell_b != nb || ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_col.p ||
+            !ctx->sell_layout_ok) {
             ctx->sell_b = nb;
             BHIP_PROPAGATE(build_sell(ctx));
         }
