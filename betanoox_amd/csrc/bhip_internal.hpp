@@ -123,7 +123,9 @@ struct bhip_ctx {
     // w_s block columns; entry (k, lane) of slice s is at sell_ptr[s] + 64k + lane
     int n_slices = 0;
     long long n_sell = 0;    // padded block count
-    dvec<int> sell_ptr, sell_col;
+    dvec<int> sell_ptr, sell_col;  // slice starts; block columns (node index)
+    dvec<short> sell_col16;        // block columns as offsets from the row's node (narrow form)
+    bool sell_narrow = false;      // 16-bit offsets in use
     dvec<float> bval32;      // SELL block values, SoA: B*B x n_sell
     bool sell_layout_ok = false;  // SELL pointers/columns built for the current pattern
     int sell_shape = 0;      // tile shape id of the FP32 SELL SpMV

@@ -59,7 +59,9 @@ struct PackState {
 };
 
 struct PackSysDesc {
-    const int *sptr, *scol;
+    const int *sptr, *scol;  // scol: node indices (wide form)
+    const short *scol16;     // offsets from the row's node (narrow form)
+    int narrow, pad0;
     const float *sval;
     long long nsell;
     const float *dinv;
@@ -128,10 +130,37 @@ __device__ __forceinline__ void item_range(long long n, int lb, int nb, long lon
     if (b > n) b = n;
 }
 
+// Row sums of one lane over block columns k = k0, k0 + kstep, ... < ke of its
+// node's row for all R cases (CT: wide node indices or narrow offsets).
+template <int R, int NT, class CT>
+__device__ __forceinline__ void pk_rows(const CT *__restrict__ scol, const float *__restrict__ sval, long long nsell,
+                                        const float *__restrict__ x, long long ld, long long node, int k0, int ke,
+                                        int kstep, float (&a0)[R], float (&a1)[R], float (&a2)[R]) {
+#pragma unroll 2
+    for (int k = k0; k < ke; k += kstep) {
+        long long m;
+        if constexpr (sizeof(CT) == 2) m = node + (long long)ld_stream<NT>(scol + k);
+        else m = ld_stream<NT>(scol + k);
+        const float v0 = ld_stream<NT>(sval + 0 * nsell + k), v1 = ld_stream<NT>(sval + 1 * nsell + k);
+        const float v2 = ld_stream<NT>(sval + 2 * nsell + k), v3 = ld_stream<NT>(sval + 3 * nsell + k);
+        const float v4 = ld_stream<NT>(sval + 4 * nsell + k), v5 = ld_stream<NT>(sval + 5 * nsell + k);
+        const float v6 = ld_stream<NT>(sval + 6 * nsell + k), v7 = ld_stream<NT>(sval + 7 * nsell + k);
+        const float v8 = ld_stream<NT>(sval + 8 * nsell + k);
+        const float *xm = x + 3 * m;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float x0 = xm[r * ld], x1 = xm[r * ld + 1], x2 = xm[r * ld + 2];
+            a0[r] += v0 * x0 + v1 * x1 + v2 * x2;
+            a1[r] += v3 * x0 + v4 * x1 + v5 * x2;
+            a2[r] += v6 * x0 + v7 * x1 + v8 * x2;
+        }
+    }
+}
+
 // q_r = K p_r for every case of every live system; record of p_r.q_r per
 // item.  Small systems: the four waves share the item's slice, each taking
 // every fourth block column, and wave 0 adds the others' row sums in wave
-// order through LDS.
+// order through LDS.  Lanes past the system's last node skip their rows.
 template <int R, int NT>
 __global__ void __launch_bounds__(kThreads) k_pk_spmv(const PackHdr *__restrict__ hdr) {
     __shared__ float red[3 * R][kThreads];
@@ -157,30 +186,11 @@ __global__ void __launch_bounds__(kThreads) k_pk_spmv(const PackHdr *__restrict_
             dots[r] = 0;
             a0[r] = a1[r] = a2[r] = 0.f;
         }
-        if (slice < D.n_slices) {
-            const int *scol = D.scol;
-            const float *sval = D.sval;
-            const float *x = D.p;
-            const long long nsell = D.nsell;
+        if (slice < D.n_slices && node < D.n_nodes) {
             const int kb = D.sptr[slice], ke = D.sptr[slice + 1];
-            const int kstep = split ? 256 : 64;
-#pragma unroll 2
-            for (int k = kb + lane + (split ? 64 * wave : 0); k < ke; k += kstep) {
-                const long long m = ld_stream<NT>(scol + k);
-                const float v0 = ld_stream<NT>(sval + 0 * nsell + k), v1 = ld_stream<NT>(sval + 1 * nsell + k);
-                const float v2 = ld_stream<NT>(sval + 2 * nsell + k), v3 = ld_stream<NT>(sval + 3 * nsell + k);
-                const float v4 = ld_stream<NT>(sval + 4 * nsell + k), v5 = ld_stream<NT>(sval + 5 * nsell + k);
-                const float v6 = ld_stream<NT>(sval + 6 * nsell + k), v7 = ld_stream<NT>(sval + 7 * nsell + k);
-                const float v8 = ld_stream<NT>(sval + 8 * nsell + k);
-                const float *xm = x + 3 * m;
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const float x0 = xm[r * ld], x1 = xm[r * ld + 1], x2 = xm[r * ld + 2];
-                    a0[r] += v0 * x0 + v1 * x1 + v2 * x2;
-                    a1[r] += v3 * x0 + v4 * x1 + v5 * x2;
-                    a2[r] += v6 * x0 + v7 * x1 + v8 * x2;
-                }
-            }
+            const int k0 = kb + lane + (split ? 64 * wave : 0), kstep = split ? 256 : 64;
+            if (D.narrow) pk_rows<R, NT, short>(D.scol16, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
+            else pk_rows<R, NT, int>(D.scol, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
         }
         if (split) {
             if (wave > 0)
@@ -520,7 +530,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
         nt = bhip_sell_streams(ctxs[0]);
     } else {
         double by = 0;
-        for (int i = 0; i < n_all; i++) by += (double)ctxs[i]->n_sell * 40.0;
+        for (int i = 0; i < n_all; i++) by += (double)ctxs[i]->n_sell * (ctxs[i]->sell_narrow ? 38.0 : 40.0);
         nt = by > kStreamBytes;
     }
     std::vector<int> total(n_all, 0), outer(n_all, 0), active(n_all, 1), ok(n_all, 0);
@@ -585,6 +595,9 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
             PackSysDesc &D = desc[k];
             D.sptr = c->sell_ptr.p;
             D.scol = c->sell_col.p;
+            D.scol16 = c->sell_col16.p;
+            D.narrow = c->sell_narrow ? 1 : 0;
+            D.pad0 = 0;
             D.sval = c->bval32.p;
             D.nsell = c->n_sell;
             D.dinv = c->dinv32.p;

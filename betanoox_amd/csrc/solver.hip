@@ -39,6 +39,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kFinThreads = 1024;
+constexpr int kPf = 8;  // elements per thread the fused vector kernels load ahead
 
 // ------------------------------------------------------------- partials
 template <int NV>
@@ -158,6 +159,11 @@ __global__ void __launch_bounds__(kThreads) k_csr_spmv(int n, const int *__restr
 // neighbouring x entries.  No cross-lane reduction; padded blocks carry zeros
 // and the node's own column.
 //
+// Columns: CT = int holds the neighbour's node index; CT = short holds its
+// offset from the row's node (used when every offset fits 16 bits, which
+// saves 2 of the 40 bytes a 3x3 block costs).  Lanes past the last node skip
+// the loop, so a padded entry's offset 0 is never gathered beyond the mesh.
+//
 // Tile shape (chosen per matrix by bhip_tune_spmv): SPB slices per workgroup
 // (one wave each, SPB = 4, 8 or 16), UNR block columns per loop trip, NT
 // streaming (nontemporal) loads of block values and columns, which are read
@@ -171,10 +177,10 @@ __device__ __forceinline__ T ld_stream(const T *p) {
     else return *p;
 }
 
-template <int B, int MODE, int SPB, int UNR, int NT>
+template <int B, int MODE, int SPB, int UNR, int NT, class CT>
 __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slices,
                                                       const int *__restrict__ sptr,
-                                                      const int *__restrict__ scol,
+                                                      const CT *__restrict__ scol,
                                                       const float *__restrict__ sval, long long nsell,
                                                       const float *__restrict__ x,
                                                       float *__restrict__ y, const PcgState *st,
@@ -186,13 +192,15 @@ __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slice
     double acc = 0;
     if (s < n_slices) {
         const long long node = s * 64 + lane;
-        const int kb = sptr[s], ke = sptr[s + 1];
+        const int kb = sptr[s], ke = node < n_nodes ? sptr[s + 1] : kb;
         float a[B];
 #pragma unroll
         for (int c = 0; c < B; c++) a[c] = 0.f;
 #pragma unroll UNR
         for (int k = kb + lane; k < ke; k += 64) {
-            const long long m = ld_stream<NT>(scol + k);
+            long long m;
+            if constexpr (sizeof(CT) == 2) m = node + (long long)ld_stream<NT>(scol + k);
+            else m = ld_stream<NT>(scol + k);
             float xv[B];
 #pragma unroll
             for (int d = 0; d < B; d++) xv[d] = x[B * m + d];
@@ -280,6 +288,23 @@ __global__ void __launch_bounds__(kThreads) k_pcg_update_fa(long long n, int par
                                                           const T *__restrict__ q, const T *__restrict__ dinv,
                                                           PcgState *st, double *__restrict__ pout) {
     if (st->done) return;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    long long i0, i1;
+    block_range(n, lb, gridDim.x, i0, i1);
+    // this thread's first kPf elements are loaded before the record sum, whose
+    // latency they then overlap (they do not depend on alpha)
+    T px[kPf], pp[kPf], pr[kPf], pq[kPf], pdv[kPf];
+#pragma unroll
+    for (int u = 0; u < kPf; u++) {
+        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
+        if (i < i1) {
+            px[u] = x[i];
+            pp[u] = p[i];
+            pr[u] = r[i];
+            pq[u] = q[i];
+            pdv[u] = dinv[i];
+        }
+    }
     const double rz = st->rzb[par];
     double pap[1];
     sum_records_all<1>(pin, nb_in, pap);
@@ -288,12 +313,23 @@ __global__ void __launch_bounds__(kThreads) k_pcg_update_fa(long long n, int par
         st->pAp = pap[0];
         st->alpha = alpha_d;
     }
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    long long i0, i1;
-    block_range(n, lb, gridDim.x, i0, i1);
     const T alpha = (T)alpha_d;
     double rzn = 0, rr = 0, xx = 0;
-    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < kPf; u++) {
+        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
+        if (i < i1) {
+            T xi = px[u] + alpha * pp[u];
+            T ri = pr[u] - alpha * pq[u];
+            T zi = ri * pdv[u];
+            x[i] = xi;
+            r[i] = ri;
+            rzn += (double)ri * (double)zi;
+            rr += (double)ri * (double)ri;
+            xx += (double)xi * (double)xi;
+        }
+    }
+    for (long long i = i0 + threadIdx.x + (long long)kPf * blockDim.x; i < i1; i += blockDim.x) {
         T xi = x[i] + alpha * p[i];
         T ri = r[i] - alpha * q[i];
         T zi = ri * dinv[i];
@@ -318,6 +354,19 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate_fb(long long n, int pa
                                                            const T *__restrict__ r, const T *__restrict__ dinv,
                                                            PcgState *st) {
     if (st->done) return;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    long long i0, i1;
+    block_range(n, lb, gridDim.x, i0, i1);
+    T pp[kPf], pr[kPf], pdv[kPf];  // loaded before the record sum, as in k_pcg_update_fa
+#pragma unroll
+    for (int u = 0; u < kPf; u++) {
+        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
+        if (i < i1) {
+            pp[u] = p[i];
+            pr[u] = r[i];
+            pdv[u] = dinv[i];
+        }
+    }
     const double rz = st->rzb[par];
     double t[3];
     sum_records_all<3>(pin, nb_in, t);
@@ -347,11 +396,14 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate_fb(long long n, int pa
         if (conv) st->converged = 1;
         if (conv || iter >= maxit) st->done = 1;
     }
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    long long i0, i1;
-    block_range(n, lb, gridDim.x, i0, i1);
     const T beta = (T)beta_d;
-    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = r[i] * dinv[i] + beta * p[i];
+#pragma unroll
+    for (int u = 0; u < kPf; u++) {
+        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
+        if (i < i1) p[i] = pr[u] * pdv[u] + beta * pp[u];
+    }
+    for (long long i = i0 + threadIdx.x + (long long)kPf * blockDim.x; i < i1; i += blockDim.x)
+        p[i] = r[i] * dinv[i] + beta * p[i];
 }
 
 // Initial: z = r*dinv, p = z; partials r.z, r.r, x.x.
@@ -487,28 +539,39 @@ __global__ void k_sell_width(int n_nodes, int n_slices, const int *nbr_ptr, int 
     width[s] = w * 64;
 }
 
-// Padding: own column (zero values are set by a fill beforehand).
-__global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, int *scol) {
+// Padding: own column (zero values are set by a fill beforehand); offset 0
+// in the 16-bit form.
+template <class CT>
+__global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, CT *scol) {
     long long s = blockIdx.x;
     if (s >= n_slices) return;
     for (int k = sptr[s] + threadIdx.x; k < sptr[s + 1]; k += blockDim.x) {
         long long node = s * 64 + (k - sptr[s]) % 64;
-        scol[k] = node < n_nodes ? (int)node : 0;
+        if constexpr (sizeof(CT) == 2) scol[k] = 0;
+        else scol[k] = node < n_nodes ? (int)node : 0;
     }
+}
+
+// Largest |neighbour - node| over the node adjacency (into *out, zeroed before).
+__global__ void k_max_offset(long long n_pairs, const int *pair_row, const int *nbr, int *out) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    atomicMax(out, abs(nbr[p] - pair_row[p]));
 }
 
 // FP64 CSR -> FP32 SELL-64 B x B blocks (SoA); restrained entries zero,
 // restrained diagonal 1.  Pair p = (node n, k-th neighbour m).
-template <int B>
+template <int B, class CT>
 __global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
                               const int *nbr_off, const int *rowptr, const unsigned char *rmask,
-                              const double *val, const int *sptr, long long nsell, int *scol, float *sval) {
+                              const double *val, const int *sptr, long long nsell, CT *scol, float *sval) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     int n = pair_row[p], m = nbr[p];
     int k = (int)(p - nbr_ptr[n]);
     long long dst = (long long)sptr[n / 64] + 64LL * k + (n % 64);
-    scol[dst] = m;
+    if constexpr (sizeof(CT) == 2) scol[dst] = (CT)(m - n);
+    else scol[dst] = m;
     unsigned rn = rmask[n], rm = rmask[m];
     for (int c = 0; c < B; c++) {
         int r = 0;
@@ -594,9 +657,15 @@ void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *
 template <int B, int MODE, int SPB, int UNR, int NT>
 void launch_sell_t(bhip_ctx *ctx, const float *x, float *y, const PcgState *st) {
     long long g = (ctx->n_slices + SPB - 1) / SPB;
-    k_sell_spmv<B, MODE, SPB, UNR, NT><<<(int)(g > 0 ? g : 1), 64 * SPB, 0, ctx->stream>>>(
-        ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
-        ctx->partials.p);
+    const int grid = (int)(g > 0 ? g : 1);
+    if (ctx->sell_narrow)
+        k_sell_spmv<B, MODE, SPB, UNR, NT, short><<<grid, 64 * SPB, 0, ctx->stream>>>(
+            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col16.p, ctx->bval32.p, ctx->n_sell, x, y, st,
+            ctx->partials.p);
+    else
+        k_sell_spmv<B, MODE, SPB, UNR, NT, int><<<grid, 64 * SPB, 0, ctx->stream>>>(
+            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
+            ctx->partials.p);
 }
 
 // Tile shapes in shape-id order (bhip_tune_spmv reports and sets the id).
@@ -634,8 +703,8 @@ void launch_sell(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgSta
 // for the blocked form the stored (padded) blocks are counted.
 double spmv_bytes(bhip_ctx *ctx, bool fp32, bool bsr) {
     if (bsr)
-        return (double)ctx->n_sell * (ctx->sell_b * ctx->sell_b * 4 + 4) + (ctx->n_slices + 1.0) * 4 +
-               2.0 * ctx->n_dof * 4;
+        return (double)ctx->n_sell * (ctx->sell_b * ctx->sell_b * 4 + (ctx->sell_narrow ? 2 : 4)) +
+               (ctx->n_slices + 1.0) * 4 + 2.0 * ctx->n_dof * 4;
     double sv = fp32 ? 4 : 8;
     return (double)ctx->nnz * (sv + 4) + (ctx->n_dof + 1.0) * 4 + 2.0 * ctx->n_dof * sv;
 }
@@ -704,9 +773,23 @@ int build_sell(bhip_ctx *ctx) {
     bhip_invalidate_graphs(ctx);
     BHIP_TRY(ctx, ctx->sell_ptr.resize(ns + 1));
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->sell_ptr.p, hp.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
-    BHIP_TRY(ctx, ctx->sell_col.resize(tot > 0 ? tot : 1));
+    // 16-bit column offsets when every neighbour lies within 32767 nodes of its row's node
+    BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p + 3, 0, sizeof(int), s));
+    k_max_offset<<<grid_for(ctx->n_pairs, 256), 256, 0, s>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p,
+                                                              ctx->flags.p + 3);
+    BHIP_TRY(ctx, hipGetLastError());
+    int max_off = 0;
+    BHIP_TRY(ctx, hipMemcpyAsync(&max_off, ctx->flags.p + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    BHIP_TRY(ctx, hipStreamSynchronize(s));
+    const char *wide = getenv("BHIP_SELL_WIDE_COLUMNS");
+    ctx->sell_narrow = max_off <= 32767 && !(wide && wide[0] == '1');
+    if (ctx->sell_narrow) BHIP_TRY(ctx, ctx->sell_col16.resize(tot > 0 ? tot : 1));
+    else BHIP_TRY(ctx, ctx->sell_col.resize(tot > 0 ? tot : 1));
     BHIP_TRY(ctx, ctx->bval32.resize((size_t)ctx->sell_b * ctx->sell_b * (tot > 0 ? tot : 1)));
-    k_sell_pad<<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
+    if (ctx->sell_narrow)
+        k_sell_pad<short><<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col16.p);
+    else
+        k_sell_pad<int><<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
     BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
     BHIP_TRY(ctx, hipGetLastError());
     if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
@@ -720,15 +803,21 @@ int prepare_f32(bhip_ctx *ctx) {
     const int nb = ctx->max_ndof;
     ctx->use_bsr = (ctx->uniform_ndof && nb >= 1 && nb <= 3 && (long long)ctx->n_dof == (long long)nb * ctx->n_nodes);
     if (ctx->use_bsr) {
-        if (ctx->sell_b != nb || ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_col.p ||
-            !ctx->sell_layout_ok) {
+        if (ctx->sell_b != nb || ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_layout_ok) {
             ctx->sell_b = nb;
             BHIP_PROPAGATE(build_sell(ctx));
         }
 #define BHIP_TO_SELL(BB)                                                                                      \
-    k_csr_to_sell<BB><<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(                                       \
-        ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,             \
-        ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col.p, ctx->bval32.p)
+    do {                                                                                                      \
+        if (ctx->sell_narrow)                                                                                 \
+            k_csr_to_sell<BB, short><<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(                        \
+                ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,     \
+                ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col16.p, ctx->bval32.p); \
+        else                                                                                                  \
+            k_csr_to_sell<BB, int><<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(                          \
+                ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,     \
+                ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col.p, ctx->bval32.p);  \
+    } while (0)
         if (nb == 3) BHIP_TO_SELL(3);
         else if (nb == 2) BHIP_TO_SELL(2);
         else BHIP_TO_SELL(1);
@@ -1088,6 +1177,7 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
     const double by = spmv_bytes(ctx, true, true);
     for (int id = 0; id < ns && id < max_entries; id++) {
         bhip_spmv_shape_info(id, &table[id]);
+        table[id].column_bytes = ctx->sell_narrow ? 2 : 4;
         table[id].ms = best[id];
         table[id].gbps = by / best[id] / 1e6;
     }

@@ -209,6 +209,9 @@ typedef struct bhip_spmv_shape {
     int    slices_per_block; /* 64-node slices (one wave each) per workgroup: 4, 8 or 16 */
     int    unroll;           /* block columns per loop trip */
     int    nontemporal;      /* 1: streaming loads of block values and columns */
+    int    column_bytes;     /* bytes per stored block column of the tuned matrix: 2 (16-bit
+                                offsets from the row's node) or 4 (node indices); 0 from
+                                bhip_spmv_shape_describe */
     double ms;               /* best mean kernel time of the tuning rounds (HIP events) */
     double gbps;             /* algorithmic bytes / ms */
 } bhip_spmv_shape;

@@ -73,6 +73,7 @@ def test_tune_keeps_fastest_and_bits(hip_ctx_factory):
     ms = np.array([t["ms"] for t in table])
     assert np.all(ms > 0) and np.all(np.isfinite(ms))
     assert chosen == int(np.argmin(ms))
+    assert all(t["column_bytes"] == 2 for t in table)  # 16-bit offsets on this mesh
     assert ctx.spmv_shape() == chosen
     st, x = _solve(ctx)
     assert st["iters"] == st0["iters"]
@@ -86,3 +87,66 @@ def test_shape_argument_errors(hip_ctx_factory):
     with pytest.raises(capi.BhipError):
         ctx.set_spmv_shape(-1)
     assert ctx.spmv_shape() == 0
+
+
+def _permuted(prob, seed):
+    """The same mesh with its nodes renumbered at random (wide node bandwidth)."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(prob.n_nodes)          # old node i -> new node perm[i]
+    d = prob.ndof
+    dof_new = (perm[:, None] * d + np.arange(d)).ravel()
+    def per_dof(a):
+        out = np.empty_like(a)
+        out[dof_new] = a
+        return out
+    xyz = np.empty_like(prob.xyz)
+    xyz[perm] = prob.xyz
+    q = dataclasses.replace(prob, xyz=xyz, conn=perm[prob.conn].astype(np.int32),
+                            restrained=per_dof(prob.restrained), prescribed=per_dof(prob.prescribed),
+                            load=per_dof(prob.load), extra=dict(prob.extra))
+    return q, dof_new
+
+
+def test_narrow_and_wide_columns(hip_ctx_factory, monkeypatch):
+    """16-bit column offsets and int32 columns give the same bits; a mesh whose
+    node numbering spans more than 32767 nodes between neighbours runs on the
+    int32 columns and agrees with the natural numbering."""
+    prob, _, _ = _problem("hex11")
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    st_n, x_n = _solve(ctx)
+    cases = np.stack([prob.prescribed] * 2)
+    o = capi.Context.opts(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=20000)
+    Xc_n, _ = ctx.solve_cases(cases, opts=o)
+    monkeypatch.setenv("BHIP_SELL_WIDE_COLUMNS", "1")
+    ctx_w = setup_problem(hip_ctx_factory(), prob)
+    ctx_w.assemble(capi.ASM_STIFFNESS)
+    ctx_w.set_load(prob.load)
+    ctx_w.build_rhs()
+    st_w, x_w = _solve(ctx_w)
+    Xc_w, _ = ctx_w.solve_cases(cases, opts=o)
+    _, by_n = ctx.bench_spmv(fp32=True, reps=1)
+    _, by_w = ctx_w.bench_spmv(fp32=True, reps=1)
+    monkeypatch.delenv("BHIP_SELL_WIDE_COLUMNS")
+    assert by_w > by_n  # 4-byte against 2-byte columns
+    assert st_w["iters"] == st_n["iters"]
+    assert np.array_equal(x_w.view(np.uint64), x_n.view(np.uint64))
+    assert np.array_equal(Xc_w, Xc_n)
+    # natural (narrow) against randomly renumbered (wide) numbering, 33^3 nodes
+    big = meshgen.elastic_cube(32)
+    perm_prob, dof_new = _permuted(big, 5)
+    res = []
+    for p in (big, perm_prob):
+        c = setup_problem(hip_ctx_factory(), p)
+        c.assemble(capi.ASM_STIFFNESS)
+        c.set_load(p.load)
+        c.build_rhs()
+        st = c.solve(tol=1e-10, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
+        assert st["rel_resid"] <= 1e-10
+        res.append(c.solution())
+    x_nat, x_perm = res
+    err = np.abs(x_perm[dof_new] - x_nat).max() / np.abs(x_nat).max()
+    assert err < 1e-7, err
