@@ -4,9 +4,10 @@ Two levels:
   * at run time, bhip_tune_spmv (Context.tune_spmv) times every SELL tile
     shape on a context's own matrix and keeps the fastest;
   * this module sweeps that tuner over the seeded workload generators (1, 2
-    and 3 DOFs per node, a range of mesh sizes) and, for the multi-case SpMV
-    of the served path, times six-case solves with streaming loads on and
-    off.  --emit-header turns a sweep into csrc/tuned_shapes.hpp, the table a
+    and 3 DOFs per node, a range of mesh sizes) and, for the packed SpMV of
+    the multi-case and served path, times six-case solves with streaming
+    loads on and off (BHIP_PACK_STREAMING; the packed solver streams when its
+    systems' SELL arrays exceed 240 MB).  --emit-header turns a sweep into csrc/tuned_shapes.hpp, the table a
     context uses when nobody tuned it (rebuild the library afterwards), and
     --emit-md into a short summary for profiles/.
 
@@ -43,7 +44,6 @@ QUICK = [
     ("plane_stress_quad", 2, [256, 1024]),
 ]
 RVE_SIZES = [16, 24, 32, 48, 64, 80, 96]
-STREAM_SHAPE = {3: 9, 2: 9, 1: 11}   # streaming counterparts used for the multi-case check
 TIE = 0.01                          # shapes within 1% + 0.2 us of the best count as equal
 TIE_US = 0.2
 
@@ -87,8 +87,8 @@ def sweep(quick=False, reps=10):
         with capi.Context(0) as ctx:
             setup_problem(ctx, prob)
             ctx.assemble()
-            for label, sid in (("plain", 0), ("stream", STREAM_SHAPE[3])):
-                ctx.set_spmv_shape(sid)
+            for label, flag in (("plain", "0"), ("stream", "1")):
+                os.environ["BHIP_PACK_STREAMING"] = flag
                 ctx.solve_cases(P, opts=o)          # warm-up (graph capture)
                 best = 1e300
                 for _ in range(3):
@@ -96,6 +96,7 @@ def sweep(quick=False, reps=10):
                     best = min(best, s["ms_total"])
                 rec["ms"][label] = best
                 rec["iters"] = s["iters"]
+            os.environ.pop("BHIP_PACK_STREAMING", None)
         out["multi"].append(rec)
         log("rve %d: six cases plain %.2f ms, streaming %.2f ms" % (n, rec["ms"]["plain"], rec["ms"]["stream"]))
     return out

@@ -251,8 +251,6 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
                         int *chosen);
 int bhip_spmv_shape_info(int id, bhip_spmv_shape *info);
 int bhip_set_spmv_shape_impl(bhip_ctx *ctx, int id);
-// 1 if the context's SELL shape uses streaming loads (the multi-case SpMV follows it)
-bool bhip_sell_streams(const bhip_ctx *ctx);
 void bhip_invalidate_graphs(bhip_ctx *ctx);
 void bhip_plastic_release(bhip_ctx *ctx);
 void bhip_multi_invalidate(bhip_ctx *ctx);

@@ -524,15 +524,14 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
     const int max_outer = o.max_outer > 0 ? o.max_outer : 50;
     const int maxit = o.maxit > 0 ? o.maxit : 10000;
     // streaming loads when the pack's SELL arrays exceed the last-level cache
-    // (a single system follows its context's SpMV shape)
-    bool nt;
-    if (n_all == 1) {
-        nt = bhip_sell_streams(ctxs[0]);
-    } else {
-        double by = 0;
-        for (int i = 0; i < n_all; i++) by += (double)ctxs[i]->n_sell * (ctxs[i]->sell_narrow ? 38.0 : 40.0);
-        nt = by > kStreamBytes;
-    }
+    // (measured crossover for six cases: RVE 48^3, 121 MB, runs faster with
+    // plain loads; 64^3, 277 MB, with streaming loads)
+    // BHIP_PACK_STREAMING=0/1 forces the choice (the autotune sweep compares both)
+    double by = 0;
+    for (int i = 0; i < n_all; i++) by += (double)ctxs[i]->n_sell * (ctxs[i]->sell_narrow ? 38.0 : 40.0);
+    bool nt = by > kStreamBytes;
+    const char *force = getenv("BHIP_PACK_STREAMING");
+    if (force && (force[0] == '0' || force[0] == '1')) nt = force[0] == '1';
     std::vector<int> total(n_all, 0), outer(n_all, 0), active(n_all, 1), ok(n_all, 0);
     for (int i = 0; i < n_all; i++) {
         memset(&stats[i], 0, sizeof(bhip_stats));

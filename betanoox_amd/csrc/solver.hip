@@ -39,7 +39,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kFinThreads = 1024;
-constexpr int kPf = 8;  // elements per thread the fused vector kernels load ahead
 
 // ------------------------------------------------------------- partials
 template <int NV>
@@ -288,23 +287,6 @@ __global__ void __launch_bounds__(kThreads) k_pcg_update_fa(long long n, int par
                                                           const T *__restrict__ q, const T *__restrict__ dinv,
                                                           PcgState *st, double *__restrict__ pout) {
     if (st->done) return;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    long long i0, i1;
-    block_range(n, lb, gridDim.x, i0, i1);
-    // this thread's first kPf elements are loaded before the record sum, whose
-    // latency they then overlap (they do not depend on alpha)
-    T px[kPf], pp[kPf], pr[kPf], pq[kPf], pdv[kPf];
-#pragma unroll
-    for (int u = 0; u < kPf; u++) {
-        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
-        if (i < i1) {
-            px[u] = x[i];
-            pp[u] = p[i];
-            pr[u] = r[i];
-            pq[u] = q[i];
-            pdv[u] = dinv[i];
-        }
-    }
     const double rz = st->rzb[par];
     double pap[1];
     sum_records_all<1>(pin, nb_in, pap);
@@ -313,23 +295,12 @@ __global__ void __launch_bounds__(kThreads) k_pcg_update_fa(long long n, int par
         st->pAp = pap[0];
         st->alpha = alpha_d;
     }
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    long long i0, i1;
+    block_range(n, lb, gridDim.x, i0, i1);
     const T alpha = (T)alpha_d;
     double rzn = 0, rr = 0, xx = 0;
-#pragma unroll
-    for (int u = 0; u < kPf; u++) {
-        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
-        if (i < i1) {
-            T xi = px[u] + alpha * pp[u];
-            T ri = pr[u] - alpha * pq[u];
-            T zi = ri * pdv[u];
-            x[i] = xi;
-            r[i] = ri;
-            rzn += (double)ri * (double)zi;
-            rr += (double)ri * (double)ri;
-            xx += (double)xi * (double)xi;
-        }
-    }
-    for (long long i = i0 + threadIdx.x + (long long)kPf * blockDim.x; i < i1; i += blockDim.x) {
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         T xi = x[i] + alpha * p[i];
         T ri = r[i] - alpha * q[i];
         T zi = ri * dinv[i];
@@ -354,19 +325,6 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate_fb(long long n, int pa
                                                            const T *__restrict__ r, const T *__restrict__ dinv,
                                                            PcgState *st) {
     if (st->done) return;
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    long long i0, i1;
-    block_range(n, lb, gridDim.x, i0, i1);
-    T pp[kPf], pr[kPf], pdv[kPf];  // loaded before the record sum, as in k_pcg_update_fa
-#pragma unroll
-    for (int u = 0; u < kPf; u++) {
-        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
-        if (i < i1) {
-            pp[u] = p[i];
-            pr[u] = r[i];
-            pdv[u] = dinv[i];
-        }
-    }
     const double rz = st->rzb[par];
     double t[3];
     sum_records_all<3>(pin, nb_in, t);
@@ -396,14 +354,11 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate_fb(long long n, int pa
         if (conv) st->converged = 1;
         if (conv || iter >= maxit) st->done = 1;
     }
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    long long i0, i1;
+    block_range(n, lb, gridDim.x, i0, i1);
     const T beta = (T)beta_d;
-#pragma unroll
-    for (int u = 0; u < kPf; u++) {
-        const long long i = i0 + threadIdx.x + (long long)u * blockDim.x;
-        if (i < i1) p[i] = pr[u] * pdv[u] + beta * pp[u];
-    }
-    for (long long i = i0 + threadIdx.x + (long long)kPf * blockDim.x; i < i1; i += blockDim.x)
-        p[i] = r[i] * dinv[i] + beta * p[i];
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = r[i] * dinv[i] + beta * p[i];
 }
 
 // Initial: z = r*dinv, p = z; partials r.z, r.r, x.x.
@@ -552,11 +507,16 @@ __global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, CT *scol)
     }
 }
 
-// Largest |neighbour - node| over the node adjacency (into *out, zeroed before).
+// Largest |neighbour - node| over the node adjacency (into *out, zeroed
+// before): grid-stride per thread, wave maximum, one atomic per wave.
 __global__ void k_max_offset(long long n_pairs, const int *pair_row, const int *nbr, int *out) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
-    atomicMax(out, abs(nbr[p] - pair_row[p]));
+    int v = 0;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs;
+         p += (long long)gridDim.x * blockDim.x)
+        v = max(v, abs(nbr[p] - pair_row[p]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, v);
 }
 
 // FP64 CSR -> FP32 SELL-64 B x B blocks (SoA); restrained entries zero,
@@ -775,8 +735,8 @@ int build_sell(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->sell_ptr.p, hp.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
     // 16-bit column offsets when every neighbour lies within 32767 nodes of its row's node
     BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p + 3, 0, sizeof(int), s));
-    k_max_offset<<<grid_for(ctx->n_pairs, 256), 256, 0, s>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p,
-                                                              ctx->flags.p + 3);
+    k_max_offset<<<grid_for(ctx->n_pairs, 256, 4096), 256, 0, s>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p,
+                                                                    ctx->flags.p + 3);
     BHIP_TRY(ctx, hipGetLastError());
     int max_off = 0;
     BHIP_TRY(ctx, hipMemcpyAsync(&max_off, ctx->flags.p + 3, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1196,9 +1156,4 @@ int bhip_set_spmv_shape_impl(bhip_ctx *ctx, int id) {
     ctx->sell_shape = id;
     ctx->sell_shape_set = true;
     return BHIP_OK;
-}
-
-bool bhip_sell_streams(const bhip_ctx *ctx) {
-    bhip_spmv_shape info;
-    return bhip_spmv_shape_info(ctx->sell_shape, &info) == BHIP_OK && info.nontemporal != 0;
 }

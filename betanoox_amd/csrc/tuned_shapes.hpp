@@ -13,19 +13,19 @@ struct TunedSellRow {
 
 constexpr TunedSellRow kTunedSell[] = {
     // B = 3, measured at n = 16, 24, 32, 40, 48, 56, 64, 72, 80, 96, 112 (elastic_cube)
-    {3, 371, 0},
     {3, 777, 2},
     {3, 2306, 3},
     {3, 3524, 1},
+    {3, 5107, 2},
     {3, 1LL << 40, 9},
     // B = 2, measured at n = 128, 256, 512, 768, 1024, 1536, 2048 (plane_stress_quad)
     {2, 2061, 0},
     {2, 6165, 9},
-    {2, 12317, 3},
-    {2, 24617, 4},
+    {2, 24617, 3},
     {2, 1LL << 40, 9},
     // B = 1, measured at n = 32, 48, 64, 80, 96, 112, 128, 160 (diffusion_cube)
     {1, 2809, 11},
-    {1, 17931, 2},
+    {1, 5969, 0},
+    {1, 27500, 2},
     {1, 1LL << 40, 11},
 };
