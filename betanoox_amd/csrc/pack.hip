@@ -49,7 +49,8 @@ constexpr int kThreads = 256;
 constexpr int kMaxR = 8;
 constexpr int kChunk = 32;             // iterations per captured graph (even: r.z parity)
 constexpr int kSplitSlices = 2048;     // systems with fewer slices run four waves per slice
-constexpr double kStreamBytes = 240e6;  // packs whose SELL arrays exceed the 256 MB last-level cache stream
+constexpr double kStreamBytes = 240e6;
+constexpr int kPackUnroll = 2;         // SpMV block columns per loop trip  // packs whose SELL arrays exceed the 256 MB last-level cache stream
 
 struct PackState {
     double rzb[2][kMaxR];  // r.z by iteration parity
@@ -132,11 +133,11 @@ __device__ __forceinline__ void item_range(long long n, int lb, int nb, long lon
 
 // Row sums of one lane over block columns k = k0, k0 + kstep, ... < ke of its
 // node's row for all R cases (CT: wide node indices or narrow offsets).
-template <int R, int NT, class CT>
+template <int R, int NT, int UNR, class CT>
 __device__ __forceinline__ void pk_rows(const CT *__restrict__ scol, const float *__restrict__ sval, long long nsell,
                                         const float *__restrict__ x, long long ld, long long node, int k0, int ke,
                                         int kstep, float (&a0)[R], float (&a1)[R], float (&a2)[R]) {
-#pragma unroll 2
+#pragma unroll UNR
     for (int k = k0; k < ke; k += kstep) {
         long long m;
         if constexpr (sizeof(CT) == 2) m = node + (long long)ld_stream<NT>(scol + k);
@@ -161,7 +162,8 @@ __device__ __forceinline__ void pk_rows(const CT *__restrict__ scol, const float
 // item.  Small systems: the four waves share the item's slice, each taking
 // every fourth block column, and wave 0 adds the others' row sums in wave
 // order through LDS.  Lanes past the system's last node skip their rows.
-template <int R, int NT>
+// UNR block columns per loop trip (1: fewer registers, more waves per SIMD).
+template <int R, int NT, int UNR>
 __global__ void __launch_bounds__(kThreads) k_pk_spmv(const PackHdr *__restrict__ hdr) {
     __shared__ float red[3 * R][kThreads];
     __shared__ double sm[16 * R];
@@ -189,8 +191,10 @@ __global__ void __launch_bounds__(kThreads) k_pk_spmv(const PackHdr *__restrict_
         if (slice < D.n_slices && node < D.n_nodes) {
             const int kb = D.sptr[slice], ke = D.sptr[slice + 1];
             const int k0 = kb + lane + (split ? 64 * wave : 0), kstep = split ? 256 : 64;
-            if (D.narrow) pk_rows<R, NT, short>(D.scol16, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
-            else pk_rows<R, NT, int>(D.scol, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
+            if (D.narrow)
+                pk_rows<R, NT, UNR, short>(D.scol16, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
+            else
+                pk_rows<R, NT, UNR, int>(D.scol, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
         }
         if (split) {
             if (wave > 0)
@@ -361,23 +365,28 @@ __global__ void __launch_bounds__(kThreads) k_pk_init_state(const PackHdr *__res
 }
 
 template <int R>
-void enqueue_pk_iteration(int grid, hipStream_t s, const PackHdr *hdr, int par, bool nt) {
-    if (nt) k_pk_spmv<R, 1><<<grid, kThreads, 0, s>>>(hdr);
-    else k_pk_spmv<R, 0><<<grid, kThreads, 0, s>>>(hdr);
+void enqueue_pk_iteration(int grid, hipStream_t s, const PackHdr *hdr, int par, bool nt, int unr) {
+    if (unr == 1) {
+        if (nt) k_pk_spmv<R, 1, 1><<<grid, kThreads, 0, s>>>(hdr);
+        else k_pk_spmv<R, 0, 1><<<grid, kThreads, 0, s>>>(hdr);
+    } else {
+        if (nt) k_pk_spmv<R, 1, 2><<<grid, kThreads, 0, s>>>(hdr);
+        else k_pk_spmv<R, 0, 2><<<grid, kThreads, 0, s>>>(hdr);
+    }
     k_pk_update<R><<<grid, kThreads, 0, s>>>(hdr, par);
     k_pk_pupdate<R><<<grid, kThreads, 0, s>>>(hdr, par);
 }
 
-void enqueue_iteration(int R, int grid, hipStream_t s, const PackHdr *hdr, int par, bool nt) {
+void enqueue_iteration(int R, int grid, hipStream_t s, const PackHdr *hdr, int par, bool nt, int unr) {
     switch (R) {
-        case 1: enqueue_pk_iteration<1>(grid, s, hdr, par, nt); break;
-        case 2: enqueue_pk_iteration<2>(grid, s, hdr, par, nt); break;
-        case 3: enqueue_pk_iteration<3>(grid, s, hdr, par, nt); break;
-        case 4: enqueue_pk_iteration<4>(grid, s, hdr, par, nt); break;
-        case 5: enqueue_pk_iteration<5>(grid, s, hdr, par, nt); break;
-        case 6: enqueue_pk_iteration<6>(grid, s, hdr, par, nt); break;
-        case 7: enqueue_pk_iteration<7>(grid, s, hdr, par, nt); break;
-        default: enqueue_pk_iteration<8>(grid, s, hdr, par, nt); break;
+        case 1: enqueue_pk_iteration<1>(grid, s, hdr, par, nt, unr); break;
+        case 2: enqueue_pk_iteration<2>(grid, s, hdr, par, nt, unr); break;
+        case 3: enqueue_pk_iteration<3>(grid, s, hdr, par, nt, unr); break;
+        case 4: enqueue_pk_iteration<4>(grid, s, hdr, par, nt, unr); break;
+        case 5: enqueue_pk_iteration<5>(grid, s, hdr, par, nt, unr); break;
+        case 6: enqueue_pk_iteration<6>(grid, s, hdr, par, nt, unr); break;
+        case 7: enqueue_pk_iteration<7>(grid, s, hdr, par, nt, unr); break;
+        default: enqueue_pk_iteration<8>(grid, s, hdr, par, nt, unr); break;
     }
 }
 
@@ -435,7 +444,7 @@ struct PackEngine {
     size_t h_resid_cap = 0;
     double *h_scale = nullptr;      // per system: R scales then R norms
     size_t h_scale_cap = 0;
-    hipGraphExec_t graph[2 * (kMaxR + 1)] = {};
+    hipGraphExec_t graph[4 * (kMaxR + 1)] = {};  // by (R, streaming, unroll)
 };
 
 namespace {
@@ -532,6 +541,10 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
     bool nt = by > kStreamBytes;
     const char *force = getenv("BHIP_PACK_STREAMING");
     if (force && (force[0] == '0' || force[0] == '1')) nt = force[0] == '1';
+    // block columns per SpMV loop trip (BHIP_PACK_UNROLL=1/2 forces it)
+    int unr = kPackUnroll;
+    const char *fu = getenv("BHIP_PACK_UNROLL");
+    if (fu && (fu[0] == '1' || fu[0] == '2')) unr = fu[0] - '0';
     std::vector<int> total(n_all, 0), outer(n_all, 0), active(n_all, 1), ok(n_all, 0);
     for (int i = 0; i < n_all; i++) {
         memset(&stats[i], 0, sizeof(bhip_stats));
@@ -660,11 +673,11 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
         // chunks of kChunk iterations until every case of every system is done
         hipGraphExec_t g = nullptr;
         if (graphs_enabled()) {
-            hipGraphExec_t &slot = pe->graph[2 * R + (nt ? 1 : 0)];
+            hipGraphExec_t &slot = pe->graph[4 * R + (nt ? 2 : 0) + (unr == 1 ? 1 : 0)];
             if (!slot) {
                 hipGraph_t graph;
                 PK_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt);
+                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt, unr);
                 hipError_t e = hipStreamEndCapture(s, &graph);
                 PK_TRY(e);
                 e = hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0);
@@ -692,7 +705,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
             if (g) {
                 PK_TRY(hipGraphLaunch(g, s));
             } else {
-                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt);
+                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt, unr);
             }
             PK_TRY(hipGetLastError());
             for (int k = 0; k < ns; k++) stats[run[k]].spmv_launches += kChunk;
