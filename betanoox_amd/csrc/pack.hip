@@ -419,9 +419,13 @@ bool graphs_enabled() {
 // Per-system item counts (they depend on the system alone).
 int sys_split(const bhip_ctx *c) { return c->n_slices < kSplitSlices ? 1 : 0; }
 int sys_sitems(const bhip_ctx *c) { return sys_split(c) ? c->n_slices : (c->n_slices + 3) / 4; }
-int sys_vitems(const bhip_ctx *c) {
+// Vector items per case: about 2048 DOFs each, at most grid / R so that a
+// system's R x items fit the persistent grid in one round (grid is fixed per
+// device, so the count still depends on the system and R alone).
+int sys_vitems(const bhip_ctx *c, int R, int grid) {
     const long long want = ((long long)c->n_dof + kThreads * 8 - 1) / (kThreads * 8);
-    return (int)std::max<long long>(1, std::min<long long>(want, kRedBlocks));
+    const long long cap = std::max(1, grid / R);
+    return (int)std::max<long long>(1, std::min<long long>(want, cap));
 }
 
 }  // namespace
@@ -623,7 +627,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
             D.split = sys_split(c);
             D.n_sitems = sys_sitems(c);
             D.sitem0 = n_s;
-            D.n_vitems = sys_vitems(c);
+            D.n_vitems = sys_vitems(c, R, pe->grid);
             D.vitem0 = n_v;
             D.maxit = maxit - total[run[k]];
             D.tol = inner_tol;
