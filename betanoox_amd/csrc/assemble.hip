@@ -7,77 +7,26 @@
 // Ke(I,J) into the lower-triangle store for free I >= J, sets K(I,I) = 1 for
 // a restrained I, and lifts F[I] -= Ke(I,J) u_J for free I, restrained J.
 //
-// MI355X design ("owner computes", no colouring, no atomics): one thread per
-// (node n, neighbour m) entry of the node adjacency computes the ndof x ndof
-// block K(n,m) = sum over the elements holding both nodes, in ascending
-// element order (the reference's serial summation order), of the
-// Gauss-point integral, and writes it once into the full symmetric CSR.
-// The pair (m,n) computes the same canonical block (lower node index first)
-// and writes its transpose, so the stored matrix is exactly symmetric.  The
-// lift and element-force vectors are gathered per node by a second kernel
-// that re-integrates only the blocks it needs.
+// MI355X design, two stages, no colouring and no atomics:
+//  1. element stage (k_elem_blocks): a workgroup takes 16 elements, stages
+//     their coordinates, materials and Gauss-point gradients (and J2 tangent
+//     descriptors) in LDS and integrates each element once into its 36
+//     canonical 3x3 blocks (one per unordered pair of local nodes, rows of the
+//     lower global node), written to HBM;
+//  2. gather stage (k_asm_pairs): one thread per (node n, neighbour m) entry
+//     of the node adjacency adds the blocks of the elements holding both
+//     nodes in ascending element order -- the reference's serial summation
+//     order, element matrices first -- and writes the block once into the
+//     full symmetric CSR.  The pair (m,n) reads the same canonical blocks and
+//     writes the transpose, so the stored matrix is exactly symmetric.
+// The lift and element-force vectors (k_gather_rhs) and the lift table read
+// the same element blocks.
+#include <algorithm>
+
 #include "bhip_internal.hpp"
 #include "gauss_tables.hpp"
 
 namespace {
-
-// ------------------------------------------------------------ geometry
-// IsoElement::jacobian, 3-direction branch (isoElement.cpp:324-372);
-// returns detJ and the global gradients of nodes a and b.
-__device__ __forceinline__ double hex8_grads(const double *dN, const double *x, const double *y,
-                                             const double *z, int ip, int a, int b, double *ga, double *gb) {
-    const double *dxi = dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
-    double px1 = 0, px2 = 0, px3 = 0, py1 = 0, py2 = 0, py3 = 0, pz1 = 0, pz2 = 0, pz3 = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        px1 += dxi[i] * x[i];
-        px2 += deta[i] * x[i];
-        px3 += dzeta[i] * x[i];
-        py1 += dxi[i] * y[i];
-        py2 += deta[i] * y[i];
-        py3 += dzeta[i] * y[i];
-        pz1 += dxi[i] * z[i];
-        pz2 += deta[i] * z[i];
-        pz3 += dzeta[i] * z[i];
-    }
-    double d1 = py2 * pz3 - py3 * pz2, d2 = px2 * pz3 - px3 * pz2, d3 = px2 * py3 - px3 * py2;
-    double d4 = py1 * pz3 - py3 * pz1, d5 = px1 * pz3 - px3 * pz1, d6 = px1 * py3 - px3 * py1;
-    double d7 = py1 * pz2 - py2 * pz1, d8 = px1 * pz2 - px2 * pz1, d9 = px1 * py2 - px2 * py1;
-    double detJ = px1 * d1 - py1 * d2 + pz1 * d3;
-    double inv = 1. / detJ;
-    double a11 = d1 * inv, a12 = -d4 * inv, a13 = d7 * inv;
-    double a21 = -d2 * inv, a22 = d5 * inv, a23 = -d8 * inv;
-    double a31 = d3 * inv, a32 = -d6 * inv, a33 = d9 * inv;
-    ga[0] = a11 * dxi[a] + a12 * deta[a] + a13 * dzeta[a];
-    ga[1] = a21 * dxi[a] + a22 * deta[a] + a23 * dzeta[a];
-    ga[2] = a31 * dxi[a] + a32 * deta[a] + a33 * dzeta[a];
-    gb[0] = a11 * dxi[b] + a12 * deta[b] + a13 * dzeta[b];
-    gb[1] = a21 * dxi[b] + a22 * deta[b] + a23 * dzeta[b];
-    gb[2] = a31 * dxi[b] + a32 * deta[b] + a33 * dzeta[b];
-    return detJ;
-}
-
-// 2-direction branch (isoElement.cpp:290-322).
-__device__ __forceinline__ double quad4_grads(const double *dN, const double *x, const double *y, int ip,
-                                              int a, int b, double *ga, double *gb) {
-    const double *dxi = dN + ip * 24, *deta = dxi + 4;
-    double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        J00 += dxi[i] * x[i];
-        J01 += dxi[i] * y[i];
-        J10 += deta[i] * x[i];
-        J11 += deta[i] * y[i];
-    }
-    double detJ = J00 * J11 - J01 * J10;
-    double t = 1 / detJ;
-    double i00 = J11 * t, i01 = -J01 * t, i10 = -J10 * t, i11 = J00 * t;
-    ga[0] = i00 * dxi[a] + i01 * deta[a];
-    ga[1] = i10 * dxi[a] + i11 * deta[a];
-    gb[0] = i00 * dxi[b] + i01 * deta[b];
-    gb[1] = i10 * dxi[b] + i11 * deta[b];
-    return detJ;
-}
 
 struct AsmArgs {
     const double *xyz;
@@ -92,6 +41,8 @@ struct AsmArgs {
     // Gauss tables (ctx->gauss): weights, derivatives [ip][xi|eta|zeta][8]
     // (quad4: [ip][xi|eta][4]), shape values [ip][node]
     const double *gw, *gdN, *gN;
+    // element-stage blocks: element e, local pair pr at (e * NP + pr) * NBS
+    const double *eblk;
 };
 
 // Point the element policies at the Gauss tables passed as a restrict
@@ -109,174 +60,308 @@ __device__ __forceinline__ int group_of(const AsmArgs &A, int e) {
 }
 
 // -------------------------------------------------------------- policies
-// Each policy accumulates the canonical block (lo, hi) of one element into
-// K (row comps of lo, column comps of hi) and, when useful, a second block.
+// The assembly runs in two stages.  The element stage integrates every
+// element once: a workgroup stages its elements' nodal coordinates and
+// material in LDS, then the global shape-function gradients and weights of
+// every Gauss point (and, for J2, the tangent descriptor of the point), then
+// computes each element's canonical blocks -- one per unordered pair of local
+// nodes, oriented with the rows of the lower global node index -- as the sum
+// over Gauss points of B_a^T C B_b detJ w (the reference's per-element Ke,
+// ElasticityElement3D.cpp:560-641), and writes them to HBM.  The gather stage
+// (one thread per node pair) then adds the blocks of the elements holding
+// both nodes in ascending element order, which is the order in which the
+// reference's serial assembler adds its element matrices (SerialAssembler.cpp
+// :35-59, Equations::addMatrix).
+//
+// A policy describes one element family: ND DOFs per node, NN nodes, DIM
+// space dimensions, NB block entries, NBS stored doubles per pair (NB, plus
+// the capacity entry for heat), G doubles of staged Gauss-point data, MS
+// doubles of staged material.
 
-// 3D elasticity, isotropic (E, nu) or full 6x6 C.  Block of B_a^T C B_b with
-// Voigt order xx yy zz xy yz xz and engineering shear
-// (ElasticityElement3D::calculateBmatrix, ElasticityElement3D.cpp:57-87).
+__device__ __forceinline__ int n_pairs_of(int nn) { return nn * (nn + 1) / 2; }
+// packed index of local pair (i, j), i <= j, row by row
+__device__ __forceinline__ int pair_index(int nn, int i, int j) { return i * nn - i * (i - 1) / 2 + (j - i); }
+__device__ __forceinline__ void pair_of_index(int nn, int pr, int &i, int &j) {
+    i = 0;
+    int rem = pr;
+    while (rem >= nn - i) {
+        rem -= nn - i;
+        i++;
+    }
+    j = i + rem;
+}
+
+// 3D elasticity, isotropic (E, nu) or full 6x6 C; J2 adds the plastic
+// correction of the consistent tangent.  Block of B_a^T C B_b with Voigt order
+// xx yy zz xy yz xz and engineering shear (ElasticityElement3D::
+// calculateBmatrix, ElasticityElement3D.cpp:57-87).
 template <int KIND>
 struct ElasHex8 {
-    static constexpr int ND = 3, NN = 8, NB = 9;
-    static __device__ __forceinline__ void accumulate(const AsmArgs &A, int e, const int *ec, int a,
-                                                      int b, bool self, double *K, double *K2) {
-        double x[8], y[8], z[8];
+    static constexpr int ND = 3, NN = 8, DIM = 3, NB = 9, NBS = 9, NP = 36;
+    static constexpr bool J2 = KIND == BHIP_MAT_J2;
+    static constexpr int G = NN * DIM + 1 + (J2 ? 8 : 0);
+    static constexpr int MS = KIND == BHIP_MAT_CMAT3D ? 36 : 2;
+    static __device__ __forceinline__ void stage_mat(const AsmArgs &A, int e, double *m) {
+        const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
+        if (KIND == BHIP_MAT_CMAT3D) {
+#pragma unroll
+            for (int i = 0; i < 36; i++) m[i] = mp[i];
+        } else {
+            const double E = mp[0], nu = mp[1];
+            m[0] = E * nu / ((1 + nu) * (1 - 2 * nu));  // lambda
+            m[1] = E / (2 * (1 + nu));                  // mu
+        }
+    }
+    // IsoElement::jacobian, 3-direction branch (isoElement.cpp:324-372):
+    // gradients of the 8 nodes and f = detJ w factor at Gauss point ip.
+    static __device__ __forceinline__ bool stage_ip(const AsmArgs &A, int e, int ip, const double *xe, double *g) {
+        const double *x = xe, *y = xe + 8, *z = xe + 16;
+        const double *dxi = A.gdN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
+        double px1 = 0, px2 = 0, px3 = 0, py1 = 0, py2 = 0, py3 = 0, pz1 = 0, pz2 = 0, pz3 = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const double *q = A.xyz + 3 * (long long)ec[i];
-            x[i] = q[0];
-            y[i] = q[1];
-            z[i] = q[2];
+            px1 += dxi[i] * x[i];
+            px2 += deta[i] * x[i];
+            px3 += dzeta[i] * x[i];
+            py1 += dxi[i] * y[i];
+            py2 += deta[i] * y[i];
+            py3 += dzeta[i] * y[i];
+            pz1 += dxi[i] * z[i];
+            pz2 += deta[i] * z[i];
+            pz3 += dzeta[i] * z[i];
         }
-        const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
-        double lam = 0, mu = 0;
-        double C[36];
-        if (KIND == BHIP_MAT_ISO_ELASTIC || KIND == BHIP_MAT_J2) {
-            double E = mp[0], nu = mp[1];
-            lam = E * nu / ((1 + nu) * (1 - 2 * nu));
-            mu = E / (2 * (1 + nu));
-        } else {
+        double d1 = py2 * pz3 - py3 * pz2, d2 = px2 * pz3 - px3 * pz2, d3 = px2 * py3 - px3 * py2;
+        double d4 = py1 * pz3 - py3 * pz1, d5 = px1 * pz3 - px3 * pz1, d6 = px1 * py3 - px3 * py1;
+        double d7 = py1 * pz2 - py2 * pz1, d8 = px1 * pz2 - px2 * pz1, d9 = px1 * py2 - px2 * py1;
+        double detJ = px1 * d1 - py1 * d2 + pz1 * d3;
+        double inv = 1. / detJ;
+        double a11 = d1 * inv, a12 = -d4 * inv, a13 = d7 * inv;
+        double a21 = -d2 * inv, a22 = d5 * inv, a23 = -d8 * inv;
+        double a31 = d3 * inv, a32 = -d6 * inv, a33 = d9 * inv;
 #pragma unroll
-            for (int i = 0; i < 36; i++) C[i] = mp[i];
+        for (int i = 0; i < 8; i++) {
+            g[3 * i] = a11 * dxi[i] + a12 * deta[i] + a13 * dzeta[i];
+            g[3 * i + 1] = a21 * dxi[i] + a22 * deta[i] + a23 * dzeta[i];
+            g[3 * i + 2] = a31 * dxi[i] + a32 * deta[i] + a33 * dzeta[i];
         }
-        int bad = 0;
-        double Ke[9];
-        if (self)
+        g[24] = detJ * A.gw[ip] * A.factor;
+        if (J2) {
+            const double *t = A.tan + ((long long)e * A.nip + ip) * 8;
 #pragma unroll
-            for (int i = 0; i < 9; i++) Ke[i] = 0;
-        for (int ip = 0; ip < A.nip; ip++) {
-            double ga[3], gb[3];
-            double detJ = hex8_grads(A.gdN, x, y, z, ip, a, b, ga, gb);
-            if (detJ < 0) bad = 1;
-            double f = detJ * A.gw[ip] * A.factor;
-            double blk[9];
-            if (KIND == BHIP_MAT_ISO_ELASTIC || KIND == BHIP_MAT_J2) {
-                double gg = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+            for (int i = 0; i < 8; i++) g[25 + i] = t[i];
+        }
+        return detJ < 0;
+    }
+    // K += (B_a^T C B_b)(ip) f
+    static __device__ __forceinline__ void add_ip(const double *m, const double *g, int a, int b, double *K,
+                                                  double *) {
+        const double *ga = g + 3 * a, *gb = g + 3 * b;
+        const double f = g[24];
+        double blk[9];
+        if (KIND == BHIP_MAT_ISO_ELASTIC || J2) {
+            const double lam = m[0], mu = m[1];
+            double gg = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
 #pragma unroll
-                for (int c = 0; c < 3; c++)
+            for (int c = 0; c < 3; c++)
 #pragma unroll
-                    for (int d = 0; d < 3; d++)
-                        blk[c * 3 + d] = lam * ga[c] * gb[d] + mu * gb[c] * ga[d] + (c == d ? gg : 0.0);
-                if (KIND == BHIP_MAT_J2) {
-                    const double *t = A.tan + ((long long)e * A.nip + ip) * 8;
-                    if (t[7] != 0.0) {
-                        double sa[3] = {ga[0] * t[0] + ga[1] * t[3] + ga[2] * t[5],
-                                        ga[1] * t[1] + ga[0] * t[3] + ga[2] * t[4],
-                                        ga[2] * t[2] + ga[1] * t[4] + ga[0] * t[5]};
-                        double sb[3] = {gb[0] * t[0] + gb[1] * t[3] + gb[2] * t[5],
-                                        gb[1] * t[1] + gb[0] * t[3] + gb[2] * t[4],
-                                        gb[2] * t[2] + gb[1] * t[4] + gb[0] * t[5]};
+                for (int d = 0; d < 3; d++) blk[c * 3 + d] = lam * ga[c] * gb[d] + mu * gb[c] * ga[d] + (c == d ? gg : 0.0);
+            if (J2) {
+                const double *t = g + 25;
+                if (t[7] != 0.0) {
+                    double sa[3] = {ga[0] * t[0] + ga[1] * t[3] + ga[2] * t[5], ga[1] * t[1] + ga[0] * t[3] + ga[2] * t[4],
+                                    ga[2] * t[2] + ga[1] * t[4] + ga[0] * t[5]};
+                    double sb[3] = {gb[0] * t[0] + gb[1] * t[3] + gb[2] * t[5], gb[1] * t[1] + gb[0] * t[3] + gb[2] * t[4],
+                                    gb[2] * t[2] + gb[1] * t[4] + gb[0] * t[5]};
 #pragma unroll
-                        for (int c = 0; c < 3; c++)
+                    for (int c = 0; c < 3; c++)
 #pragma unroll
-                            for (int d = 0; d < 3; d++) blk[c * 3 + d] -= sa[c] * sb[d] * t[6];
-                    }
-                }
-            } else {
-                // CB = C * B_b (6x3), then B_a^T * CB
-                double CB[6][3];
-#pragma unroll
-                for (int s = 0; s < 6; s++) {
-                    CB[s][0] = gb[0] * C[s * 6 + 0] + gb[1] * C[s * 6 + 3] + gb[2] * C[s * 6 + 5];
-                    CB[s][1] = gb[1] * C[s * 6 + 1] + gb[0] * C[s * 6 + 3] + gb[2] * C[s * 6 + 4];
-                    CB[s][2] = gb[2] * C[s * 6 + 2] + gb[1] * C[s * 6 + 4] + gb[0] * C[s * 6 + 5];
-                }
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    blk[0 * 3 + d] = ga[0] * CB[0][d] + ga[1] * CB[3][d] + ga[2] * CB[5][d];
-                    blk[1 * 3 + d] = ga[1] * CB[1][d] + ga[0] * CB[3][d] + ga[2] * CB[4][d];
-                    blk[2 * 3 + d] = ga[2] * CB[2][d] + ga[1] * CB[4][d] + ga[0] * CB[5][d];
+                        for (int d = 0; d < 3; d++) blk[c * 3 + d] -= sa[c] * sb[d] * t[6];
                 }
             }
+        } else {
+            // CB = C * B_b (6x3), then B_a^T * CB
+            const double *C = m;
+            double CB[6][3];
 #pragma unroll
-            for (int i = 0; i < 9; i++) K[i] += blk[i] * f;
-            if (self)
+            for (int s = 0; s < 6; s++) {
+                CB[s][0] = gb[0] * C[s * 6 + 0] + gb[1] * C[s * 6 + 3] + gb[2] * C[s * 6 + 5];
+                CB[s][1] = gb[1] * C[s * 6 + 1] + gb[0] * C[s * 6 + 3] + gb[2] * C[s * 6 + 4];
+                CB[s][2] = gb[2] * C[s * 6 + 2] + gb[1] * C[s * 6 + 4] + gb[0] * C[s * 6 + 5];
+            }
 #pragma unroll
-                for (int i = 0; i < 9; i++) Ke[i] += blk[i] * f;
+            for (int d = 0; d < 3; d++) {
+                blk[0 * 3 + d] = ga[0] * CB[0][d] + ga[1] * CB[3][d] + ga[2] * CB[5][d];
+                blk[1 * 3 + d] = ga[1] * CB[1][d] + ga[0] * CB[3][d] + ga[2] * CB[4][d];
+                blk[2 * 3 + d] = ga[2] * CB[2][d] + ga[1] * CB[4][d] + ga[0] * CB[5][d];
+            }
         }
-        if (bad && A.flags) atomicOr(&A.flags[0], 1);
-        // BasicElement::checkForNegativeDiagonal (BasicElement.cpp:116-130)
-        if (self && KIND != BHIP_MAT_J2 && A.flags)
-            if (Ke[0] <= 0 || Ke[4] <= 0 || Ke[8] <= 0) atomicOr(&A.flags[0], 2);
+#pragma unroll
+        for (int i = 0; i < 9; i++) K[i] += blk[i] * f;
+    }
+    // BasicElement::checkForNegativeDiagonal (BasicElement.cpp:116-130); the
+    // reference checks the elastic stiffness only
+    static __device__ __forceinline__ bool bad_diag(const double *K) {
+        return !J2 && (K[0] <= 0 || K[4] <= 0 || K[8] <= 0);
+    }
+    static __device__ __forceinline__ void load(const double *s, double *K, double *) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) K[i] += s[i];
     }
 };
 
 // 2D plane stress quad4 (ElasticityElement2D.cpp:22-119, pStress.cpp:68-85).
 struct ElasQuad4 {
-    static constexpr int ND = 2, NN = 4, NB = 4;
-    static __device__ __forceinline__ void accumulate(const AsmArgs &A, int e, const int *ec, int a,
-                                                      int b, bool self, double *K, double *K2) {
-        double x[4], y[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            x[i] = A.xyz[2 * (long long)ec[i]];
-            y[i] = A.xyz[2 * (long long)ec[i] + 1];
-        }
+    static constexpr int ND = 2, NN = 4, DIM = 2, NB = 4, NBS = 4, NP = 10;
+    static constexpr int G = NN * DIM + 1, MS = 9;
+    static __device__ __forceinline__ void stage_mat(const AsmArgs &A, int e, double *m) {
         const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
         double s11 = 1 / mp[0], s12 = -mp[2] / mp[0], s22 = 1. / mp[1], s66 = 1. / mp[3];
         double den = s11 * s22 - s12 * s12;
-        double C[9] = {s22 / den, -s12 / den, 0., -s12 / den, s11 / den, 0., 0., 0., 1. / s66};
-        int bad = 0;
-        double Ke[4] = {0, 0, 0, 0};
-        for (int ip = 0; ip < A.nip; ip++) {
-            double ga[2], gb[2];
-            double detJ = quad4_grads(A.gdN, x, y, ip, a, b, ga, gb);
-            if (detJ < 0) bad = 1;
-            double f = detJ * A.gw[ip] * A.factor;
-            double CB[3][2];
+        m[0] = s22 / den;
+        m[1] = -s12 / den;
+        m[2] = 0.;
+        m[3] = -s12 / den;
+        m[4] = s11 / den;
+        m[5] = 0.;
+        m[6] = 0.;
+        m[7] = 0.;
+        m[8] = 1. / s66;
+    }
+    // 2-direction branch of IsoElement::jacobian (isoElement.cpp:290-322).
+    static __device__ __forceinline__ bool stage_ip(const AsmArgs &A, int, int ip, const double *xe, double *g) {
+        const double *x = xe, *y = xe + 4;
+        const double *dxi = A.gdN + ip * 24, *deta = dxi + 4;
+        double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
 #pragma unroll
-            for (int s = 0; s < 3; s++) {
-                CB[s][0] = gb[0] * C[s * 3 + 0] + gb[1] * C[s * 3 + 2];
-                CB[s][1] = gb[1] * C[s * 3 + 1] + gb[0] * C[s * 3 + 2];
-            }
-            double blk[4];
-#pragma unroll
-            for (int d = 0; d < 2; d++) {
-                blk[0 * 2 + d] = ga[0] * CB[0][d] + ga[1] * CB[2][d];
-                blk[1 * 2 + d] = ga[1] * CB[1][d] + ga[0] * CB[2][d];
-            }
-#pragma unroll
-            for (int i = 0; i < 4; i++) K[i] += blk[i] * f;
-            if (self)
-#pragma unroll
-                for (int i = 0; i < 4; i++) Ke[i] += blk[i] * f;
+        for (int i = 0; i < 4; i++) {
+            J00 += dxi[i] * x[i];
+            J01 += dxi[i] * y[i];
+            J10 += deta[i] * x[i];
+            J11 += deta[i] * y[i];
         }
-        if (bad && A.flags) atomicOr(&A.flags[0], 1);
-        if (self && A.flags && (Ke[0] <= 0 || Ke[3] <= 0)) atomicOr(&A.flags[0], 2);
+        double detJ = J00 * J11 - J01 * J10;
+        double t = 1 / detJ;
+        double i00 = J11 * t, i01 = -J01 * t, i10 = -J10 * t, i11 = J00 * t;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            g[2 * i] = i00 * dxi[i] + i01 * deta[i];
+            g[2 * i + 1] = i10 * dxi[i] + i11 * deta[i];
+        }
+        g[8] = detJ * A.gw[ip] * A.factor;
+        return detJ < 0;
+    }
+    static __device__ __forceinline__ void add_ip(const double *C, const double *g, int a, int b, double *K, double *) {
+        const double *ga = g + 2 * a, *gb = g + 2 * b;
+        const double f = g[8];
+        double CB[3][2];
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            CB[s][0] = gb[0] * C[s * 3 + 0] + gb[1] * C[s * 3 + 2];
+            CB[s][1] = gb[1] * C[s * 3 + 1] + gb[0] * C[s * 3 + 2];
+        }
+        double blk[4];
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            blk[0 * 2 + d] = ga[0] * CB[0][d] + ga[1] * CB[2][d];
+            blk[1 * 2 + d] = ga[1] * CB[1][d] + ga[0] * CB[2][d];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) K[i] += blk[i] * f;
+    }
+    static __device__ __forceinline__ bool bad_diag(const double *K) { return K[0] <= 0 || K[3] <= 0; }
+    static __device__ __forceinline__ void load(const double *s, double *K, double *) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) K[i] += s[i];
     }
 };
 
-// Transient heat hex8: K (HeatTransferElement3DTransient.cpp:153-197) into
-// K, consistent capacity M (60-77) into K2.
+// Transient heat hex8: K (HeatTransferElement3DTransient.cpp:153-197) and
+// the consistent capacity M (60-77); stored as (K, M) per pair.
 struct HeatHex8 {
-    static constexpr int ND = 1, NN = 8, NB = 1;
-    static __device__ __forceinline__ void accumulate(const AsmArgs &A, int e, const int *ec, int a,
-                                                      int b, bool self, double *K, double *K2) {
-        double x[8], y[8], z[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const double *q = A.xyz + 3 * (long long)ec[i];
-            x[i] = q[0];
-            y[i] = q[1];
-            z[i] = q[2];
-        }
+    static constexpr int ND = 1, NN = 8, DIM = 3, NB = 1, NBS = 2, NP = 36;
+    static constexpr int G = NN * DIM + 1 + NN, MS = 4;  // gradients, f, shape values
+    static __device__ __forceinline__ void stage_mat(const AsmArgs &A, int e, double *m) {
         const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
-        double k11 = mp[0], k22 = mp[1], k33 = mp[2], cap = mp[3];
-        int bad = 0;
-        for (int ip = 0; ip < A.nip; ip++) {
-            double ga[3], gb[3];
-            double detJ = hex8_grads(A.gdN, x, y, z, ip, a, b, ga, gb);
-            if (detJ < 0) bad = 1;
-            double f = detJ * A.gw[ip] * A.factor;
-            K[0] += (gb[0] * k11 * ga[0] + gb[1] * k22 * ga[1] + gb[2] * k33 * ga[2]) * f;
-            K2[0] += A.gN[ip * 8 + a] * A.gN[ip * 8 + b] * f * cap;
-        }
-        if (bad && A.flags) atomicOr(&A.flags[0], 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) m[i] = mp[i];  // k11 k22 k33 capacity
+    }
+    static __device__ __forceinline__ bool stage_ip(const AsmArgs &A, int e, int ip, const double *xe, double *g) {
+        const bool bad = ElasHex8<BHIP_MAT_ISO_ELASTIC>::stage_ip(A, e, ip, xe, g);
+#pragma unroll
+        for (int i = 0; i < 8; i++) g[25 + i] = A.gN[ip * 8 + i];
+        return bad;
+    }
+    static __device__ __forceinline__ void add_ip(const double *m, const double *g, int a, int b, double *K,
+                                                  double *K2) {
+        const double *ga = g + 3 * a, *gb = g + 3 * b;
+        const double f = g[24];
+        K[0] += (gb[0] * m[0] * ga[0] + gb[1] * m[1] * ga[1] + gb[2] * m[2] * ga[2]) * f;
+        K2[0] += g[25 + a] * g[25 + b] * f * m[3];
+    }
+    static __device__ __forceinline__ bool bad_diag(const double *) { return false; }
+    static __device__ __forceinline__ void load(const double *s, double *K, double *K2) {
+        K[0] += s[0];
+        K2[0] += s[1];
     }
 };
+
+// Element stage: workgroup of 256 threads, epw elements.  LDS: coordinates
+// (epw x DIM x NN), material (epw x MS), Gauss-point data (epw x nip x G).
+template <class P>
+__global__ void __launch_bounds__(256) k_elem_blocks(AsmArgs A, const double *__restrict__ gtab, int epw,
+                                                     double *eblk) {
+    bind_tables(A, gtab);
+    extern __shared__ double sh[];
+    const int nip = A.nip;
+    double *xs = sh;
+    double *ms = xs + epw * P::DIM * P::NN;
+    double *gs = ms + epw * P::MS;
+    const long long e0 = (long long)blockIdx.x * epw;
+    for (int t = threadIdx.x; t < epw * P::NN; t += blockDim.x) {
+        const int el = t / P::NN, i = t - el * P::NN;
+        const long long e = e0 + el;
+        if (e < A.n_el) {
+            const long long node = A.conn[e * P::NN + i];
+#pragma unroll
+            for (int d = 0; d < P::DIM; d++) xs[(el * P::DIM + d) * P::NN + i] = A.xyz[P::DIM * node + d];
+        }
+    }
+    for (int el = threadIdx.x; el < epw; el += blockDim.x)
+        if (e0 + el < A.n_el) P::stage_mat(A, (int)(e0 + el), ms + el * P::MS);
+    __syncthreads();
+    int bad = 0;
+    for (int t = threadIdx.x; t < epw * nip; t += blockDim.x) {
+        const int el = t / nip, ip = t - el * nip;
+        const long long e = e0 + el;
+        if (e < A.n_el)
+            bad |= P::stage_ip(A, (int)e, ip, xs + el * P::DIM * P::NN, gs + (long long)(el * nip + ip) * P::G);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < epw * P::NP; t += blockDim.x) {
+        const int el = t / P::NP, pr = t - el * P::NP;
+        const long long e = e0 + el;
+        if (e >= A.n_el) continue;
+        int i, j;
+        pair_of_index(P::NN, pr, i, j);
+        const int ni = A.conn[e * P::NN + i], nj = A.conn[e * P::NN + j];
+        const int a = ni <= nj ? i : j, b = ni <= nj ? j : i;
+        double K[P::NB], K2[P::NB];
+#pragma unroll
+        for (int q = 0; q < P::NB; q++) K[q] = K2[q] = 0;
+        const double *m = ms + el * P::MS;
+        for (int ip = 0; ip < nip; ip++) P::add_ip(m, gs + (long long)(el * nip + ip) * P::G, a, b, K, K2);
+        double *out = eblk + ((long long)e * P::NP + pr) * P::NBS;
+#pragma unroll
+        for (int q = 0; q < P::NB; q++) out[q] = K[q];
+        if (P::NBS > P::NB) out[P::NB] = K2[0];
+        if (i == j && P::bad_diag(K)) bad |= 2;
+    }
+    if (bad && A.flags) atomicOr(&A.flags[0], bad);
+}
 
 // Canonical block of pair (lo, hi) summed over the elements of `n` (which is
-// lo or hi) that also hold the other node, in ascending element order.
+// lo or hi) that also hold the other node, in ascending element order, from
+// the element stage's blocks.
 template <class P>
 __device__ __forceinline__ void pair_block(const AsmArgs &A, const int *n2e_ptr, const int *n2e,
                                            int n, int lo, int hi, double *K, double *K2) {
@@ -285,16 +370,16 @@ __device__ __forceinline__ void pair_block(const AsmArgs &A, const int *n2e_ptr,
     const bool self = (lo == hi);
     for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
         int e = n2e[k];
-        int ec[P::NN];
         int a = -1, b = -1;
 #pragma unroll
         for (int i = 0; i < P::NN; i++) {
-            ec[i] = A.conn[(long long)e * P::NN + i];
-            if (ec[i] == lo) a = i;
-            if (ec[i] == hi) b = i;
+            const int c = A.conn[(long long)e * P::NN + i];
+            if (c == lo) a = i;
+            if (c == hi) b = i;
         }
         if (a < 0 || b < 0) continue;
-        P::accumulate(A, e, ec, a, b, self, K, K2);
+        const int pr = a <= b ? pair_index(P::NN, a, b) : pair_index(P::NN, b, a);
+        P::load(A.eblk + ((long long)e * P::NP + pr) * P::NBS, K, K2);
     }
     if (self) {  // keep the upper triangle of a diagonal block (Ke is upper-only)
 #pragma unroll
@@ -417,10 +502,36 @@ int upload_gauss(bhip_ctx *ctx, int nnpe, int order, int *nip_out) {
     return bhip_gauss_upload(ctx, nnpe, order, nip_out);
 }
 
+// Elements per workgroup of the element stage and its LDS bytes.
+template <class P>
+int elem_stage_epw(int nip) {
+    return std::max(1, std::min(16, 256 / std::max(nip, 1)));
+}
+template <class P>
+size_t elem_stage_lds(int epw, int nip) {
+    return sizeof(double) * ((size_t)epw * P::DIM * P::NN + (size_t)epw * P::MS + (size_t)epw * nip * P::G);
+}
+
+// Element stage into ctx->eblk (kept until the next assembly: the lift table
+// is built from it on first use).
+template <class P>
+int element_stage(bhip_ctx *ctx, AsmArgs &A) {
+    BHIP_TRY(ctx, ctx->eblk.resize((size_t)ctx->n_el * P::NP * P::NBS > 0 ? (size_t)ctx->n_el * P::NP * P::NBS : 1));
+    const int epw = elem_stage_epw<P>(A.nip);
+    const size_t lds = elem_stage_lds<P>(epw, A.nip);
+    if (lds > 64 * 1024) return bhip_fail(ctx, BHIP_ERR_CAPACITY, "integration order too high for the element stage");
+    const int grid = grid_for(ctx->n_el, epw);
+    k_elem_blocks<P><<<grid, 256, lds, ctx->stream>>>(A, ctx->gauss.p, epw, ctx->eblk.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    A.eblk = ctx->eblk.p;
+    return BHIP_OK;
+}
+
 template <class P>
 int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, double fcoef) {
     const int B = 256;
     hipStream_t s = ctx->stream;
+    BHIP_PROPAGATE(element_stage<P>(ctx, A));
     k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p);
     BHIP_TRY(ctx, hipGetLastError());
     k_restrained_diag<<<grid_for(ctx->n_dof, B), B, 0, s>>>(ctx->n_dof, ctx->rest.p, ctx->rowptr.p,
@@ -463,6 +574,7 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     A.factor = ctx->factor;
     A.tan = nullptr;
     A.flags = ctx->flags.p;
+    A.eblk = nullptr;
     AsmOut O;
     O.pair_row = ctx->pair_row.p;
     O.nbr = ctx->nbr.p;
@@ -705,6 +817,7 @@ int bhip_lift_build(bhip_ctx *ctx) {
         A.factor = ctx->factor;
         A.tan = nullptr;
         A.flags = nullptr;
+        A.eblk = ctx->eblk.p;  // element blocks of the assembly this table belongs to
         const int g = grid_for(total, B);
         if (heat)
             k_lift_blocks_heat<<<g, B, 0, s>>>(A, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,

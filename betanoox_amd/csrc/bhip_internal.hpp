@@ -114,6 +114,7 @@ struct bhip_ctx {
     dvec<int> rowptr, colind, diag_pos;          // full symmetric CSR
     dvec<double> val;                            // FP64 values
     dvec<double> val_aux;                        // second value array (heat: K part)
+    dvec<double> eblk;                           // element-stage blocks of the last assembly
     // ---- FP32 mirror for the inner solve
     bool f32_ready = false;
     bool use_bsr = false;    // FP32 inner solve on SELL-64 node blocks

@@ -84,6 +84,7 @@ void bhip_destroy(bhip_ctx *ctx) {
     dvec<float> *v32[] = {&ctx->val32, &ctx->bval32, &ctx->r32, &ctx->p32, &ctx->q32, &ctx->d32, &ctx->dinv32};
     for (auto *v : v32) v->release();
     ctx->sell_col16.release();
+    ctx->eblk.release();
     ctx->work.release();
     ctx->tmp_i.release();
     ctx->tmp_k1.release();

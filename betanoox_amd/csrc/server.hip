@@ -109,11 +109,12 @@ namespace {
 // Device bytes of one hex8 request with 3 DOFs per node and `cases` load
 // cases, from the buffers the context allocates (per node: node graph and
 // incidence ~360 B, FP64 CSR ~2.9 KB, FP32 SELL-64 ~1.1 KB, solver vectors
-// ~0.3 KB, cases 120 B each; per element: ~1 KB of sort keys while the
-// pattern is built) plus 64 MB for tables, partials and allocator rounding.
+// ~0.3 KB, cases 120 B each; per element: 1 KB of pattern sort keys and
+// 2.6 KB of element-stage blocks) plus 64 MB for tables, partials and
+// allocator rounding.
 double estimate_bytes(long long n_nodes, long long n_el, int cases) {
     const int r = std::min(cases, 8);
-    return (double)n_nodes * (4700.0 + 120.0 * r) + 1024.0 * (double)n_el + 64.0 * (1 << 20);
+    return (double)n_nodes * (4700.0 + 120.0 * r) + 3648.0 * (double)n_el + 64.0 * (1 << 20);
 }
 
 bool topo_same(const Seat &s, const Request &r) {
