@@ -43,6 +43,8 @@ struct AsmArgs {
     const double *gw, *gdN, *gN;
     // element-stage blocks: element e, local pair pr at (e * NP + pr) * NBS
     const double *eblk;
+    // local index of each (node, incident element) entry of the incidence lists
+    const unsigned char *n2e_loc;
 };
 
 // Point the element policies at the Gauss tables passed as a restrict
@@ -359,27 +361,57 @@ __global__ void __launch_bounds__(256) k_elem_blocks(AsmArgs A, const double *__
     if (bad && A.flags) atomicOr(&A.flags[0], bad);
 }
 
-// Canonical block of pair (lo, hi) summed over the elements of `n` (which is
-// lo or hi) that also hold the other node, in ascending element order, from
-// the element stage's blocks.
+// Canonical block of pair (lo, hi) summed over the elements holding both
+// nodes, in ascending element order, from the element stage's blocks: the
+// two nodes' sorted incidence lists are intersected, and their local indices
+// come with them.  Lists of up to kShortList entries (every node of a
+// structured hex8 or quad4 mesh) are loaded at once and intersected in
+// registers; longer ones are merged entry by entry.
+constexpr int kShortList = 8;
 template <class P>
 __device__ __forceinline__ void pair_block(const AsmArgs &A, const int *n2e_ptr, const int *n2e,
-                                           int n, int lo, int hi, double *K, double *K2) {
+                                           int, int lo, int hi, double *K, double *K2) {
 #pragma unroll
     for (int i = 0; i < P::NB; i++) K[i] = K2[i] = 0;
     const bool self = (lo == hi);
-    for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
-        int e = n2e[k];
-        int a = -1, b = -1;
+    int i = n2e_ptr[lo], j = n2e_ptr[hi];
+    const int ie = n2e_ptr[lo + 1], je = n2e_ptr[hi + 1];
+    if (ie - i <= kShortList && je - j <= kShortList) {
+        int el[kShortList], eh[kShortList], la[kShortList], lh[kShortList];
 #pragma unroll
-        for (int i = 0; i < P::NN; i++) {
-            const int c = A.conn[(long long)e * P::NN + i];
-            if (c == lo) a = i;
-            if (c == hi) b = i;
+        for (int k = 0; k < kShortList; k++) {
+            el[k] = i + k < ie ? n2e[i + k] : -1;
+            la[k] = i + k < ie ? A.n2e_loc[i + k] : 0;
+            eh[k] = j + k < je ? n2e[j + k] : -2;
+            lh[k] = j + k < je ? A.n2e_loc[j + k] : 0;
         }
-        if (a < 0 || b < 0) continue;
-        const int pr = a <= b ? pair_index(P::NN, a, b) : pair_index(P::NN, b, a);
-        P::load(A.eblk + ((long long)e * P::NP + pr) * P::NBS, K, K2);
+#pragma unroll
+        for (int a = 0; a < kShortList; a++) {
+            int b = -1;
+#pragma unroll
+            for (int c = 0; c < kShortList; c++)
+                if (eh[c] == el[a]) b = lh[c];
+            if (b >= 0) {
+                const int la_ = la[a];
+                const int pr = la_ <= b ? pair_index(P::NN, la_, b) : pair_index(P::NN, b, la_);
+                P::load(A.eblk + ((long long)el[a] * P::NP + pr) * P::NBS, K, K2);
+            }
+        }
+        i = ie;  // done
+    }
+    while (i < ie && j < je) {
+        const int el = n2e[i], eh = n2e[j];
+        if (el < eh) {
+            i++;
+        } else if (eh < el) {
+            j++;
+        } else {
+            const int a = A.n2e_loc[i], b = A.n2e_loc[j];
+            const int pr = a <= b ? pair_index(P::NN, a, b) : pair_index(P::NN, b, a);
+            P::load(A.eblk + ((long long)el * P::NP + pr) * P::NBS, K, K2);
+            i++;
+            j++;
+        }
     }
     if (self) {  // keep the upper triangle of a diagonal block (Ke is upper-only)
 #pragma unroll
@@ -575,6 +607,7 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     A.tan = nullptr;
     A.flags = ctx->flags.p;
     A.eblk = nullptr;
+    A.n2e_loc = ctx->n2e_loc.p;
     AsmOut O;
     O.pair_row = ctx->pair_row.p;
     O.nbr = ctx->nbr.p;
@@ -818,6 +851,7 @@ int bhip_lift_build(bhip_ctx *ctx) {
         A.tan = nullptr;
         A.flags = nullptr;
         A.eblk = ctx->eblk.p;  // element blocks of the assembly this table belongs to
+        A.n2e_loc = ctx->n2e_loc.p;
         const int g = grid_for(total, B);
         if (heat)
             k_lift_blocks_heat<<<g, B, 0, s>>>(A, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, total, ctx->lift_row.p,

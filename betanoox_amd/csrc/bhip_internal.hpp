@@ -110,6 +110,7 @@ struct bhip_ctx {
     bool pattern_ready = false;
     long long n_pairs = 0, nnz = 0, nnz_lower = 0;
     dvec<int> n2e_ptr, n2e;         // node -> incident elements (ascending)
+    dvec<unsigned char> n2e_loc;    // the node's local index in each of those elements
     dvec<int> nbr_ptr, nbr, pair_row, nbr_off;  // node adjacency incl. self
     dvec<int> rowptr, colind, diag_pos;          // full symmetric CSR
     dvec<double> val;                            // FP64 values

@@ -85,6 +85,7 @@ void bhip_destroy(bhip_ctx *ctx) {
     for (auto *v : v32) v->release();
     ctx->sell_col16.release();
     ctx->eblk.release();
+    ctx->n2e_loc.release();
     ctx->work.release();
     ctx->tmp_i.release();
     ctx->tmp_k1.release();

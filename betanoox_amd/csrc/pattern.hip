@@ -28,27 +28,32 @@ __global__ void k_count_incidence(const int *conn, long long total, int *cnt) {
 }
 
 __global__ void k_fill_incidence(const int *conn, long long total, int nnpe, const int *ptr,
-                                 int *cursor, int *n2e) {
+                                 int *cursor, int *n2e, unsigned char *n2e_loc) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < total) {
         int n = conn[t];
         int pos = atomicAdd(&cursor[n], 1);
         n2e[ptr[n] + pos] = (int)(t / nnpe);
+        n2e_loc[ptr[n] + pos] = (unsigned char)(t % nnpe);  // the node's local index in the element
     }
 }
 
-// Sort each node's incident-element list ascending (lists are short).
-__global__ void k_sort_incidence(int n_nodes, const int *ptr, int *n2e) {
+// Sort each node's incident-element list ascending (lists are short), local
+// indices alongside.
+__global__ void k_sort_incidence(int n_nodes, const int *ptr, int *n2e, unsigned char *n2e_loc) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     int b = ptr[n], e = ptr[n + 1];
     for (int i = b + 1; i < e; i++) {
         int v = n2e[i], j = i - 1;
+        unsigned char l = n2e_loc[i];
         while (j >= b && n2e[j] > v) {
             n2e[j + 1] = n2e[j];
+            n2e_loc[j + 1] = n2e_loc[j];
             j--;
         }
         n2e[j + 1] = v;
+        n2e_loc[j + 1] = l;
     }
 }
 
@@ -209,10 +214,11 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     BHIP_TRY(ctx, ctx->n2e_ptr.resize(nn + 1));
     BHIP_PROPAGATE(exclusive_scan(ctx, cnt.p, ctx->n2e_ptr.p, nn + 1));
     BHIP_TRY(ctx, ctx->n2e.resize(total > 0 ? total : 1));
+    BHIP_TRY(ctx, ctx->n2e_loc.resize(total > 0 ? total : 1));
     BHIP_TRY(ctx, hipMemsetAsync(cnt.p, 0, sizeof(int) * (nn + 1), s));
     k_fill_incidence<<<grid_for(total, B), B, 0, s>>>(ctx->conn.p, total, ctx->nnpe,
-                                                      ctx->n2e_ptr.p, cnt.p, ctx->n2e.p);
-    k_sort_incidence<<<grid_for(nn, B), B, 0, s>>>(nn, ctx->n2e_ptr.p, ctx->n2e.p);
+                                                      ctx->n2e_ptr.p, cnt.p, ctx->n2e.p, ctx->n2e_loc.p);
+    k_sort_incidence<<<grid_for(nn, B), B, 0, s>>>(nn, ctx->n2e_ptr.p, ctx->n2e.p, ctx->n2e_loc.p);
     BHIP_TRY(ctx, hipGetLastError());
 
     // ---- node adjacency: sort + unique the (node_a, node_b) keys
