@@ -128,3 +128,35 @@ def test_failing_request_reported(hip_ctx_factory):
         X2, _ = srv.wait(srv.submit(good))
     assert np.array_equal(X, _direct(hip_ctx_factory, good)[0])
     assert np.array_equal(X2, X)
+
+
+def test_other_case_counts(hip_ctx_factory):
+    """Requests with 3 cases pack together; a request with 10 cases (more than
+    one case group) is solved alone in groups of 8 + 2; both equal their
+    private solves bit for bit."""
+    small = [meshgen.rve_request(n, 700 + n) for n in (6, 8, 7)]
+    cases3 = [np.stack(p.extra["cases"])[:3] for p in small]
+    big = meshgen.rve_request(7, 777)
+    c6 = np.stack(big.extra["cases"])
+    cases10 = np.concatenate([c6, 0.5 * c6[:4]])
+    o = capi.Context.opts(tol=TOL, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR, inner_tol=1e-3,
+                          maxit=40000, max_outer=0, check_every=0)
+    ref3 = []
+    for p, c in zip(small, cases3):
+        ctx = setup_problem(hip_ctx_factory(), p)
+        ctx.assemble(capi.ASM_STIFFNESS)
+        ref3.append(ctx.solve_cases(c, opts=o)[0])
+    ctx = setup_problem(hip_ctx_factory(), big)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    ref10 = np.concatenate([ctx.solve_cases(cases10[:8], opts=o)[0], ctx.solve_cases(cases10[8:], opts=o)[0]])
+    with capi.Server(0, max_slots=1, max_pack=8) as srv:
+        ib = srv.submit(big, cases=cases10)
+        ids = [srv.submit(p, cases=c) for p, c in zip(small, cases3)]
+        Xb, sb = srv.wait(ib)
+        out = [srv.wait(i) for i in ids]
+    assert sb["status"] == capi.OK and sb["pack_size"] == 1 and sb["rel_resid"] <= TOL
+    assert np.array_equal(Xb, ref10)
+    for (X, s), Xr in zip(out, ref3):
+        assert s["status"] == capi.OK and s["rel_resid"] <= TOL
+        assert np.array_equal(X, Xr)
+    assert max(s["pack_size"] for _, s in out) > 1
