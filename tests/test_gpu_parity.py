@@ -237,3 +237,40 @@ def test_error_paths(hip_ctx_factory):
     # load on a restrained DOF (addLoadToEqn, equation.cpp:862)
     with pytest.raises(capi.BhipError):
         ctx.set_load(r.astype(float))
+
+
+def _fan_quads(k=10):
+    """A quad4 fan: k kites around a centre node (k incident elements, more
+    than the 8 of any node of a structured mesh), two rings of nodes;
+    plane stress, the outer ring clamped, a load on the centre node."""
+    import dataclasses
+    th = 2 * np.pi * np.arange(k) / k
+    ph = th + np.pi / k
+    xy = np.vstack([[0.0, 0.0], np.c_[np.cos(th), np.sin(th)], np.c_[2 * np.cos(ph), 2 * np.sin(ph)]])
+    conn = np.array([[0, 1 + i, 1 + k + i, 1 + (i + 1) % k] for i in range(k)], dtype=np.int32)
+    base = meshgen.plane_stress_quad(2)
+    nn = xy.shape[0]
+    rest = np.zeros(2 * nn, np.uint8)
+    rest[2 * (1 + k):] = 1
+    load = np.zeros(2 * nn)
+    load[0], load[1] = 0.3, -0.7
+    return dataclasses.replace(base, name="fan_quad4_%d" % k, xyz=xy, conn=conn, restrained=rest,
+                               prescribed=np.zeros(2 * nn), load=load, extra={})
+
+
+def test_high_valence_node(hip_ctx_factory):
+    """Nodes with more incident elements than the register path of the
+    assembly gather holds: pattern, K and the solution against the oracle."""
+    prob = _fan_quads(10)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    eq = O.assemble_problem(prob)
+    rp_o, ci_o = eq.pattern()
+    rp_g, ci_g = ctx.pattern_lower()
+    assert np.array_equal(rp_o, rp_g) and np.array_equal(ci_o, ci_g)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    assert _row_scaled_err(rp_o, ci_o, ctx.values_lower(), eq.values()) <= 1e-12
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    st, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-13, stop_rule=1)
+    s = ctx.solve(tol=1e-12, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64)
+    assert np.linalg.norm(ctx.solution() - x_o) <= 1e-9 * np.linalg.norm(x_o)
