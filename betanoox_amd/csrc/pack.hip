@@ -163,8 +163,9 @@ __device__ __forceinline__ void pk_rows(const CT *__restrict__ scol, const float
 // every fourth block column, and wave 0 adds the others' row sums in wave
 // order through LDS.  Lanes past the system's last node skip their rows.
 // UNR block columns per loop trip (1: fewer registers, more waves per SIMD).
+// At least 4 waves per SIMD: the six-case form fits 126 VGPRs without spills.
 template <int R, int NT, int UNR>
-__global__ void __launch_bounds__(kThreads) k_pk_spmv(const PackHdr *__restrict__ hdr) {
+__global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restrict__ hdr) {
     __shared__ float red[3 * R][kThreads];
     __shared__ double sm[16 * R];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
