@@ -50,7 +50,8 @@ constexpr int kMaxR = 8;
 constexpr int kChunk = 32;             // iterations per captured graph (even: r.z parity)
 constexpr int kSplitSlices = 2048;     // systems with fewer slices run four waves per slice
 constexpr double kStreamBytes = 240e6;
-constexpr int kPackUnroll = 2;         // SpMV block columns per loop trip  // packs whose SELL arrays exceed the 256 MB last-level cache stream
+constexpr int kPackUnroll = 2;         // SpMV block columns per loop trip
+constexpr int kBatch = 4;              // elements per thread and trip in the vector kernels  // packs whose SELL arrays exceed the 256 MB last-level cache stream
 
 struct PackState {
     double rzb[2][kMaxR];  // r.z by iteration parity
@@ -260,12 +261,33 @@ __global__ void __launch_bounds__(kThreads) k_pk_update(const PackHdr *__restric
         const float *__restrict__ q = D.q;
         const float *__restrict__ dinv = D.dinv;
         double acc[2] = {0, 0};
-        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            const float ri = rv[o + i] - alpha * q[o + i];
-            x[o + i] = x[o + i] + alpha * p[o + i];
-            rv[o + i] = ri;
-            acc[0] += (double)ri * (ri * dinv[i]);
-            acc[1] += (double)ri * ri;
+        // kBatch elements per trip, operands loaded before any update (same
+        // elements and order of sums as a plain loop)
+        const long long stride = blockDim.x;
+        for (long long i = i0 + threadIdx.x; i < i1; i += kBatch * stride) {
+            float xv[kBatch], pv[kBatch], rr[kBatch], qv[kBatch], dv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const long long j = i + u * stride;
+                if (j < i1) {
+                    xv[u] = x[o + j];
+                    pv[u] = p[o + j];
+                    rr[u] = rv[o + j];
+                    qv[u] = q[o + j];
+                    dv[u] = dinv[j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const long long j = i + u * stride;
+                if (j < i1) {
+                    const float ri = rr[u] - alpha * qv[u];
+                    x[o + j] = xv[u] + alpha * pv[u];
+                    rv[o + j] = ri;
+                    acc[0] += (double)ri * (ri * dv[u]);
+                    acc[1] += (double)ri * ri;
+                }
+            }
         }
         block_sum<2>(acc, sm);
         if (threadIdx.x == 0) {
@@ -309,7 +331,24 @@ __global__ void __launch_bounds__(kThreads) k_pk_pupdate(const PackHdr *__restri
         float *__restrict__ p = D.p;
         const float *__restrict__ rv = D.r;
         const float *__restrict__ dinv = D.dinv;
-        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[o + i] = rv[o + i] * dinv[i] + beta * p[o + i];
+        const long long stride = blockDim.x;
+        for (long long i = i0 + threadIdx.x; i < i1; i += kBatch * stride) {
+            float pv[kBatch], rr[kBatch], dv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const long long j = i + u * stride;
+                if (j < i1) {
+                    pv[u] = p[o + j];
+                    rr[u] = rv[o + j];
+                    dv[u] = dinv[j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const long long j = i + u * stride;
+                if (j < i1) p[o + j] = rr[u] * dv[u] + beta * pv[u];
+            }
+        }
     }
 }
 
@@ -326,6 +365,7 @@ __global__ void __launch_bounds__(kThreads) k_pk_init(const PackHdr *__restrict_
         item_range(D.n, lv, D.n_vitems, i0, i1);
         const long long o = (long long)r * D.n;
         double acc[2] = {0, 0};
+#pragma unroll 4
         for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
             const float ri = D.r[o + i];
             const float zi = ri * D.dinv[i];

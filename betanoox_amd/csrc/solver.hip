@@ -39,6 +39,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kFinThreads = 1024;
+constexpr int kBatch = 4;  // elements per thread and trip in the fused vector kernels
 
 // ------------------------------------------------------------- partials
 template <int NV>
@@ -248,6 +249,7 @@ __global__ void __launch_bounds__(kThreads) k_pcg_update(long long n, T *__restr
     block_range(n, lb, gridDim.x, i0, i1);
     const T alpha = (T)st->alpha;
     double rz = 0, rr = 0, xx = 0;
+#pragma unroll 4
     for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         T xi = x[i] + alpha * p[i];
         T ri = r[i] - alpha * q[i];
@@ -273,6 +275,7 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate(long long n, T *__rest
     long long i0, i1;
     block_range(n, lb, gridDim.x, i0, i1);
     const T beta = (T)st->beta;
+#pragma unroll 4
     for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = r[i] * dinv[i] + beta * p[i];
 }
 
@@ -300,15 +303,37 @@ __global__ void __launch_bounds__(kThreads) k_pcg_update_fa(long long n, int par
     block_range(n, lb, gridDim.x, i0, i1);
     const T alpha = (T)alpha_d;
     double rzn = 0, rr = 0, xx = 0;
-    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        T xi = x[i] + alpha * p[i];
-        T ri = r[i] - alpha * q[i];
-        T zi = ri * dinv[i];
-        x[i] = xi;
-        r[i] = ri;
-        rzn += (double)ri * (double)zi;
-        rr += (double)ri * (double)ri;
-        xx += (double)xi * (double)xi;
+    // kBatch elements per trip: all their operands are loaded before any is
+    // updated (the elements and the order of the sums are those of a plain
+    // loop over i)
+    const long long stride = blockDim.x;
+    for (long long i = i0 + threadIdx.x; i < i1; i += kBatch * stride) {
+        T xv[kBatch], pv[kBatch], rv[kBatch], qv[kBatch], dv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const long long j = i + u * stride;
+            if (j < i1) {
+                xv[u] = x[j];
+                pv[u] = p[j];
+                rv[u] = r[j];
+                qv[u] = q[j];
+                dv[u] = dinv[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const long long j = i + u * stride;
+            if (j < i1) {
+                T xi = xv[u] + alpha * pv[u];
+                T ri = rv[u] - alpha * qv[u];
+                T zi = ri * dv[u];
+                x[j] = xi;
+                r[j] = ri;
+                rzn += (double)ri * (double)zi;
+                rr += (double)ri * (double)ri;
+                xx += (double)xi * (double)xi;
+            }
+        }
     }
     double v[3] = {rzn, rr, xx};
     store_partials<3>(v, pout);
@@ -358,7 +383,24 @@ __global__ void __launch_bounds__(kThreads) k_pcg_pupdate_fb(long long n, int pa
     long long i0, i1;
     block_range(n, lb, gridDim.x, i0, i1);
     const T beta = (T)beta_d;
-    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = r[i] * dinv[i] + beta * p[i];
+    const long long stride = blockDim.x;
+    for (long long i = i0 + threadIdx.x; i < i1; i += kBatch * stride) {
+        T pv[kBatch], rv[kBatch], dv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const long long j = i + u * stride;
+            if (j < i1) {
+                pv[u] = p[j];
+                rv[u] = r[j];
+                dv[u] = dinv[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const long long j = i + u * stride;
+            if (j < i1) p[j] = rv[u] * dv[u] + beta * pv[u];
+        }
+    }
 }
 
 // Initial: z = r*dinv, p = z; partials r.z, r.r, x.x.
@@ -370,6 +412,7 @@ __global__ void __launch_bounds__(kThreads) k_pcg_init(long long n, const T *__r
     long long i0, i1;
     block_range(n, lb, gridDim.x, i0, i1);
     double rz = 0, rr = 0, xx = 0;
+#pragma unroll 4
     for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         T zi = r[i] * dinv[i];
         p[i] = zi;
