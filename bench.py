@@ -282,6 +282,19 @@ def main():
     tuning = {"chosen": table[chosen], "shape0_ms": table[0]["ms"], "shapes_timed": len(table)}
     log("spmv tuning: %s" % json.dumps(tuning))
 
+    # plain FP64 Jacobi-PCG on FP64 3x3 blocks in the same SELL layout and tile
+    # shape: the FP64 comparison point with the FP32 path's storage
+    result_fp64b = None
+    if not args.no_fp64:
+        s = ctx.solve(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64_BLOCKED, maxit=20000)
+        msb, byb = ctx.bench_spmv(blocked64=True, reps=20)
+        result_fp64b = {
+            "iters": s["iters"], "ms": s["ms_total"], "rel_resid": s["rel_resid"],
+            "dof_iter_per_s": prob.n_dof * s["iters"] / (s["ms_total"] / 1e3),
+            "spmv_ms": msb, "spmv_GBps": byb / msb / 1e6, "spmv_bytes": byb,
+        }
+        log("fp64 blocked PCG: %s" % json.dumps(result_fp64b))
+
     opts = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
                              inner_tol=args.inner_tol, maxit=40000)
     for w in range(args.warmup):
@@ -334,6 +347,7 @@ def main():
         },
         "roofline": roofline,
         "plain_fp64": result_fp64,
+        "fp64_blocked": result_fp64b,
         "assembly": {"ms": 1e3 * min(t_asm), "elements_per_s": prob.n_el / min(t_asm)},
         "spmv_fp32_GBps": achieved,
         "spmv_tuning": tuning,
@@ -342,6 +356,9 @@ def main():
     if result_fp64:
         line["speedup_vs_fp64_dof_iter"] = value / result_fp64["dof_iter_per_s"]
         line["speedup_vs_fp64_time_to_solution"] = result_fp64["ms"] / (1e3 * dt / args.steps)
+    if result_fp64b:
+        line["speedup_vs_fp64_blocked_dof_iter"] = value / result_fp64b["dof_iter_per_s"]
+        line["speedup_vs_fp64_blocked_time_to_solution"] = result_fp64b["ms"] / (1e3 * dt / args.steps)
     ctx.close()
     if args.heat_n > 0:
         line["heat"] = heat_leg(args.heat_n, args.heat_steps, args.heat_fp64_steps)

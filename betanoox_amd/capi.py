@@ -39,6 +39,7 @@ STOP_REF_R_OVER_X = 0
 STOP_REL_RESID = 1
 PREC_FP64 = 0
 PREC_FP32_IR = 1
+PREC_FP64_BLOCKED = 2
 
 FMT_RCV_LOWER = 1
 FMT_RCV_UPPER = 2
@@ -488,9 +489,12 @@ class Context:
         self._chk(self.lib.bhip_bench_stream(self.h, nbytes, reps, C.byref(ms), C.byref(g)))
         return ms.value, g.value
 
-    def bench_spmv(self, fp32=False, reps=20):
+    def bench_spmv(self, fp32=False, reps=20, blocked64=False):
+        """Mean ms and algorithmic bytes of one SpMV: FP64 CSR, the FP32 form of
+        the inner solve (fp32=True) or FP64 SELL blocks (blocked64=True)."""
         ms, by = C.c_double(), C.c_double()
-        self._chk(self.lib.bhip_bench_spmv(self.h, int(fp32), reps, C.byref(ms), C.byref(by)))
+        form = 2 if blocked64 else int(bool(fp32))
+        self._chk(self.lib.bhip_bench_spmv(self.h, form, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
     def tune_spmv(self, reps=10):

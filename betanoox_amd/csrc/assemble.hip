@@ -672,7 +672,7 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->assembled = true;
     ctx->asm_kind = what;
-    ctx->f32_ready = false;
+    ctx->f32_ready = ctx->f64b_ready = false;
     ctx->lift_ready = false;
     if (hflags[0] & 1) return bhip_fail(ctx, BHIP_ERR_NEG_DETJ, "negative Jacobian determinant (inverted element)");
     if (hflags[0] & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "bad diagonal in an element stiffness");

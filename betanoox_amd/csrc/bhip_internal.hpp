@@ -129,6 +129,10 @@ struct bhip_ctx {
     dvec<short> sell_col16;        // block columns as offsets from the row's node (narrow form)
     bool sell_narrow = false;      // 16-bit offsets in use
     dvec<float> bval32;      // SELL block values, SoA: B*B x n_sell
+    // ---- FP64 values in the same SELL layout (BHIP_PREC_FP64_BLOCKED)
+    bool f64b_ready = false;       // bval64 holds the current matrix
+    bool bval64_layout_ok = false; // bval64 sized and zero-padded for the current layout
+    dvec<double> bval64;           // SELL block values, SoA: B*B x n_sell
     bool sell_layout_ok = false;  // SELL pointers/columns built for the current pattern
     int sell_shape = 0;      // tile shape id of the FP32 SELL SpMV
     bool sell_shape_set = false;  // chosen by bhip_tune_spmv / bhip_set_spmv_shape (else the
@@ -166,8 +170,8 @@ struct bhip_ctx {
     dvec<int> flags;        // assembly / diagonal check flags
     PcgState *h_state = nullptr;  // pinned mirror
     // ---- graphs
-    hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
-    int graph64_iters = 0, graph32_iters = 0;
+    hipGraphExec_t graph64 = nullptr, graph32 = nullptr, graph64b = nullptr;
+    int graph64_iters = 0, graph32_iters = 0, graph64b_iters = 0;
     double spmv_bytes64 = 0, spmv_bytes32 = 0;
     // ---- J2 plasticity state (plastic.hip)
     struct PlasticData *plastic = nullptr;

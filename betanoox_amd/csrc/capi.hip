@@ -84,6 +84,7 @@ void bhip_destroy(bhip_ctx *ctx) {
     dvec<float> *v32[] = {&ctx->val32, &ctx->bval32, &ctx->r32, &ctx->p32, &ctx->q32, &ctx->d32, &ctx->dinv32};
     for (auto *v : v32) v->release();
     ctx->sell_col16.release();
+    ctx->bval64.release();
     ctx->eblk.release();
     ctx->n2e_loc.release();
     ctx->work.release();
@@ -156,7 +157,7 @@ int bhip_set_mesh(bhip_ctx *ctx, int dim, int n_nodes, const double *xyz, int n_
     ctx->n_dof = (int)dof;
     ctx->max_ndof = mx;
     ctx->uniform_ndof = uni;
-    ctx->pattern_ready = ctx->assembled = ctx->f32_ready = false;
+    ctx->pattern_ready = ctx->assembled = ctx->f32_ready = ctx->f64b_ready = false;
     ctx->have_rest = false;
     ctx->external_matrix = false;
     hipStream_t s = ctx->stream;
@@ -198,7 +199,7 @@ int bhip_update_coords(bhip_ctx *ctx, const double *xyz) {
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->xyz.p, xyz, sizeof(double) * ctx->n_nodes * ctx->dim, hipMemcpyHostToDevice,
                                  ctx->stream));
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->assembled = ctx->f32_ready = ctx->lift_ready = false;
+    ctx->assembled = ctx->f32_ready = ctx->f64b_ready = ctx->lift_ready = false;
     return BHIP_OK;
 }
 
@@ -220,7 +221,7 @@ int bhip_set_restraints(bhip_ctx *ctx, const unsigned char *restrained, const do
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->have_rest = true;
     if (!same) {
-        ctx->pattern_ready = ctx->assembled = ctx->f32_ready = false;
+        ctx->pattern_ready = ctx->assembled = ctx->f32_ready = ctx->f64b_ready = false;
         bhip_invalidate_graphs(ctx);
     }
     return BHIP_OK;

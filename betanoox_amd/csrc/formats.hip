@@ -339,13 +339,14 @@ int bhip_set_csr(bhip_ctx *ctx, int n, long long nnz, const int *ia, const int *
     ctx->max_ndof = 0;
     ctx->uniform_ndof = false;
     ctx->have_rest = ctx->have_mat = false;
-    ctx->f32_ready = ctx->use_bsr = ctx->lift_ready = false;
+    ctx->f32_ready = ctx->f64b_ready = ctx->use_bsr = ctx->lift_ready = false;
     ctx->sell_b = 0;
     ctx->sell_ptr.release();
     ctx->sell_col.release();
     ctx->bval32.release();
+    ctx->bval64.release();
     ctx->val_aux.release();
-    ctx->sell_layout_ok = false;
+    ctx->sell_layout_ok = ctx->bval64_layout_ok = false;
     ctx->n_pairs = 0;
     ctx->nnz = (long long)ci.size();
     ctx->nnz_lower = nl;

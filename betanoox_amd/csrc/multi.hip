@@ -268,7 +268,8 @@ int bhip_cases_begin(bhip_ctx *ctx, int nrhs, const double *presc, const double 
     if (!presc || !opts) return bhip_fail(ctx, BHIP_ERR_ARG, "presc and opts are required");
     if (nrhs < 1 || nrhs > kMaxR) return bhip_fail(ctx, BHIP_ERR_ARG, "nrhs must be in [1, 8]");
     if (!(opts->tol > 0)) return bhip_fail(ctx, BHIP_ERR_ARG, "tol must be positive");
-    if (opts->precision != BHIP_PREC_FP64 && opts->precision != BHIP_PREC_FP32_IR)
+    if (opts->precision != BHIP_PREC_FP64 && opts->precision != BHIP_PREC_FP32_IR &&
+        opts->precision != BHIP_PREC_FP64_BLOCKED)
         return bhip_fail(ctx, BHIP_ERR_ARG, "unknown precision");
     if (!ctx->assembled) return bhip_fail(ctx, BHIP_ERR_STATE, "matrix not assembled");
     (void)hipSetDevice(ctx->device);

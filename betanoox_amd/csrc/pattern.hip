@@ -303,7 +303,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipStreamSynchronize(s));
     ctx->pattern_ready = true;
     ctx->assembled = false;
-    ctx->f32_ready = false;
+    ctx->f32_ready = ctx->f64b_ready = false;
     ctx->sell_shape_set = false;  // a tuned shape belongs to the pattern it was measured on
     // the FP32 layouts belong to the old pattern; their buffers are kept for reuse
     ctx->sell_layout_ok = false;

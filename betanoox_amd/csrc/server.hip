@@ -398,7 +398,7 @@ int bhip_server_create(int device, const bhip_server_opts *opts, bhip_server **o
     if (!(o.tol > 0)) o.tol = 1e-8;
     if (o.maxit <= 0) o.maxit = 40000;
     if (!(o.inner_tol > 0)) o.inner_tol = 1e-3;
-    if (o.precision != BHIP_PREC_FP64) o.precision = BHIP_PREC_FP32_IR;
+    if (o.precision != BHIP_PREC_FP64 && o.precision != BHIP_PREC_FP64_BLOCKED) o.precision = BHIP_PREC_FP32_IR;
     if (o.max_pack <= 0) o.max_pack = 8;
     if (o.max_pack > 64) o.max_pack = 64;
     srv->opts = o;

@@ -157,7 +157,8 @@ __global__ void __launch_bounds__(kThreads) k_csr_spmv(int n, const int *__restr
 // entry of the B x B block) and column indices are laid out so that the 64
 // lanes of a wave read 64 consecutive entries and consecutive nodes gather
 // neighbouring x entries.  No cross-lane reduction; padded blocks carry zeros
-// and the node's own column.
+// and the node's own column.  VT = float for the inner solve of FP32_IR,
+// double for the FP64 blocked solve (same layout, eight-byte values).
 //
 // Columns: CT = int holds the neighbour's node index; CT = short holds its
 // offset from the row's node (used when every offset fits 16 bits, which
@@ -177,13 +178,13 @@ __device__ __forceinline__ T ld_stream(const T *p) {
     else return *p;
 }
 
-template <int B, int MODE, int SPB, int UNR, int NT, class CT>
+template <int B, int MODE, int SPB, int UNR, int NT, class CT, class VT>
 __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slices,
                                                       const int *__restrict__ sptr,
                                                       const CT *__restrict__ scol,
-                                                      const float *__restrict__ sval, long long nsell,
-                                                      const float *__restrict__ x,
-                                                      float *__restrict__ y, const PcgState *st,
+                                                      const VT *__restrict__ sval, long long nsell,
+                                                      const VT *__restrict__ x,
+                                                      VT *__restrict__ y, const PcgState *st,
                                                       double *partials) {
     if (MODE == 1 && st->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -193,20 +194,20 @@ __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slice
     if (s < n_slices) {
         const long long node = s * 64 + lane;
         const int kb = sptr[s], ke = node < n_nodes ? sptr[s + 1] : kb;
-        float a[B];
+        VT a[B];
 #pragma unroll
-        for (int c = 0; c < B; c++) a[c] = 0.f;
+        for (int c = 0; c < B; c++) a[c] = 0;
 #pragma unroll UNR
         for (int k = kb + lane; k < ke; k += 64) {
             long long m;
             if constexpr (sizeof(CT) == 2) m = node + (long long)ld_stream<NT>(scol + k);
             else m = ld_stream<NT>(scol + k);
-            float xv[B];
+            VT xv[B];
 #pragma unroll
             for (int d = 0; d < B; d++) xv[d] = x[B * m + d];
 #pragma unroll
             for (int c = 0; c < B; c++) {
-                float t = ld_stream<NT>(sval + (c * B) * nsell + k) * xv[0];
+                VT t = ld_stream<NT>(sval + (c * B) * nsell + k) * xv[0];
 #pragma unroll
                 for (int d = 1; d < B; d++) t += ld_stream<NT>(sval + (c * B + d) * nsell + k) * xv[d];
                 a[c] += t;
@@ -562,12 +563,12 @@ __global__ void k_max_offset(long long n_pairs, const int *pair_row, const int *
     if ((threadIdx.x & 63) == 0) atomicMax(out, v);
 }
 
-// FP64 CSR -> FP32 SELL-64 B x B blocks (SoA); restrained entries zero,
+// FP64 CSR -> SELL-64 B x B blocks of VT (SoA); restrained entries zero,
 // restrained diagonal 1.  Pair p = (node n, k-th neighbour m).
-template <int B, class CT>
+template <int B, class CT, class VT>
 __global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
                               const int *nbr_off, const int *rowptr, const unsigned char *rmask,
-                              const double *val, const int *sptr, long long nsell, CT *scol, float *sval) {
+                              const double *val, const int *sptr, long long nsell, CT *scol, VT *sval) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     int n = pair_row[p], m = nbr[p];
@@ -580,10 +581,10 @@ __global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int 
         int r = 0;
         long long base = (long long)rowptr[B * n + c] + nbr_off[p];
         for (int d = 0; d < B; d++) {
-            float v = 0.f;
+            VT v = 0;
             bool rc = (rn >> c) & 1u, rd = (rm >> d) & 1u;
-            if (!rc && !rd) v = (float)val[base + r];
-            else if (rc && n == m && c == d) v = 1.f;
+            if (!rc && !rd) v = (VT)val[base + r];
+            else if (rc && n == m && c == d) v = 1;
             if (!rd) r++;
             sval[(long long)(c * B + d) * nsell + dst] = v;
         }
@@ -657,17 +658,17 @@ void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *
 #undef BHIP_CSR_CASE
 }
 
-template <int B, int MODE, int SPB, int UNR, int NT>
-void launch_sell_t(bhip_ctx *ctx, const float *x, float *y, const PcgState *st) {
+template <int B, int MODE, int SPB, int UNR, int NT, class VT>
+void launch_sell_t(bhip_ctx *ctx, const VT *sval, const VT *x, VT *y, const PcgState *st) {
     long long g = (ctx->n_slices + SPB - 1) / SPB;
     const int grid = (int)(g > 0 ? g : 1);
     if (ctx->sell_narrow)
-        k_sell_spmv<B, MODE, SPB, UNR, NT, short><<<grid, 64 * SPB, 0, ctx->stream>>>(
-            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col16.p, ctx->bval32.p, ctx->n_sell, x, y, st,
+        k_sell_spmv<B, MODE, SPB, UNR, NT, short, VT><<<grid, 64 * SPB, 0, ctx->stream>>>(
+            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col16.p, sval, ctx->n_sell, x, y, st,
             ctx->partials.p);
     else
-        k_sell_spmv<B, MODE, SPB, UNR, NT, int><<<grid, 64 * SPB, 0, ctx->stream>>>(
-            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p, ctx->bval32.p, ctx->n_sell, x, y, st,
+        k_sell_spmv<B, MODE, SPB, UNR, NT, int, VT><<<grid, 64 * SPB, 0, ctx->stream>>>(
+            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p, sval, ctx->n_sell, x, y, st,
             ctx->partials.p);
 }
 
@@ -678,36 +679,50 @@ constexpr int kNumSellShapes = 18;
     X(7, 16, 2, 0) X(8, 16, 8, 0) X(9, 4, 4, 1) X(10, 4, 2, 1) X(11, 4, 8, 1) X(12, 8, 4, 1)             \
     X(13, 8, 2, 1) X(14, 8, 8, 1) X(15, 16, 4, 1) X(16, 16, 2, 1) X(17, 16, 8, 1)
 
-template <int B>
-void launch_sell_b(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgState *st) {
+template <int B, class VT>
+void launch_sell_b(bhip_ctx *ctx, int mode, int shape, const VT *sval, const VT *x, VT *y, const PcgState *st) {
     if (mode == 0) {
-        launch_sell_t<B, 0, 4, 4, 0>(ctx, x, y, st);
+        launch_sell_t<B, 0, 4, 4, 0>(ctx, sval, x, y, st);
         return;
     }
-    switch (ctx->sell_shape) {
+    switch (shape) {
 #define BHIP_SHAPE_CASE(ID, SPB, UNR, NT) \
     case ID:                              \
-        launch_sell_t<B, 1, SPB, UNR, NT>(ctx, x, y, st); \
+        launch_sell_t<B, 1, SPB, UNR, NT>(ctx, sval, x, y, st); \
         break;
         BHIP_SELL_SHAPES(BHIP_SHAPE_CASE)
 #undef BHIP_SHAPE_CASE
         default:
-            launch_sell_t<B, 1, 4, 4, 0>(ctx, x, y, st);
+            launch_sell_t<B, 1, 4, 4, 0>(ctx, sval, x, y, st);
     }
 }
 
+// FP32 SELL SpMV of the inner solve (block values bval32) in the context's shape.
 void launch_sell(bhip_ctx *ctx, int mode, const float *x, float *y, const PcgState *st) {
-    if (ctx->sell_b == 3) launch_sell_b<3>(ctx, mode, x, y, st);
-    else if (ctx->sell_b == 2) launch_sell_b<2>(ctx, mode, x, y, st);
-    else launch_sell_b<1>(ctx, mode, x, y, st);
+    const float *v = ctx->bval32.p;
+    if (ctx->sell_b == 3) launch_sell_b<3>(ctx, mode, ctx->sell_shape, v, x, y, st);
+    else if (ctx->sell_b == 2) launch_sell_b<2>(ctx, mode, ctx->sell_shape, v, x, y, st);
+    else launch_sell_b<1>(ctx, mode, ctx->sell_shape, v, x, y, st);
+}
+
+// FP64 SELL SpMV of the FP64 blocked solve (block values bval64), same shape.
+void launch_sell64(bhip_ctx *ctx, int mode, const double *x, double *y, const PcgState *st) {
+    const double *v = ctx->bval64.p;
+    const int shape = ctx->sell_shape;
+    if (ctx->sell_b == 3) launch_sell_b<3>(ctx, mode, shape, v, x, y, st);
+    else if (ctx->sell_b == 2) launch_sell_b<2>(ctx, mode, shape, v, x, y, st);
+    else launch_sell_b<1>(ctx, mode, shape, v, x, y, st);
 }
 
 // Algorithmic bytes of one SpMV (values + indices + row pointers + x + y);
 // for the blocked form the stored (padded) blocks are counted.
+double sell_bytes(bhip_ctx *ctx, int value_bytes) {
+    return (double)ctx->n_sell * (ctx->sell_b * ctx->sell_b * value_bytes + (ctx->sell_narrow ? 2 : 4)) +
+           (ctx->n_slices + 1.0) * 4 + 2.0 * ctx->n_dof * value_bytes;
+}
+
 double spmv_bytes(bhip_ctx *ctx, bool fp32, bool bsr) {
-    if (bsr)
-        return (double)ctx->n_sell * (ctx->sell_b * ctx->sell_b * 4 + (ctx->sell_narrow ? 2 : 4)) +
-               (ctx->n_slices + 1.0) * 4 + 2.0 * ctx->n_dof * 4;
+    if (bsr) return sell_bytes(ctx, 4);
     double sv = fp32 ? 4 : 8;
     return (double)ctx->nnz * (sv + 4) + (ctx->n_dof + 1.0) * 4 + 2.0 * ctx->n_dof * sv;
 }
@@ -797,34 +812,73 @@ int build_sell(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipGetLastError());
     if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
     ctx->sell_layout_ok = true;
+    ctx->bval64_layout_ok = false;
+    ctx->f32_ready = ctx->f64b_ready = false;
+    return BHIP_OK;
+}
+
+// Blocked form applies: the same 1-3 DOFs at every node.
+bool bsr_eligible(bhip_ctx *ctx) {
+    const int nb = ctx->max_ndof;
+    return ctx->uniform_ndof && nb >= 1 && nb <= 3 && (long long)ctx->n_dof == (long long)nb * ctx->n_nodes;
+}
+
+// SELL pointers and columns for the current pattern with B = DOFs per node.
+int ensure_sell_layout(bhip_ctx *ctx) {
+    const int nb = ctx->max_ndof;
+    if (ctx->sell_b != nb || ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_layout_ok) {
+        ctx->sell_b = nb;
+        BHIP_PROPAGATE(build_sell(ctx));
+    }
+    return BHIP_OK;
+}
+
+// CSR values -> SELL block values of VT (columns rewritten too, same values).
+template <class VT>
+void enqueue_to_sell(bhip_ctx *ctx, VT *sval) {
+    const int B = 256, nb = ctx->sell_b;
+    const int grid = grid_for(ctx->n_pairs, B);
+#define BHIP_TO_SELL(BB, CT, COL)                                                                                   \
+    k_csr_to_sell<BB, CT, VT><<<grid, B, 0, ctx->stream>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, \
+                                                           ctx->nbr_off.p, ctx->rowptr.p, ctx->node_rmask.p,          \
+                                                           ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, COL, sval)
+    if (ctx->sell_narrow) {
+        if (nb == 3) BHIP_TO_SELL(3, short, ctx->sell_col16.p);
+        else if (nb == 2) BHIP_TO_SELL(2, short, ctx->sell_col16.p);
+        else BHIP_TO_SELL(1, short, ctx->sell_col16.p);
+    } else {
+        if (nb == 3) BHIP_TO_SELL(3, int, ctx->sell_col.p);
+        else if (nb == 2) BHIP_TO_SELL(2, int, ctx->sell_col.p);
+        else BHIP_TO_SELL(1, int, ctx->sell_col.p);
+    }
+#undef BHIP_TO_SELL
+}
+
+// FP64 block values for BHIP_PREC_FP64_BLOCKED.
+int prepare_f64b(bhip_ctx *ctx) {
+    if (ctx->f64b_ready) return BHIP_OK;
+    if (!bsr_eligible(ctx))
+        return bhip_fail(ctx, BHIP_ERR_ARG, "FP64 blocked precision needs the same 1-3 DOFs at every node");
+    BHIP_PROPAGATE(ensure_sell_layout(ctx));
+    if (!ctx->bval64_layout_ok) {
+        bhip_invalidate_graphs(ctx);
+        BHIP_TRY(ctx, ctx->bval64.resize((size_t)ctx->sell_b * ctx->sell_b * (ctx->n_sell > 0 ? ctx->n_sell : 1)));
+        BHIP_TRY(ctx, hipMemsetAsync(ctx->bval64.p, 0, ctx->bval64.bytes(), ctx->stream));
+        ctx->bval64_layout_ok = true;
+    }
+    enqueue_to_sell<double>(ctx, ctx->bval64.p);
+    BHIP_TRY(ctx, hipGetLastError());
+    ctx->f64b_ready = true;
     return BHIP_OK;
 }
 
 int prepare_f32(bhip_ctx *ctx) {
     if (ctx->f32_ready) return BHIP_OK;
     const int B = 256;
-    const int nb = ctx->max_ndof;
-    ctx->use_bsr = (ctx->uniform_ndof && nb >= 1 && nb <= 3 && (long long)ctx->n_dof == (long long)nb * ctx->n_nodes);
+    ctx->use_bsr = bsr_eligible(ctx);
     if (ctx->use_bsr) {
-        if (ctx->sell_b != nb || ctx->sell_ptr.n != (size_t)((ctx->n_nodes + 63) / 64 + 1) || !ctx->sell_layout_ok) {
-            ctx->sell_b = nb;
-            BHIP_PROPAGATE(build_sell(ctx));
-        }
-#define BHIP_TO_SELL(BB)                                                                                      \
-    do {                                                                                                      \
-        if (ctx->sell_narrow)                                                                                 \
-            k_csr_to_sell<BB, short><<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(                        \
-                ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,     \
-                ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col16.p, ctx->bval32.p); \
-        else                                                                                                  \
-            k_csr_to_sell<BB, int><<<grid_for(ctx->n_pairs, B), B, 0, ctx->stream>>>(                          \
-                ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, ctx->nbr_off.p, ctx->rowptr.p,     \
-                ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, ctx->sell_col.p, ctx->bval32.p);  \
-    } while (0)
-        if (nb == 3) BHIP_TO_SELL(3);
-        else if (nb == 2) BHIP_TO_SELL(2);
-        else BHIP_TO_SELL(1);
-#undef BHIP_TO_SELL
+        BHIP_PROPAGATE(ensure_sell_layout(ctx));
+        enqueue_to_sell<float>(ctx, ctx->bval32.p);
         if (ctx->val32.p) {
             bhip_invalidate_graphs(ctx);
             ctx->val32.release();
@@ -841,14 +895,26 @@ int prepare_f32(bhip_ctx *ctx) {
     return BHIP_OK;
 }
 
-// Enqueue one PCG iteration: three kernels on the FP32 blocked form (par =
-// parity of the iteration, which selects the r.z buffer), five otherwise.
-void enqueue_iteration(bhip_ctx *ctx, bool fp32, int par) {
+// Iteration kinds of run_iterations: FP64 on CSR values, the FP32 inner solve
+// (SELL or CSR values), FP64 on SELL block values.
+enum IterKind { kIterCsr64 = 0, kIterInner32 = 1, kIterSell64 = 2 };
+
+// Enqueue one PCG iteration: three kernels on a blocked form (par = parity of
+// the iteration, which selects the r.z buffer), five on CSR.
+void enqueue_iteration(bhip_ctx *ctx, int kind, int par) {
     hipStream_t s = ctx->stream;
     PcgState *st = ctx->state.p;
     const long long n = ctx->n_dof;
     const int gv = vec_grid(ctx, n);
-    if (!fp32) {
+    if (kind == kIterSell64) {
+        double *p2 = ctx->partials.p + fused_offset(ctx);
+        launch_sell64(ctx, 1, ctx->p.p, ctx->q.p, st);
+        k_pcg_update_fa<double><<<gv, kThreads, 0, s>>>(n, par, sell_groups(ctx), ctx->partials.p, ctx->x.p,
+                                                        ctx->r.p, ctx->p.p, ctx->q.p, ctx->dinv.p, st, p2);
+        k_pcg_pupdate_fb<double><<<gv, kThreads, 0, s>>>(n, par, gv, p2, ctx->p.p, ctx->r.p, ctx->dinv.p, st);
+        return;
+    }
+    if (kind == kIterCsr64) {
         launch_csr<double, double, 1>(ctx, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
         k_fin_alpha<<<1, kFinThreads, 0, s>>>(csr_grid(ctx), ctx->partials.p, st);
         k_pcg_update<double><<<gv, kThreads, 0, s>>>(n, ctx->x.p, ctx->r.p, ctx->p.p, ctx->q.p, ctx->dinv.p, st,
@@ -882,9 +948,9 @@ bool graphs_enabled() {
 }
 
 // A captured chunk of `chunk` iterations (kernel nodes only).
-int get_graph(bhip_ctx *ctx, bool fp32, int chunk, hipGraphExec_t *out) {
-    hipGraphExec_t &g = fp32 ? ctx->graph32 : ctx->graph64;
-    int &gi = fp32 ? ctx->graph32_iters : ctx->graph64_iters;
+int get_graph(bhip_ctx *ctx, int kind, int chunk, hipGraphExec_t *out) {
+    hipGraphExec_t &g = kind == kIterInner32 ? ctx->graph32 : kind == kIterSell64 ? ctx->graph64b : ctx->graph64;
+    int &gi = kind == kIterInner32 ? ctx->graph32_iters : kind == kIterSell64 ? ctx->graph64b_iters : ctx->graph64_iters;
     if (g && gi == chunk) {
         *out = g;
         return BHIP_OK;
@@ -895,7 +961,7 @@ int get_graph(bhip_ctx *ctx, bool fp32, int chunk, hipGraphExec_t *out) {
     }
     hipGraph_t graph;
     BHIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < chunk; k++) enqueue_iteration(ctx, fp32, k & 1);
+    for (int k = 0; k < chunk; k++) enqueue_iteration(ctx, kind, k & 1);
     hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     BHIP_TRY(ctx, e);
     e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
@@ -907,7 +973,7 @@ int get_graph(bhip_ctx *ctx, bool fp32, int chunk, hipGraphExec_t *out) {
 }
 
 // Run PCG iterations already initialised in the device state until done.
-int run_iterations(bhip_ctx *ctx, bool fp32, int check_every, int *spmv_launches) {
+int run_iterations(bhip_ctx *ctx, int kind, int check_every, int *spmv_launches) {
     hipStream_t s = ctx->stream;
     PcgState *st = ctx->state.p;
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
@@ -915,7 +981,7 @@ int run_iterations(bhip_ctx *ctx, bool fp32, int check_every, int *spmv_launches
     if (ctx->h_state->done) return BHIP_OK;
     const bool use_graph = graphs_enabled();
     hipGraphExec_t g = nullptr;
-    if (use_graph) BHIP_PROPAGATE(get_graph(ctx, fp32, check_every, &g));
+    if (use_graph) BHIP_PROPAGATE(get_graph(ctx, kind, check_every, &g));
     // every chunk advances at least one iteration unless the device marked the
     // solve done, so maxit/check_every + 2 chunks always suffice
     const long long max_chunks = (long long)ctx->h_state->maxit / check_every + 2;
@@ -925,7 +991,7 @@ int run_iterations(bhip_ctx *ctx, bool fp32, int check_every, int *spmv_launches
         if (use_graph) {
             BHIP_TRY(ctx, hipGraphLaunch(g, s));
         } else {
-            for (int k = 0; k < check_every; k++) enqueue_iteration(ctx, fp32, k & 1);
+            for (int k = 0; k < check_every; k++) enqueue_iteration(ctx, kind, k & 1);
         }
         BHIP_TRY(ctx, hipGetLastError());
         if (spmv_launches) *spmv_launches += check_every;
@@ -955,8 +1021,9 @@ void enqueue_init(bhip_ctx *ctx, const T *x, const T *r, T *p, const T *dinv) {
 void bhip_invalidate_graphs(bhip_ctx *ctx) {
     if (ctx->graph64) (void)hipGraphExecDestroy(ctx->graph64);
     if (ctx->graph32) (void)hipGraphExecDestroy(ctx->graph32);
-    ctx->graph64 = ctx->graph32 = nullptr;
-    ctx->graph64_iters = ctx->graph32_iters = 0;
+    if (ctx->graph64b) (void)hipGraphExecDestroy(ctx->graph64b);
+    ctx->graph64 = ctx->graph32 = ctx->graph64b = nullptr;
+    ctx->graph64_iters = ctx->graph32_iters = ctx->graph64b_iters = 0;
     bhip_multi_invalidate(ctx);
 }
 
@@ -1003,6 +1070,8 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
 
     BHIP_PROPAGATE(prepare_diag(ctx));
     if (opts.precision == BHIP_PREC_FP32_IR) BHIP_PROPAGATE(prepare_f32(ctx));
+    else if (opts.precision == BHIP_PREC_FP64_BLOCKED) BHIP_PROPAGATE(prepare_f64b(ctx));
+    else if (opts.precision != BHIP_PREC_FP64) return bhip_fail(ctx, BHIP_ERR_ARG, "unknown precision");
     BHIP_PROPAGATE(ensure_partials(ctx));
 
     BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
@@ -1010,19 +1079,21 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
     else BHIP_TRY(ctx, hipMemsetAsync(ctx->x.p, 0, sizeof(double) * n, s));
 
     int st_code = BHIP_OK;
-    if (opts.precision == BHIP_PREC_FP64) {
+    if (opts.precision == BHIP_PREC_FP64 || opts.precision == BHIP_PREC_FP64_BLOCKED) {
         // r = b - K x0 (sparse.cpp:152-163), z = p = r/diag, then the PCG loop
+        // (on the CSR values or on the same values in SELL block form)
+        const bool blocked = opts.precision == BHIP_PREC_FP64_BLOCKED;
         k_state_init<<<1, 1, 0, s>>>(st, opts.tol, opts.maxit, opts.stop_rule);
         enqueue_residual(ctx, ctx->r.p);
         enqueue_init<double>(ctx, ctx->x.p, ctx->r.p, ctx->p.p, ctx->dinv.p);
         BHIP_TRY(ctx, hipGetLastError());
-        BHIP_PROPAGATE(run_iterations(ctx, false, opts.check_every, &S.spmv_launches));
+        BHIP_PROPAGATE(run_iterations(ctx, blocked ? kIterSell64 : kIterCsr64, opts.check_every, &S.spmv_launches));
         S.iters = ctx->h_state->iter;
         S.outer_iters = 1;
         S.ref_norm = ctx->h_state->stop_rule == BHIP_STOP_REF_R_OVER_X ? ctx->h_state->norm
                                                                         : sqrt(ctx->h_state->rr / ctx->h_state->xx);
         if (!ctx->h_state->converged) st_code = BHIP_ERR_NOT_CONVERGED;
-        S.spmv_bytes = spmv_bytes(ctx, false, false);
+        S.spmv_bytes = blocked ? sell_bytes(ctx, 8) : spmv_bytes(ctx, false, false);
     } else if (opts.precision == BHIP_PREC_FP32_IR) {
         double inner_tol = opts.inner_tol > 0 ? opts.inner_tol : 1e-3;
         int max_outer = opts.max_outer > 0 ? opts.max_outer : 50;
@@ -1050,7 +1121,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
             k_state_init<<<1, 1, 0, s>>>(st, inner_tol, opts.maxit - total_inner, 2);
             enqueue_init<float>(ctx, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->dinv32.p);
             BHIP_TRY(ctx, hipGetLastError());
-            BHIP_PROPAGATE(run_iterations(ctx, true, opts.check_every, &S.spmv_launches));
+            BHIP_PROPAGATE(run_iterations(ctx, kIterInner32, opts.check_every, &S.spmv_launches));
             total_inner += ctx->h_state->iter;
             k_axpy_f32<<<grid_for(n, B), B, 0, s>>>(n, rn, ctx->d32.p, ctx->x.p);
             BHIP_TRY(ctx, hipGetLastError());
@@ -1061,8 +1132,6 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
         S.ref_norm = 0;
         if (!ok) st_code = BHIP_ERR_NOT_CONVERGED;
         S.spmv_bytes = spmv_bytes(ctx, true, ctx->use_bsr);
-    } else {
-        return bhip_fail(ctx, BHIP_ERR_ARG, "unknown precision");
     }
     BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
     BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
@@ -1084,21 +1153,25 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
 
 int bhip_bench_spmv_impl(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes) {
     if (!ctx->assembled) return bhip_fail(ctx, BHIP_ERR_STATE, "matrix not assembled");
+    if (fp32 < 0 || fp32 > 2) return bhip_fail(ctx, BHIP_ERR_ARG, "unknown SpMV form");
     if (reps <= 0) reps = 1;
     hipStream_t s = ctx->stream;
     PcgState *st = ctx->state.p;
     BHIP_PROPAGATE(prepare_diag(ctx));
-    if (fp32) BHIP_PROPAGATE(prepare_f32(ctx));
+    const bool sell64 = fp32 == 2;
+    if (fp32 == 1) BHIP_PROPAGATE(prepare_f32(ctx));
+    if (sell64) BHIP_PROPAGATE(prepare_f64b(ctx));
     BHIP_PROPAGATE(ensure_partials(ctx));
-    const bool bsr = fp32 && ctx->use_bsr;
+    const bool bsr = fp32 == 1 && ctx->use_bsr;
     const long long n = ctx->n_dof;
     const int B = 256;
     // p = finite data, state not done: the PCG form of the kernel runs in full
-    if (fp32) k_to_f32<<<grid_for(n, B), B, 0, s>>>(n, ctx->dinv.p, ctx->p32.p);
+    if (fp32 == 1) k_to_f32<<<grid_for(n, B), B, 0, s>>>(n, ctx->dinv.p, ctx->p32.p);
     else BHIP_TRY(ctx, hipMemcpyAsync(ctx->p.p, ctx->dinv.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     k_state_init<<<1, 1, 0, s>>>(st, 1.0, 1, 0);
     auto launch = [&]() {
-        if (bsr) launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
+        if (sell64) launch_sell64(ctx, 1, ctx->p.p, ctx->q.p, st);
+        else if (bsr) launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
         else if (fp32) launch_csr<float, float, 1>(ctx, ctx->val32.p, ctx->p32.p, ctx->q32.p, nullptr, st);
         else launch_csr<double, double, 1>(ctx, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
     };
@@ -1111,7 +1184,7 @@ int bhip_bench_spmv_impl(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, doub
     float ms = 0;
     BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *ms_avg = ms / reps;
-    *bytes = spmv_bytes(ctx, fp32 != 0, bsr);
+    *bytes = sell64 ? sell_bytes(ctx, 8) : spmv_bytes(ctx, fp32 == 1, bsr);
     return BHIP_OK;
 }
 

@@ -59,6 +59,8 @@ extern "C" {
 
 #define BHIP_PREC_FP64          0   /* plain FP64 Jacobi-PCG on FP64 CSR */
 #define BHIP_PREC_FP32_IR       1   /* FP32 inner Jacobi-PCG (FP32 values), FP64 iterative refinement */
+#define BHIP_PREC_FP64_BLOCKED  2   /* plain FP64 Jacobi-PCG on FP64 B x B node blocks (SELL-64 layout of
+                                       the FP32 inner solve); needs the same 1-3 DOFs at every node */
 
 typedef struct bhip_ctx bhip_ctx;
 
@@ -196,8 +198,9 @@ void bhip_pack_destroy(bhip_pack *pk);
 int  bhip_pack_solve_cases(bhip_pack *pk, int n, bhip_ctx *const *ctxs, int nrhs, const double *const *presc,
                            const double *const *load, double *const *x, const bhip_solve_opts *opts,
                            bhip_stats *stats, int *status);
-/* Time `reps` FP64 (fp32 = 0) or FP32 (fp32 = 1) SpMVs on resident vectors;
- * returns the mean kernel time in ms (HIP events). */
+/* Time `reps` SpMVs on resident vectors: fp32 = 0 FP64 CSR, 1 the FP32 form of
+ * the inner solve, 2 FP64 SELL blocks (BHIP_PREC_FP64_BLOCKED); returns the
+ * mean kernel time in ms (HIP events) and the algorithmic bytes per SpMV. */
 int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *bytes);
 /* Tile shapes of the FP32 blocked (SELL-64) SpMV of the inner solve.  No
  * counterpart in the reference (its SpMV is the serial vectorProduct,
@@ -355,7 +358,7 @@ typedef struct bhip_server_opts {
     double tol;          /* per case |b - K x| / |b| (0: 1e-8) */
     double inner_tol;    /* FP32_IR inner residual reduction (0: 1e-3) */
     int    maxit;        /* inner iteration cap per solve (0: 40000) */
-    int    precision;    /* BHIP_PREC_FP32_IR (default) or BHIP_PREC_FP64 */
+    int    precision;    /* BHIP_PREC_FP32_IR (default), BHIP_PREC_FP64 or BHIP_PREC_FP64_BLOCKED */
     int    max_pack;     /* small requests solved together in one packed batched PCG
                             (0: 8; 1: no packing; at most 64) */
 } bhip_server_opts;
