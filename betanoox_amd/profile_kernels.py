@@ -7,9 +7,10 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   reps x plain device copy of 1 GiB       (bhip_bench_stream: k_stream_copy)
   reps x FP32 SELL-64 3x3-block SpMV      (k_sell_spmv<3, 1>)
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
+  reps x FP64 SELL-64 3x3-block SpMV      (k_sell_spmv<3, 1, ..., double>, BHIP_PREC_FP64_BLOCKED)
   one FP32-inner + FP64-refined solve     (unless --no-solve)
 --workload cases (RVE request, --n 64, six load cases):
-  one fused six-case solve                (k_sell3_spmv_multi<6>, k_update_multi<6>, ...)
+  one six-case solve in the packed solver (k_pk_spmv<6, ...>, k_pk_update<6>, k_pk_pupdate<6>)
 --workload heat (diffusion, --n 128, 1 DOF per node):
   reps x FP32 SELL-64 scalar SpMV         (k_sell_spmv<1, 1>)
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
@@ -55,6 +56,8 @@ def main(argv=None):
             out["sell3_fp32"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6, "shape": ctx.spmv_shape()}
             ms, by = ctx.bench_spmv(fp32=False, reps=a.reps)
             out["csr_fp64"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
+            ms, by = ctx.bench_spmv(blocked64=True, reps=a.reps)
+            out["sell3_fp64"] = {"bytes": by, "ms": ms, "GBps": by / ms / 1e6}
             if not a.no_solve:
                 s = ctx.solve(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
                 out["solve_fp32_ir"] = {k: s[k] for k in ("iters", "outer_iters", "rel_resid", "ms_total")}
