@@ -607,8 +607,10 @@ __global__ void k_axpy_f32(long long n, double s, const float *d32, double *x) {
 }
 
 // ---------------------------------------------------------------- helpers
+// Workgroups of the vector kernels: 16 elements per thread (measured on
+// gfx950 against 8 and 32: 1-2% faster per inner iteration at 2-3M DOFs).
 int vec_grid(bhip_ctx *ctx, long long n) {
-    long long want = (n + (long long)kThreads * 8 - 1) / ((long long)kThreads * 8);
+    long long want = (n + (long long)kThreads * 16 - 1) / ((long long)kThreads * 16);
     long long cap = (long long)ctx->n_cu * 8;
     if (cap > kRedBlocks) cap = kRedBlocks;
     if (want > cap) want = cap;
