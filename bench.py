@@ -4,19 +4,20 @@ MI355X, with the other workloads of the path measured beside it.
 One step = one full solve of the resident system to a true relative residual
 of 1e-8 with the chosen form (FP32-inner Jacobi-PCG on FP32 3x3-block values
 with FP64 iterative refinement).  value = DOF * PCG iterations / second over
-the timed steps.  The plain FP64 CSR PCG is measured beside it, the FP64
-assembly is timed separately, the dominant kernel (the SpMV of the inner
+the timed steps.  The plain FP64 PCG is measured beside it on the FP64 CSR
+and on FP64 3x3 blocks in the same layout, the FP64 assembly is timed separately, the dominant kernel (the SpMV of the inner
 solve) is timed with HIP events for the roofline, and the oracle (serial CPU
 restatement of the reference path) is timed on a bounded sample.
 
 Also reported in the same JSON line:
   heat    -- transient diffusion 128^3, 100 backward-Euler steps to 1e-8
+  packing -- 8 small RVE requests solved one after another and in one pack
   served  -- RVE request stream (prefix of the seeded 4,096-request stream),
              solves/s and p50/p99 latency at 1e-8
   plastic -- J2 plasticity bar, 20 increments (only with --plastic-n 64; ~70 s)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96]
-                       [--heat-n 128] [--served 128] [--plastic-n 0]
+                       [--heat-n 128] [--pack-demo 8] [--served 128] [--plastic-n 0]
 """
 import argparse
 import json
@@ -81,6 +82,46 @@ def cpu_baseline(sample_n, budget_s):
                   % (sample_n, prob.n_dof, it, t_pcg, t_asm, prob.n_el / t_asm),
         "cpu": platform.processor() or platform.machine(),
     }
+
+
+def pack_leg(count, reps=3):
+    """Packing: `count` small RVE requests (n = 16..24, six load cases each,
+    seeded) assembled on their own contexts, solved to 1e-8 one after another
+    (bhip_solve_cases per context) and together in one packed solve
+    (bhip_pack_solve_cases); best of `reps` for each."""
+    from betanoox_amd import capi, meshgen, setup_problem
+    sizes = [16 + (i * 8) // max(count - 1, 1) for i in range(count)]
+    probs = [meshgen.rve_request(n, 500 + i) for i, n in enumerate(sizes)]
+    o = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
+                          inner_tol=1e-3, maxit=40000)
+    ctxs = []
+    try:
+        for p in probs:
+            c = capi.Context(0)
+            ctxs.append(c)
+            setup_problem(c, p)
+            c.assemble(capi.ASM_STIFFNESS)
+        P = [np.stack(p.extra["cases"]) for p in probs]
+        seq = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            for c, pc in zip(ctxs, P):
+                c.solve_cases(pc, opts=o)
+            seq.append(time.perf_counter() - t0)
+        with capi.Pack(0) as pk:
+            packed = []
+            for _ in range(reps + 1):
+                t0 = time.perf_counter()
+                _, sts, status = pk.solve_cases(ctxs, P, opts=o)
+                packed.append(time.perf_counter() - t0)
+        assert all(x == 0 for x in status)
+        t_seq, t_pack = min(seq[1:]), min(packed[1:])
+        return {"requests": count, "n": sizes, "cases_per_request": 6, "tol": 1e-8,
+                "one_after_another_ms": 1e3 * t_seq, "packed_ms": 1e3 * t_pack, "speedup": t_seq / t_pack,
+                "inner_iters": [st["iters"] for st in sts]}
+    finally:
+        for c in ctxs:
+            c.close()
 
 
 def served_stream(count, slots, rate_frac, probe_count, seed=None, max_pack=0):
@@ -232,6 +273,7 @@ def main():
     ap.add_argument("--served-load", type=float, default=0.7, help="arrival rate / probed capacity")
     ap.add_argument("--served-probe", type=int, default=32)
     ap.add_argument("--served-pack", type=int, default=0, help="requests per packed solve (0: library default 8)")
+    ap.add_argument("--pack-demo", type=int, default=8, help="small requests of the packing leg (0: skip)")
     ap.add_argument("--heat-n", type=int, default=128, help="diffusion cube size (0: skip)")
     ap.add_argument("--heat-steps", type=int, default=100)
     ap.add_argument("--heat-fp64-steps", type=int, default=10)
@@ -366,6 +408,9 @@ def main():
     if args.plastic_n > 0:
         line["plastic"] = plastic_leg(args.plastic_n, args.plastic_tol, 300)
         log("plastic: %s" % json.dumps(line["plastic"]))
+    if args.pack_demo > 0:
+        line["packing"] = pack_leg(args.pack_demo)
+        log("packing: %s" % json.dumps(line["packing"]))
     if args.served > 0:
         line["served"] = served_stream(args.served, args.served_slots, args.served_load, args.served_probe,
                                        max_pack=args.served_pack)

@@ -222,7 +222,7 @@ static int run_mesh_file(int argc, char **argv) {
 int main(int argc, char **argv) {
     if (argc >= 3 && !std::strcmp(argv[1], "mesh")) return run_mesh_file(argc, argv);
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s quad|hex N [--tol T] [--rule ref|rel] [--fp32ir] [--dump FILE]\n"
+        std::fprintf(stderr, "usage: %s quad|hex N [--tol T] [--rule ref|rel] [--fp32ir|--fp64blocked] [--dump FILE]\n"
                              "       %s mesh FILE [--binary] [--settings FILE] [--parallel-assembly T] [--E e] [--nu v]\n"
                              "            [--solution FILE] [--parse-only]\n", argv[0], argv[0]);
         return 2;
@@ -237,6 +237,7 @@ int main(int argc, char **argv) {
         if (!std::strcmp(argv[a], "--tol") && a + 1 < argc) tol = std::atof(argv[++a]);
         else if (!std::strcmp(argv[a], "--rule") && a + 1 < argc) rule = std::strcmp(argv[++a], "ref") ? BHIP_STOP_REL_RESID : BHIP_STOP_REF_R_OVER_X;
         else if (!std::strcmp(argv[a], "--fp32ir")) prec = BHIP_PREC_FP32_IR;
+        else if (!std::strcmp(argv[a], "--fp64blocked")) prec = BHIP_PREC_FP64_BLOCKED;
         else if (!std::strcmp(argv[a], "--dump") && a + 1 < argc) dump = argv[++a];
     }
     if (n <= 0 || (kind != "quad" && kind != "hex")) return 2;

@@ -213,6 +213,9 @@ SolverSettings readSolverSettings(std::istream &in) {
             } else if (t == "HIPPCG_FP64") {
                 s.precision = BHIP_PREC_FP64;
                 s.stop_rule = BHIP_STOP_REL_RESID;
+            } else if (t == "HIPPCG_FP64_BLOCKED") {
+                s.precision = BHIP_PREC_FP64_BLOCKED;
+                s.stop_rule = BHIP_STOP_REL_RESID;
             } else if (t == "SPARSE" || t == "PARCG" || t == "MKLCG" || t == "PROFILE" || t == "OLAF" ||
                        t == "PARDISO" || t == "MKLPARDISO" || t == "WSMPSYMM" || t == "PETSC") {
                 s.precision = BHIP_PREC_FP64;  // equation.cpp:36-140 tokens; run as the FP64 Jacobi-PCG

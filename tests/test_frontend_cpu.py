@@ -55,6 +55,11 @@ def test_cpp_reader_and_settings(tmp_path):
     assert rc == 0, err
     assert out["node_offset"] == "0" and out["storage"] == "SPARSE" and out["precision"] == "0"
     assert out["parallel_assembly_hip"] == "0"
+    for token, prec in (("HIPPCG_FP64", "0"), ("HIPPCG_FP64_BLOCKED", "2"), ("hippcg", "1")):
+        (tmp_path / "s2.txt").write_text("setStorageMethod %s\nend\n" % token)
+        rc, out, err = _parse(tmp_path / "m.bin", "--binary", "--settings", tmp_path / "s2.txt")
+        assert rc == 0, err
+        assert out["storage"] == token.upper() and out["precision"] == prec
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="beta_hip_driver not built")

@@ -5,7 +5,8 @@ structured meshes of the driver, and the oracle's solution.
 Tolerances: the SPARSE settings (FP64 PCG, reference stop rule) reproduce the
 structured run's iteration count exactly and its solution to 1e-12 (the
 nodal loads are summed face by face); HIPPCG (FP32 inner + FP64 refinement)
-meets its 1e-8 relative residual and agrees with the oracle to 1e-6; the
+meets its 1e-8 relative residual and agrees with the oracle to 1e-6, and
+HIPPCG_FP64_BLOCKED meets 1e-11 and agrees to 1e-8; the
 Solution.txt the front-end writes reads back to the --dump of the same run.
 """
 import os
@@ -52,15 +53,19 @@ def test_mesh_file_matches_structured_run(tmp_path, kind, n, binary):
     assert np.abs(u - u_ref).max() <= 1e-12 * np.abs(u_ref).max()
 
 
-def test_hippcg_settings_against_oracle(tmp_path):
+@pytest.mark.parametrize("token,tol,err", [("HIPPCG", 1e-8, 1e-6), ("HIPPCG_FP64_BLOCKED", 1e-11, 1e-8)])
+def test_hippcg_settings_against_oracle(tmp_path, token, tol, err):
+    """HIPPCG (FP32 inner + FP64 refinement) to 1e-8 agrees with the oracle's
+    1e-12 solve to 1e-6; HIPPCG_FP64_BLOCKED (FP64 PCG on FP64 node blocks)
+    to 1e-11 agrees to 1e-8."""
     n = 5
     xyz, conn = meshgen.structured_hex8(n, n, n)
     meshio.write_text(tmp_path / "m.txt", xyz, conn)
-    (tmp_path / "s.txt").write_text("setStorageMethod HIPPCG\nsetSparseSolverTolerance 1e-8\nend\n")
+    (tmp_path / "s.txt").write_text("setStorageMethod %s\nsetSparseSolverTolerance %g\nend\n" % (token, tol))
     got = _kv(["mesh", tmp_path / "m.txt", "--settings", tmp_path / "s.txt", "--solution", tmp_path / "sol.txt"])
-    assert float(got["rel_resid"]) <= 1e-8 and int(got["outer"]) >= 1
+    assert float(got["rel_resid"]) <= tol and int(got["outer"]) >= 1
     prob = meshgen.elastic_cube(n, jitter=0.0, e_range=(1.0, 1.0), nu_range=(0.3, 0.3))
     eq = O.assemble_problem(prob)
     st, x, it, _ = eq.pcg(eq.rhs(), eps=1e-12, stop_rule=1)
     u = capi.read_vector(tmp_path / "sol.txt", capi.VEC_SOLUTION, prob.n_dof)
-    assert np.linalg.norm(u - x) <= 1e-6 * np.linalg.norm(x)
+    assert np.linalg.norm(u - x) <= err * np.linalg.norm(x)
