@@ -358,6 +358,7 @@ def main():
     ms32, by32 = ctx.bench_spmv(fp32=True, reps=50)
     achieved = by32 / ms32 / 1e6
     _, stream_gbps = ctx.bench_stream(1 << 30, 20)
+    _, read_gbps = ctx.bench_read(2 << 30, 10)
     sh = table[chosen]
     kname = "k_sell_spmv<3, 1, %d, %d, %d, %s, float>" % (sh["slices_per_block"], sh["unroll"], sh["nontemporal"],
                                                           "short" if sh["column_bytes"] == 2 else "int")
@@ -394,6 +395,8 @@ def main():
         "spmv_fp32_GBps": achieved,
         "spmv_tuning": tuning,
         "stream_copy_GBps": stream_gbps,
+        "stream_read_GBps": read_gbps,
+        "spmv_frac_of_stream_read": achieved / read_gbps,
     }
     if result_fp64:
         line["speedup_vs_fp64_dof_iter"] = value / result_fp64["dof_iter_per_s"]

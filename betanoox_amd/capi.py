@@ -64,7 +64,7 @@ EXPORTS = [
     "bhip_get_csr", "bhip_set_materials", "bhip_set_integration", "bhip_assemble",
     "bhip_get_pseudo_load", "bhip_set_load", "bhip_build_rhs", "bhip_get_rhs", "bhip_set_rhs",
     "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv", "bhip_solve_cases",
-    "bhip_bench_spmv", "bhip_bench_stream", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
+    "bhip_bench_spmv", "bhip_bench_stream", "bhip_bench_read", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
     "bhip_server_create", "bhip_server_destroy", "bhip_server_submit", "bhip_server_poll", "bhip_server_wait",
     "bhip_server_budget", "bhip_request_footprint", "bhip_heat_run", "bhip_write_matrix", "bhip_write_csr_file",
@@ -193,6 +193,7 @@ def load_library(path=LIB_PATH):
         "bhip_solve_cases": (C.c_int, [vp, C.c_int, _P_D, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),
         "bhip_bench_stream": (C.c_int, [vp, C.c_longlong, C.c_int, _P_D, _P_D]),
+        "bhip_bench_read": (C.c_int, [vp, C.c_longlong, C.c_int, _P_D, _P_D]),
         "bhip_tune_spmv": (C.c_int, [vp, C.c_int, C.POINTER(SpmvShape), C.c_int, _P_I, _P_I]),
         "bhip_set_spmv_shape": (C.c_int, [vp, C.c_int]),
         "bhip_get_spmv_shape": (C.c_int, [vp]),
@@ -487,6 +488,12 @@ class Context:
     def bench_stream(self, nbytes=1 << 30, reps=20):
         ms, g = C.c_double(), C.c_double()
         self._chk(self.lib.bhip_bench_stream(self.h, nbytes, reps, C.byref(ms), C.byref(g)))
+        return ms.value, g.value
+
+    def bench_read(self, nbytes=2 << 30, reps=10):
+        """bhip_bench_read: mean ms and GB/s of a read-only device stream."""
+        ms, g = C.c_double(), C.c_double()
+        self._chk(self.lib.bhip_bench_read(self.h, nbytes, reps, C.byref(ms), C.byref(g)))
         return ms.value, g.value
 
     def bench_spmv(self, fp32=False, reps=20, blocked64=False):

@@ -476,6 +476,22 @@ __global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ 
          i += (long long)gridDim.x * blockDim.x)
         b[i] = a[i];
 }
+
+// Read-only stream: nontemporal 16-byte loads, one sum per wave written out
+// (so that no load is dead).
+__global__ void __launch_bounds__(256) k_stream_read(const float4 *__restrict__ a, long long n, float *out) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v *av = reinterpret_cast<const f4v *>(a);
+    float s = 0.f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const f4v v = __builtin_nontemporal_load(av + i);
+        s += (v.x + v.y) + (v.z + v.w);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) out[blockIdx.x * 4 + (threadIdx.x >> 6)] = s;
+}
 }  // namespace
 
 extern "C" {
@@ -501,6 +517,30 @@ int bhip_bench_stream(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, 
     BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *ms_avg = ms / reps;
     *gbps = 2.0 * n4 * 16 / (*ms_avg * 1e6);
+    return BHIP_OK;
+}
+
+int bhip_bench_read(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, double *gbps) {
+    if (!ctx || !ms_avg || !gbps || bytes < 16 || reps <= 0) return BHIP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    long long n4 = bytes / 16;
+    // 8 blocks of 256 threads per CU: the best grid of this loop on gfx950
+    const int grid = ctx->n_cu * 8;
+    dvec<float4> a;
+    dvec<float> out;
+    BHIP_TRY(ctx, a.resize(n4));
+    BHIP_TRY(ctx, out.resize((size_t)grid * 4));
+    BHIP_TRY(ctx, hipMemsetAsync(a.p, 0, a.bytes(), ctx->stream));
+    for (int w = 0; w < 2; w++) k_stream_read<<<grid, 256, 0, ctx->stream>>>(a.p, n4, out.p);
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int k = 0; k < reps; k++) k_stream_read<<<grid, 256, 0, ctx->stream>>>(a.p, n4, out.p);
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    BHIP_TRY(ctx, hipGetLastError());
+    float ms = 0;
+    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *ms_avg = ms / reps;
+    *gbps = (double)n4 * 16 / (*ms_avg * 1e6);
     return BHIP_OK;
 }
 

@@ -234,6 +234,11 @@ int bhip_spmv_shape_describe(int id, bhip_spmv_shape *info);
  * bytes (read + write), timed over `reps` launches; reports the achieved
  * read+write GB/s (STREAM-copy convention) of this GPU. */
 int bhip_bench_stream(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, double *gbps);
+/* Diagnostic without a reference counterpart: a read-only stream of `bytes`
+ * bytes (16-byte nontemporal loads), timed over `reps` launches; reports the
+ * read GB/s this GPU sustains, the practical ceiling of the read-dominated
+ * SpMVs. */
+int bhip_bench_read(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, double *gbps);
 
 /* ------------------------------------------------------------ plasticity */
 typedef struct bhip_plastic_stats {
