@@ -62,6 +62,17 @@ struct dvec {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Device array owned by a scope: released when the scope ends (hipFree first
+// waits for the device, so kernels still using it finish).  For function-local
+// temporaries; dvec members are released by their owners.
+template <class T>
+struct scoped_dvec : dvec<T> {
+    scoped_dvec() = default;
+    scoped_dvec(const scoped_dvec &) = delete;
+    scoped_dvec &operator=(const scoped_dvec &) = delete;
+    ~scoped_dvec() { this->release(); }
+};
+
 // Device-side solver state of one PCG (FP64 or FP32 inner).  Lives in device
 // memory so that kernels chain without host round trips; the host polls it.
 struct PcgState {

@@ -329,7 +329,7 @@ int bhip_set_integration(bhip_ctx *ctx, int order, double factor) {
 int bhip_assemble(bhip_ctx *ctx, int what, const double *coef, const double *u_state) {
     if (!ctx) return BHIP_ERR_ARG;
     (void)hipSetDevice(ctx->device);
-    dvec<double> du;
+    scoped_dvec<double> du;
     const double *dptr = nullptr;
     if (u_state) {
         BHIP_TRY(ctx, du.resize(ctx->n_dof));
@@ -500,7 +500,7 @@ int bhip_bench_stream(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, 
     if (!ctx || !ms_avg || !gbps || bytes < 16 || reps <= 0) return BHIP_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     long long n4 = bytes / 16;
-    dvec<float4> a, b;
+    scoped_dvec<float4> a, b;
     BHIP_TRY(ctx, a.resize(n4));
     BHIP_TRY(ctx, b.resize(n4));
     BHIP_TRY(ctx, hipMemsetAsync(a.p, 0, a.bytes(), ctx->stream));
@@ -526,8 +526,8 @@ int bhip_bench_read(bhip_ctx *ctx, long long bytes, int reps, double *ms_avg, do
     long long n4 = bytes / 16;
     // 8 blocks of 256 threads per CU: the best grid of this loop on gfx950
     const int grid = ctx->n_cu * 8;
-    dvec<float4> a;
-    dvec<float> out;
+    scoped_dvec<float4> a;
+    scoped_dvec<float> out;
     BHIP_TRY(ctx, a.resize(n4));
     BHIP_TRY(ctx, out.resize((size_t)grid * 4));
     BHIP_TRY(ctx, hipMemsetAsync(a.p, 0, a.bytes(), ctx->stream));

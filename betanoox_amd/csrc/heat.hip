@@ -106,7 +106,7 @@ extern "C" int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode,
     BHIP_PROPAGATE(bhip_lift_build(ctx));
     bhip_heat_stats H;
     memset(&H, 0, sizeof(H));
-    dvec<double> us;
+    scoped_dvec<double> us;
     BHIP_TRY(ctx, us.resize((size_t)n));
     BHIP_TRY(ctx, hipMemcpyAsync(us.p, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
     // Fbar = -(a1 + a2) K u_s / dt with a1 + a2 = dt for every alpha

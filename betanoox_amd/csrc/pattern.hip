@@ -316,7 +316,7 @@ int bhip_export_lower(bhip_ctx *ctx, int *rowptr, int *colind, double *val) {
     hipStream_t s = ctx->stream;
     const int B = 256;
     const int nd = ctx->n_dof;
-    dvec<long long> len, scan;
+    scoped_dvec<long long> len, scan;
     BHIP_TRY(ctx, len.resize(nd + 1));
     BHIP_TRY(ctx, scan.resize(nd + 1));
     BHIP_TRY(ctx, hipMemsetAsync(len.p, 0, sizeof(long long) * (nd + 1), s));
@@ -327,8 +327,8 @@ int bhip_export_lower(bhip_ctx *ctx, int *rowptr, int *colind, double *val) {
     if (rowptr)
         for (int i = 0; i <= nd; i++) rowptr[i] = (int)h[i];
     long long nl = h[nd];
-    dvec<int> dcol;
-    dvec<double> dval;
+    scoped_dvec<int> dcol;
+    scoped_dvec<double> dval;
     if (colind) BHIP_TRY(ctx, dcol.resize(nl > 0 ? nl : 1));
     if (val) BHIP_TRY(ctx, dval.resize(nl > 0 ? nl : 1));
     k_copy_lower<<<grid_for(nd, B), B, 0, s>>>(nd, ctx->rowptr.p, ctx->colind.p, ctx->val.p, scan.p,
