@@ -4,6 +4,8 @@ operand) fails it.  Sources of truth: the reference's own debug-main known
 answers (matrix.cpp:687-696), the reference's own shape.cpp built into
 oracle/_ref, closed forms, invariants and brute force on tiny inputs."""
 import ctypes as C
+import json
+import os
 import math
 
 import numpy as np
@@ -108,24 +110,33 @@ def test_quad4_bit_exact_with_reference():
 
 
 # ------------------------------------------------------------------ dense LU
+# Known answers of the reference's dense-matrix debug main (matrix.cpp:662-696),
+# kept in tests/golden/matrix_debug_main.json (written by the script beside it).
+GOLDEN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                     "matrix_debug_main.json")))
+
+
 def test_invert_matches_reference_debug_main():
-    """matrix.cpp:696: B = {{2,4},{7,-3}}, B.Invert = {{0.088235,0.11765},{0.20588,-.058824}}."""
-    st, Bi = O.invert(np.array([[2.0, 4.0], [7.0, -3.0]]))
+    """matrix.cpp:696: B = {{2,4},{7,-3}}, B.Invert = {{0.088235,0.11765},{0.20588,-.058824}}
+    (printed to 5 significant digits)."""
+    st, Bi = O.invert(np.array(GOLDEN["B"]))
     assert st == 0
-    assert np.allclose(Bi, [[0.088235, 0.11765], [0.20588, -0.058824]], atol=5e-6)
+    assert np.allclose(Bi, GOLDEN["invert"], rtol=5e-5, atol=0)
+    assert np.allclose(Bi @ np.array(GOLDEN["B"]), np.eye(2), atol=1e-15)
 
 
 def test_ludcmp_matches_reference_debug_main():
-    """matrix.cpp:692: A = {{1,2},{3,1}} factors to {{3,1},{.3333,1.6667}} and
-    A.Solve({5,5}) gives (1,2)."""
-    A = np.array([[1.0, 2.0], [3.0, 1.0]])
+    """matrix.cpp:692: A = {{1,2},{3,1}} factors in place to {{3,1},{.3333,1.6667}}
+    (printed to 4 decimals) and A.Solve({5,5}) gives (1,2)."""
+    A = np.array(GOLDEN["A"])
     indx = np.zeros(2, np.int32)
     dd = C.c_double()
     O.lib().orc_ludcmp(2, O.d(A), O.i32(indx), C.byref(dd))
-    assert np.allclose(A, [[3, 1], [1 / 3, 5 / 3]], atol=1e-15)
-    b = np.array([5.0, 5.0])
+    assert np.allclose(A, GOLDEN["lu_factors"], atol=5e-5)
+    assert np.allclose(A, [[3, 1], [1 / 3, 5 / 3]], atol=1e-15)  # the exact factors
+    b = np.array(GOLDEN["solve_rhs"])
     O.lib().orc_lubksb(2, O.d(A), O.i32(indx), O.d(b))
-    assert np.allclose(b, [1.0, 2.0], atol=1e-15)
+    assert np.allclose(b, GOLDEN["solve_x"], atol=1e-15)
 
 
 def test_invert_random_against_numpy():
