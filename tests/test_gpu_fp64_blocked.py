@@ -15,8 +15,8 @@ not beyond the tolerance.  Checks (tolerances per check):
     relative of the FP64 CSR solve, both to 1e-12
   - the first 1, 2 and 5 iterates equal those of the FP64 CSR solve within
     1e-12 relative (they carry the SpMV through alpha and the residual)
-  - every tile shape gives the same bits; the same solve twice gives the
-    same bits
+  - every tile shape gives the same bits, and so do int32 block columns
+    against 16-bit offsets; the same solve twice gives the same bits
   - several load cases (solved in turn) equal single solves bit for bit
   - a matrix without a mesh behind it -> BHIP_ERR_ARG
   - bench_spmv reports the algorithmic bytes of the FP64 blocked form
@@ -122,6 +122,24 @@ def test_every_shape_same_bits(hip_ctx_factory):
         s = ctx.solve(_o64b(1e-10))
         assert s["iters"] == s0["iters"], sid
         assert np.array_equal(ctx.solution().view(np.uint64), x0.view(np.uint64)), sid
+
+
+def test_wide_columns_same_bits(hip_ctx_factory, monkeypatch):
+    """int32 block columns (the form for meshes whose neighbours lie more than
+    32767 nodes apart) give the same bits as 16-bit offsets."""
+    prob = _problem("hex11")
+    ctx = _ready(hip_ctx_factory, prob)
+    s_n = ctx.solve(_o64b(1e-10))
+    x_n = ctx.solution()
+    _, by_n = ctx.bench_spmv(blocked64=True, reps=1)
+    monkeypatch.setenv("BHIP_SELL_WIDE_COLUMNS", "1")
+    ctx_w = _ready(hip_ctx_factory, prob)
+    s_w = ctx_w.solve(_o64b(1e-10))
+    _, by_w = ctx_w.bench_spmv(blocked64=True, reps=1)
+    monkeypatch.delenv("BHIP_SELL_WIDE_COLUMNS")
+    assert by_w > by_n  # 4-byte against 2-byte columns
+    assert s_w["iters"] == s_n["iters"]
+    assert np.array_equal(ctx_w.solution().view(np.uint64), x_n.view(np.uint64))
 
 
 def test_cases_in_turn(hip_ctx_factory):
