@@ -15,6 +15,8 @@ Checks:
     requests overlap in time; a request above the budget is refused
   - a failing request (inverted element) reports its status and message and
     the server keeps serving
+  - a server run in BHIP_PREC_FP64 or BHIP_PREC_FP64_BLOCKED solves each
+    request's cases in turn with the same bits as the private solve
 """
 import numpy as np
 import pytest
@@ -160,3 +162,21 @@ def test_other_case_counts(hip_ctx_factory):
         assert s["status"] == capi.OK and s["rel_resid"] <= TOL
         assert np.array_equal(X, Xr)
     assert max(s["pack_size"] for _, s in out) > 1
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_FP64_BLOCKED])
+def test_fp64_precisions_served(hip_ctx_factory, prec):
+    probs = [meshgen.rve_request(n, 900 + n) for n in (6, 8)]
+    o = capi.Context.opts(tol=TOL, stop_rule=capi.STOP_REL_RESID, precision=prec, inner_tol=1e-3,
+                          maxit=40000, max_outer=0, check_every=0)
+    ref = []
+    for p in probs:
+        ctx = setup_problem(hip_ctx_factory(), p)
+        ctx.assemble(capi.ASM_STIFFNESS)
+        ref.append(ctx.solve_cases(np.stack(p.extra["cases"]), opts=o)[0])
+    with capi.Server(0, max_slots=2, precision=prec) as srv:
+        ids = [srv.submit(p) for p in probs]
+        out = [srv.wait(i) for i in ids]
+    for (X, st), Xr in zip(out, ref):
+        assert st["status"] == capi.OK and st["rel_resid"] <= TOL and st["pack_size"] == 1
+        assert np.array_equal(X, Xr)
