@@ -193,14 +193,16 @@ def served_stream(count, slots, rate_frac, probe_count, seed=None, max_pack=0):
     return out
 
 
-def heat_leg(n, steps, fp64_steps):
+def heat_leg(n, steps, fp64_steps, inner_tol=1e-4):
     """Workload 3: transient diffusion on an n^3 hex8 cube (1 DOF per node),
     backward Euler dt = 1e-3, Mbar assembled once, every step solved to 1e-8
-    true relative residual on the device (FP32 inner + FP64 refinement; plain
-    FP64 timed beside it on the first `fp64_steps` steps)."""
+    true relative residual on the device (FP32 inner + FP64 refinement with
+    inner reduction `inner_tol`, measured best for this workload: 1e-3 71.5,
+    3e-3 78.2, 1e-4 83.0 steps/s over 20 steps; plain FP64 timed beside it on
+    the first `fp64_steps` steps)."""
     from betanoox_amd import capi, meshgen, setup_problem
     prob = meshgen.diffusion_cube(n)
-    out = {"n": n, "n_dof": prob.n_dof, "steps": steps, "dt": prob.extra["dt"], "tol": 1e-8}
+    out = {"n": n, "n_dof": prob.n_dof, "steps": steps, "dt": prob.extra["dt"], "tol": 1e-8, "inner_tol": inner_tol}
     with capi.Context(0) as ctx:
         setup_problem(ctx, prob)
         u0 = prob.extra["u0"]
@@ -210,7 +212,7 @@ def heat_leg(n, steps, fp64_steps):
         chosen, table = ctx.tune_spmv(reps=10)
         out["spmv_tuning"] = {"chosen": table[chosen], "shape0_ms": table[0]["ms"]}
         o32 = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
-                                inner_tol=1e-3, maxit=40000)
+                                inner_tol=inner_tol, maxit=40000)
         ctx.heat_run(2, u0, capi.HEAT_X0_PREVIOUS, opts=o32)  # warm-up (graphs, tables)
         t0 = time.perf_counter()
         u, st = ctx.heat_run(steps, u0, capi.HEAT_X0_PREVIOUS, opts=o32)
@@ -277,6 +279,7 @@ def main():
     ap.add_argument("--heat-n", type=int, default=128, help="diffusion cube size (0: skip)")
     ap.add_argument("--heat-steps", type=int, default=100)
     ap.add_argument("--heat-fp64-steps", type=int, default=10)
+    ap.add_argument("--heat-inner-tol", type=float, default=1e-4)
     ap.add_argument("--plastic-n", type=int, default=0, help="J2 bar size (0: skip; the full workload is 64)")
     ap.add_argument("--plastic-tol", type=float, default=1e-9)
     args = ap.parse_args()
@@ -406,7 +409,7 @@ def main():
         line["speedup_vs_fp64_blocked_time_to_solution"] = result_fp64b["ms"] / (1e3 * dt / args.steps)
     ctx.close()
     if args.heat_n > 0:
-        line["heat"] = heat_leg(args.heat_n, args.heat_steps, args.heat_fp64_steps)
+        line["heat"] = heat_leg(args.heat_n, args.heat_steps, args.heat_fp64_steps, args.heat_inner_tol)
         log("heat: %s" % json.dumps(line["heat"]))
     if args.plastic_n > 0:
         line["plastic"] = plastic_leg(args.plastic_n, args.plastic_tol, 300)
