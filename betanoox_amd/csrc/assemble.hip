@@ -80,7 +80,6 @@ __device__ __forceinline__ int group_of(const AsmArgs &A, int e) {
 // the capacity entry for heat), G doubles of staged Gauss-point data, MS
 // doubles of staged material.
 
-__device__ __forceinline__ int n_pairs_of(int nn) { return nn * (nn + 1) / 2; }
 // packed index of local pair (i, j), i <= j, row by row
 __device__ __forceinline__ int pair_index(int nn, int i, int j) { return i * nn - i * (i - 1) / 2 + (j - i); }
 __device__ __forceinline__ void pair_of_index(int nn, int pr, int &i, int &j) {
