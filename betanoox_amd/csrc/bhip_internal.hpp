@@ -254,6 +254,17 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *smem /* 16*NV
     }
 }
 
+// Inner residual reduction of one refinement step: the requested inner_tol,
+// loosened when the outer relative residual rel is already close to tol, so
+// that the last step does not overshoot.  It stays 5x short of the reduction
+// needed (a step's true reduction trails its inner one) and at most 0.1.  It
+// depends only on computed residuals, so solves stay reproducible.
+inline double ir_inner_tol(double inner_tol, double tol, double rel) {
+    if (!(rel > 0)) return inner_tol;
+    const double need = 0.2 * tol / rel;
+    return need > inner_tol ? (need < 0.1 ? need : 0.1) : inner_tol;
+}
+
 // --------------------------------------------------------- internal entry
 int bhip_ensure_vectors(bhip_ctx *ctx);
 // New coordinates for the current mesh; pattern and restraints stay.

@@ -591,6 +591,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
     const char *fu = getenv("BHIP_PACK_UNROLL");
     if (fu && (fu[0] == '1' || fu[0] == '2')) unr = fu[0] - '0';
     std::vector<int> total(n_all, 0), outer(n_all, 0), active(n_all, 1), ok(n_all, 0);
+    std::vector<double> worst(n_all, 1.0);  // worst unconverged relative residual per system
     for (int i = 0; i < n_all; i++) {
         memset(&stats[i], 0, sizeof(bhip_stats));
         status[i] = BHIP_OK;
@@ -612,10 +613,12 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
             const double *rb = pe->h_resid + (size_t)i * 2 * R;
             double *sc = pe->h_scale + (size_t)i * 2 * kMaxR;
             bool all = true;
+            worst[i] = 0;
             for (int r = 0; r < R; r++) {
                 const double rr = rb[2 * r], bb = rb[2 * r + 1];
                 const double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
                 const bool conv = rel <= o.tol || rr == 0.0;
+                if (!conv) worst[i] = std::max(worst[i], rel);
                 const double rn = conv ? 0.0 : sqrt(rr);
                 sc[r] = conv ? 0.0 : 1.0 / rn;  // a converged case gets a zero inner RHS
                 sc[kMaxR + r] = rn;
@@ -671,7 +674,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
             D.n_vitems = sys_vitems(c, R, pe->grid);
             D.vitem0 = n_v;
             D.maxit = maxit - total[run[k]];
-            D.tol = inner_tol;
+            D.tol = ir_inner_tol(inner_tol, o.tol, worst[run[k]]);
             D.rec_s = n_rec;
             n_rec += (long long)R * D.n_sitems;
             D.rec_v = n_rec;

@@ -1120,7 +1120,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
             // inner FP32 PCG on K32 d = r/|r|
             double rn = sqrt(rr);
             k_scale_to_f32<<<grid_for(n, B), B, 0, s>>>(n, ctx->r.p, 1.0 / rn, ctx->r32.p, ctx->d32.p);
-            k_state_init<<<1, 1, 0, s>>>(st, inner_tol, opts.maxit - total_inner, 2);
+            k_state_init<<<1, 1, 0, s>>>(st, ir_inner_tol(inner_tol, opts.tol, rel), opts.maxit - total_inner, 2);
             enqueue_init<float>(ctx, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->dinv32.p);
             BHIP_TRY(ctx, hipGetLastError());
             BHIP_PROPAGATE(run_iterations(ctx, kIterInner32, opts.check_every, &S.spmv_launches));

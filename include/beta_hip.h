@@ -69,7 +69,8 @@ typedef struct bhip_solve_opts {
     int    maxit;        /* iteration cap (reference default 10000) */
     int    stop_rule;    /* BHIP_STOP_* */
     int    precision;    /* BHIP_PREC_* */
-    double inner_tol;    /* FP32_IR: relative residual reduction per inner solve */
+    double inner_tol;    /* FP32_IR: relative residual reduction per inner solve (loosened, up to
+                            0.1, for a step whose outer residual is already near tol) */
     int    max_outer;    /* FP32_IR: refinement cap */
     int    check_every;  /* host polls convergence every this many iterations (0 = 32) */
     int    use_x0;       /* 1: start from the x0 held by the context (bhip_set_x0), 0: from zero */
