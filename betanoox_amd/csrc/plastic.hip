@@ -480,7 +480,6 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
     S.ms = ms;
     // plastic points and return-map iterations of the converged state
     long long npts = (long long)ctx->n_el * P->nip;
-    std::vector<double> flags(npts);
     {
         std::vector<double> rec(npts * kIsv);
         BHIP_TRY(ctx, bhip_copy(ctx, rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
