@@ -290,6 +290,28 @@ __global__ void k_max_abs(long long n, const double *v, double *red, unsigned *c
     }
 }
 
+// Plastic points (record slot 14 non-zero) and the largest return-map
+// iteration count (slot 15) over all integration points: wave partials
+// combined with one atomic per wave into out[0] (count) and out[1] (max),
+// which the caller clears.  An unsigned count holds any point count whose
+// 128-byte records fit in device memory.
+__global__ void k_isv_stats(long long npts, const double *__restrict__ isv, unsigned *out) {
+    unsigned c = 0, m = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += (long long)gridDim.x * blockDim.x) {
+        const double *r = isv + t * kIsv;
+        c += r[14] != 0.0;
+        m = max(m, (unsigned)r[15]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o);
+        m = max(m, (unsigned)__shfl_xor(m, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (c) atomicAdd(out, c);
+        atomicMax(out + 1, m);
+    }
+}
+
 // currentSolution at the start of an increment: prescribed values on
 // restrained DOFs, zero elsewhere (equations.zeroSolution + loads applyTo).
 __global__ void k_du_start(int n, const unsigned char *rest, const double *presc, double *du) {
@@ -390,12 +412,12 @@ int bhip_plastic_init(bhip_ctx *ctx) {
     BHIP_TRY(ctx, P->utot.resize(ctx->n_dof));
     BHIP_TRY(ctx, P->corr.resize(ctx->n_dof));
     BHIP_TRY(ctx, P->red.resize(1025));
-    BHIP_TRY(ctx, P->cnt.resize(1));
+    BHIP_TRY(ctx, P->cnt.resize(3));
     BHIP_TRY(ctx, hipMemsetAsync(P->isv.p, 0, P->isv.bytes(), ctx->stream));
     BHIP_TRY(ctx, hipMemsetAsync(P->tan.p, 0, P->tan.bytes(), ctx->stream));
     BHIP_TRY(ctx, hipMemsetAsync(P->du.p, 0, P->du.bytes(), ctx->stream));
     BHIP_TRY(ctx, hipMemsetAsync(P->utot.p, 0, P->utot.bytes(), ctx->stream));
-    BHIP_TRY(ctx, hipMemsetAsync(P->cnt.p, 0, sizeof(unsigned), ctx->stream));
+    BHIP_TRY(ctx, hipMemsetAsync(P->cnt.p, 0, 3 * sizeof(unsigned), ctx->stream));
     BHIP_PROPAGATE(bhip_ensure_vectors(ctx));
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return BHIP_OK;
@@ -473,21 +495,20 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
     S.newton_iters = it;
     S.max_residual = maxres;
     BHIP_PROPAGATE(bhip_plastic_commit(ctx));
+    // plastic points and return-map iterations of the converged state
+    long long npts = (long long)ctx->n_el * P->nip;
+    BHIP_TRY(ctx, hipMemsetAsync(P->cnt.p + 1, 0, 2 * sizeof(unsigned), s));
+    k_isv_stats<<<grid_for(npts, 256, ctx->n_cu * 8), 256, 0, s>>>(npts, P->isv.p, P->cnt.p + 1);
+    BHIP_TRY(ctx, hipGetLastError());
+    unsigned isv_stats[2];
+    BHIP_TRY(ctx, hipMemcpyAsync(isv_stats, P->cnt.p + 1, sizeof(isv_stats), hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipEventRecord(ctx->drv1, s));
     BHIP_TRY(ctx, hipEventSynchronize(ctx->drv1));
     float ms = 0;
     BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->drv0, ctx->drv1));
     S.ms = ms;
-    // plastic points and return-map iterations of the converged state
-    long long npts = (long long)ctx->n_el * P->nip;
-    {
-        std::vector<double> rec(npts * kIsv);
-        BHIP_TRY(ctx, bhip_copy(ctx, rec.data(), P->isv.p, sizeof(double) * npts * kIsv, hipMemcpyDeviceToHost));
-        for (long long t = 0; t < npts; t++) {
-            S.plastic_points += rec[t * kIsv + 14] != 0.0;
-            S.max_return_iters = std::max(S.max_return_iters, (int)rec[t * kIsv + 15]);
-        }
-    }
+    S.plastic_points = isv_stats[0];
+    S.max_return_iters = (int)isv_stats[1];
     if (stats) *stats = S;
     if (!(maxres < 2.0 * max_allowable_residual))
         return bhip_fail(ctx, BHIP_ERR_NOT_CONVERGED, "Newton iterations exhausted in the load increment");

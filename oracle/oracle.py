@@ -402,7 +402,8 @@ def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, s
     when max|residual| < 2*max_allowable_residual (checked from the second
     pass on) or after max_iterations extra solves; then "U".  Returns the
     accumulated displacement, the Gauss-point records and per-increment
-    (newton solves, max residual).  presc_list optionally gives the
+    (newton solves, max residual, plastic points, largest return-map
+    iteration count of the committed records).  presc_list optionally gives the
     prescribed increment vector of every increment (default: prob.prescribed)."""
     L = _plastic_sigs()
     n = prob.n_dof
@@ -454,5 +455,5 @@ def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, s
         for e in range(prob.n_el):
             L.orc_plastic_commit(d(isv[e]), nip)
         utot += du
-        hist.append((it, maxres))
+        hist.append((it, maxres, int((isv[:, :, 14] != 0).sum()), int(isv[:, :, 15].max())))
     return utot, isv, hist

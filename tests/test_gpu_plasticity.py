@@ -87,6 +87,10 @@ def test_plastic_increments_match_oracle(hip_ctx_factory, n, jitter):
     assert np.abs(epsb - isv[:, :, 6]).max() <= 1e-8 * np.abs(isv[:, :, 6]).max()
     assert all(abs(h["newton_iters"] - ho[0]) <= 1 for h, ho in zip(hist, hist_o))
     assert hist[0]["plastic_points"] == 0 and hist[-1]["plastic_points"] == prob.n_el * 8
+    # per increment: plastic Gauss points exactly, the largest return-map
+    # iteration count within one (a point near its tolerance may take one more)
+    assert [h["plastic_points"] for h in hist] == [ho[2] for ho in hist_o]
+    assert all(abs(h["max_return_iters"] - ho[3]) <= 1 for h, ho in zip(hist, hist_o))
 
 
 def test_uniaxial_bar_matches_point_driver(hip_ctx_factory):
