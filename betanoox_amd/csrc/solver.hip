@@ -165,6 +165,10 @@ __global__ void __launch_bounds__(kThreads) k_csr_spmv(int n, const int *__restr
 // saves 2 of the 40 bytes a 3x3 block costs).  Lanes past the last node skip
 // the loop, so a padded entry's offset 0 is never gathered beyond the mesh.
 //
+// MODE 0: y = A x.  MODE 1 (PCG): y = A x and the p.q partials.  MODE 2 (the
+// FP64 residual of the refined solver, VT = double): y = b - A x and the
+// partials |y|^2, |b|^2, two per record.
+//
 // Tile shape (chosen per matrix by bhip_tune_spmv): SPB slices per workgroup
 // (one wave each, SPB = 4, 8 or 16), UNR block columns per loop trip, NT
 // streaming (nontemporal) loads of block values and columns, which are read
@@ -184,13 +188,13 @@ __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slice
                                                       const CT *__restrict__ scol,
                                                       const VT *__restrict__ sval, long long nsell,
                                                       const VT *__restrict__ x,
-                                                      VT *__restrict__ y, const PcgState *st,
-                                                      double *partials) {
+                                                      VT *__restrict__ y, const VT *__restrict__ b,
+                                                      const PcgState *st, double *partials) {
     if (MODE == 1 && st->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long s = (long long)lb * SPB + wave;
-    double acc = 0;
+    double acc = 0, acc_b = 0;
     if (s < n_slices) {
         const long long node = s * 64 + lane;
         const int kb = sptr[s], ke = node < n_nodes ? sptr[s + 1] : kb;
@@ -216,8 +220,15 @@ __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slice
         if (node < n_nodes) {
 #pragma unroll
             for (int c = 0; c < B; c++) {
-                y[B * node + c] = a[c];
-                if (MODE == 1) acc += (double)x[B * node + c] * a[c];
+                if (MODE == 2) {
+                    const VT bi = b[B * node + c], ri = bi - a[c];
+                    y[B * node + c] = ri;
+                    acc += (double)ri * (double)ri;
+                    acc_b += (double)bi * (double)bi;
+                } else {
+                    y[B * node + c] = a[c];
+                    if (MODE == 1) acc += (double)x[B * node + c] * a[c];
+                }
             }
         }
     }
@@ -232,6 +243,27 @@ __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slice
 #pragma unroll
             for (int w = 0; w < 4; w++) t += sm[4 * threadIdx.x + w];
             partials[(long long)lb * (SPB / 4) + threadIdx.x] = t;
+        }
+    } else if (MODE == 2) {
+        // the same groups of four slices, two values per record
+        __shared__ double sm[SPB][2];
+        acc = wave_sum(acc);
+        acc_b = wave_sum(acc_b);
+        if (lane == 0) {
+            sm[wave][0] = acc;
+            sm[wave][1] = acc_b;
+        }
+        __syncthreads();
+        if (threadIdx.x < SPB / 4) {
+            double t0 = 0, t1 = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                t0 += sm[4 * threadIdx.x + w][0];
+                t1 += sm[4 * threadIdx.x + w][1];
+            }
+            const long long g = (long long)lb * (SPB / 4) + threadIdx.x;
+            partials[2 * g] = t0;
+            partials[2 * g + 1] = t1;
         }
     }
 }
@@ -564,11 +596,13 @@ __global__ void k_max_offset(long long n_pairs, const int *pair_row, const int *
 }
 
 // FP64 CSR -> SELL-64 B x B blocks of VT (SoA); restrained entries zero,
-// restrained diagonal 1.  Pair p = (node n, k-th neighbour m).
+// restrained diagonal 1.  Pair p = (node n, k-th neighbour m).  sval64, when
+// given, receives the same blocks in FP64 in the same pass.
 template <int B, class CT, class VT>
 __global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
                               const int *nbr_off, const int *rowptr, const unsigned char *rmask,
-                              const double *val, const int *sptr, long long nsell, CT *scol, VT *sval) {
+                              const double *val, const int *sptr, long long nsell, CT *scol, VT *sval,
+                              double *sval64) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     int n = pair_row[p], m = nbr[p];
@@ -581,12 +615,14 @@ __global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int 
         int r = 0;
         long long base = (long long)rowptr[B * n + c] + nbr_off[p];
         for (int d = 0; d < B; d++) {
-            VT v = 0;
+            double v = 0;
             bool rc = (rn >> c) & 1u, rd = (rm >> d) & 1u;
-            if (!rc && !rd) v = (VT)val[base + r];
+            if (!rc && !rd) v = val[base + r];
             else if (rc && n == m && c == d) v = 1;
             if (!rd) r++;
-            sval[(long long)(c * B + d) * nsell + dst] = v;
+            const long long o = (long long)(c * B + d) * nsell + dst;
+            sval[o] = (VT)v;
+            if (sval64) sval64[o] = v;
         }
     }
 }
@@ -661,16 +697,17 @@ void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *
 }
 
 template <int B, int MODE, int SPB, int UNR, int NT, class VT>
-void launch_sell_t(bhip_ctx *ctx, const VT *sval, const VT *x, VT *y, const PcgState *st) {
+void launch_sell_t(bhip_ctx *ctx, const VT *sval, const VT *x, VT *y, const PcgState *st,
+                   const VT *b = nullptr) {
     long long g = (ctx->n_slices + SPB - 1) / SPB;
     const int grid = (int)(g > 0 ? g : 1);
     if (ctx->sell_narrow)
         k_sell_spmv<B, MODE, SPB, UNR, NT, short, VT><<<grid, 64 * SPB, 0, ctx->stream>>>(
-            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col16.p, sval, ctx->n_sell, x, y, st,
+            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col16.p, sval, ctx->n_sell, x, y, b, st,
             ctx->partials.p);
     else
         k_sell_spmv<B, MODE, SPB, UNR, NT, int, VT><<<grid, 64 * SPB, 0, ctx->stream>>>(
-            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p, sval, ctx->n_sell, x, y, st,
+            ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col.p, sval, ctx->n_sell, x, y, b, st,
             ctx->partials.p);
 }
 
@@ -716,6 +753,16 @@ void launch_sell64(bhip_ctx *ctx, int mode, const double *x, double *y, const Pc
     else launch_sell_b<1>(ctx, mode, shape, v, x, y, st);
 }
 
+// FP64 residual r = b - K x on the FP64 block values (bval64): one streaming
+// shape (4 slices per workgroup, 4 block columns per trip), group records of
+// |r|^2 and |b|^2 for k_fin_resid over sell_groups records.
+void launch_sell64_resid(bhip_ctx *ctx, const double *x, double *r, const double *b) {
+    const double *v = ctx->bval64.p;
+    if (ctx->sell_b == 3) launch_sell_t<3, 2, 4, 4, 1>(ctx, v, x, r, nullptr, b);
+    else if (ctx->sell_b == 2) launch_sell_t<2, 2, 4, 4, 1>(ctx, v, x, r, nullptr, b);
+    else launch_sell_t<1, 2, 4, 4, 1>(ctx, v, x, r, nullptr, b);
+}
+
 // Algorithmic bytes of one SpMV (values + indices + row pointers + x + y);
 // for the blocked form the stored (padded) blocks are counted.
 double sell_bytes(bhip_ctx *ctx, int value_bytes) {
@@ -740,7 +787,10 @@ size_t fused_offset(bhip_ctx *ctx) { return ((size_t)sell_groups(ctx) + 4 + 63) 
 int ensure_partials(bhip_ctx *ctx) {
     size_t need = (size_t)kRedBlocks * 3;
     need = std::max(need, (size_t)csr_grid(ctx) * 2);
-    if (ctx->n_slices > 0) need = std::max(need, fused_offset(ctx) + (size_t)kRedBlocks * 3);
+    if (ctx->n_slices > 0) {
+        need = std::max(need, fused_offset(ctx) + (size_t)kRedBlocks * 3);
+        need = std::max(need, (size_t)sell_groups(ctx) * 2);
+    }
     if (ctx->partials.n < need) {
         bhip_invalidate_graphs(ctx);
         BHIP_TRY(ctx, ctx->partials.resize(need));
@@ -837,13 +887,14 @@ int ensure_sell_layout(bhip_ctx *ctx) {
 
 // CSR values -> SELL block values of VT (columns rewritten too, same values).
 template <class VT>
-void enqueue_to_sell(bhip_ctx *ctx, VT *sval) {
+void enqueue_to_sell(bhip_ctx *ctx, VT *sval, double *sval64 = nullptr) {
     const int B = 256, nb = ctx->sell_b;
     const int grid = grid_for(ctx->n_pairs, B);
 #define BHIP_TO_SELL(BB, CT, COL)                                                                                   \
     k_csr_to_sell<BB, CT, VT><<<grid, B, 0, ctx->stream>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, \
                                                            ctx->nbr_off.p, ctx->rowptr.p, ctx->node_rmask.p,          \
-                                                           ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, COL, sval)
+                                                           ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, COL, sval, \
+                                                           sval64)
     if (ctx->sell_narrow) {
         if (nb == 3) BHIP_TO_SELL(3, short, ctx->sell_col16.p);
         else if (nb == 2) BHIP_TO_SELL(2, short, ctx->sell_col16.p);
@@ -856,31 +907,53 @@ void enqueue_to_sell(bhip_ctx *ctx, VT *sval) {
 #undef BHIP_TO_SELL
 }
 
-// FP64 block values for BHIP_PREC_FP64_BLOCKED.
-int prepare_f64b(bhip_ctx *ctx) {
-    if (ctx->f64b_ready) return BHIP_OK;
-    if (!bsr_eligible(ctx))
-        return bhip_fail(ctx, BHIP_ERR_ARG, "FP64 blocked precision needs the same 1-3 DOFs at every node");
-    BHIP_PROPAGATE(ensure_sell_layout(ctx));
+// FP64 block array sized and padded for the current SELL layout.
+int ensure_bval64(bhip_ctx *ctx) {
     if (!ctx->bval64_layout_ok) {
         bhip_invalidate_graphs(ctx);
         BHIP_TRY(ctx, ctx->bval64.resize((size_t)ctx->sell_b * ctx->sell_b * (ctx->n_sell > 0 ? ctx->n_sell : 1)));
         BHIP_TRY(ctx, hipMemsetAsync(ctx->bval64.p, 0, ctx->bval64.bytes(), ctx->stream));
         ctx->bval64_layout_ok = true;
     }
+    return BHIP_OK;
+}
+
+// The refined solver (bhip_solve) computes its FP64 residuals on FP64 block
+// values, written in the same pass as the FP32 ones, whenever the matrix has
+// the blocked form; BHIP_SELL_RESIDUAL=0 keeps them on the CSR values.
+bool sell_residual_enabled() {
+    const char *e = getenv("BHIP_SELL_RESIDUAL");
+    return !(e && e[0] == '0');
+}
+
+// FP64 block values for BHIP_PREC_FP64_BLOCKED.
+int prepare_f64b(bhip_ctx *ctx) {
+    if (ctx->f64b_ready) return BHIP_OK;
+    if (!bsr_eligible(ctx))
+        return bhip_fail(ctx, BHIP_ERR_ARG, "FP64 blocked precision needs the same 1-3 DOFs at every node");
+    BHIP_PROPAGATE(ensure_sell_layout(ctx));
+    BHIP_PROPAGATE(ensure_bval64(ctx));
     enqueue_to_sell<double>(ctx, ctx->bval64.p);
     BHIP_TRY(ctx, hipGetLastError());
     ctx->f64b_ready = true;
     return BHIP_OK;
 }
 
-int prepare_f32(bhip_ctx *ctx) {
+// FP32 values of the inner solve; with64 also writes the FP64 block values in
+// the same pass when the blocked form applies and they are not current.
+int prepare_f32(bhip_ctx *ctx, bool with64 = false) {
     if (ctx->f32_ready) return BHIP_OK;
     const int B = 256;
     ctx->use_bsr = bsr_eligible(ctx);
     if (ctx->use_bsr) {
         BHIP_PROPAGATE(ensure_sell_layout(ctx));
-        enqueue_to_sell<float>(ctx, ctx->bval32.p);
+        double *v64 = nullptr;
+        if (with64 && !ctx->f64b_ready) {
+            BHIP_PROPAGATE(ensure_bval64(ctx));
+            v64 = ctx->bval64.p;
+        }
+        enqueue_to_sell<float>(ctx, ctx->bval32.p, v64);
+        if (v64) ctx->f64b_ready = true;
         if (ctx->val32.p) {
             bhip_invalidate_graphs(ctx);
             ctx->val32.release();
@@ -1004,9 +1077,17 @@ int run_iterations(bhip_ctx *ctx, int kind, int check_every, int *spmv_launches)
     return BHIP_OK;
 }
 
-// r = b - K x in FP64 with |r|^2, |b|^2 into the state (rr, bb).
-void enqueue_residual(bhip_ctx *ctx, double *r) {
+// r = b - K x in FP64 with |r|^2, |b|^2 into the state (rr, bb): on the FP64
+// block values (bval64, which must be ready) when sell is set, else on the
+// CSR values.  Both sum every row's products in FP64; the block form reads
+// 8 B + 2/9 B per stored value against 12 B per CSR entry.
+void enqueue_residual(bhip_ctx *ctx, double *r, bool sell = false) {
     PcgState *st = ctx->state.p;
+    if (sell) {
+        launch_sell64_resid(ctx, ctx->x.p, r, ctx->rhs.p);
+        k_fin_resid<<<1, kFinThreads, 0, ctx->stream>>>(sell_groups(ctx), ctx->partials.p, st);
+        return;
+    }
     launch_csr<double, double, 2>(ctx, ctx->val.p, ctx->x.p, r, ctx->rhs.p, st);
     k_fin_resid<<<1, kFinThreads, 0, ctx->stream>>>(csr_grid(ctx), ctx->partials.p, st);
 }
@@ -1071,9 +1152,19 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
     memset(&S, 0, sizeof(S));
 
     BHIP_PROPAGATE(prepare_diag(ctx));
-    if (opts.precision == BHIP_PREC_FP32_IR) BHIP_PROPAGATE(prepare_f32(ctx));
-    else if (opts.precision == BHIP_PREC_FP64_BLOCKED) BHIP_PROPAGATE(prepare_f64b(ctx));
-    else if (opts.precision != BHIP_PREC_FP64) return bhip_fail(ctx, BHIP_ERR_ARG, "unknown precision");
+    bool sell_resid = false;
+    if (opts.precision == BHIP_PREC_FP32_IR) {
+        const bool want64 = sell_residual_enabled();
+        BHIP_PROPAGATE(prepare_f32(ctx, want64));
+        if (want64 && ctx->use_bsr) {
+            BHIP_PROPAGATE(prepare_f64b(ctx));
+            sell_resid = true;
+        }
+    } else if (opts.precision == BHIP_PREC_FP64_BLOCKED) {
+        BHIP_PROPAGATE(prepare_f64b(ctx));
+    } else if (opts.precision != BHIP_PREC_FP64) {
+        return bhip_fail(ctx, BHIP_ERR_ARG, "unknown precision");
+    }
     BHIP_PROPAGATE(ensure_partials(ctx));
 
     BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
@@ -1081,6 +1172,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
     else BHIP_TRY(ctx, hipMemsetAsync(ctx->x.p, 0, sizeof(double) * n, s));
 
     int st_code = BHIP_OK;
+    bool final_resid = true;
     if (opts.precision == BHIP_PREC_FP64 || opts.precision == BHIP_PREC_FP64_BLOCKED) {
         // r = b - K x0 (sparse.cpp:152-163), z = p = r/diag, then the PCG loop
         // (on the CSR values or on the same values in SELL block form)
@@ -1106,7 +1198,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
         for (outer = 0; outer <= max_outer; outer++) {
             // outer residual in FP64: r = b - K x
             k_state_init<<<1, 1, 0, s>>>(st, opts.tol, opts.maxit, opts.stop_rule);
-            enqueue_residual(ctx, ctx->r.p);
+            enqueue_residual(ctx, ctx->r.p, sell_resid);
             S.spmv_launches++;
             BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
             BHIP_TRY(ctx, hipStreamSynchronize(s));
@@ -1134,6 +1226,8 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
         S.ref_norm = 0;
         if (!ok) st_code = BHIP_ERR_NOT_CONVERGED;
         S.spmv_bytes = spmv_bytes(ctx, true, ctx->use_bsr);
+        // a converged refinement ends on the true residual of the returned x
+        final_resid = !ok;
     }
     BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
     BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
@@ -1142,11 +1236,13 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
     S.ms_total = ms;
 
     // true residual |b - K x| / |b| in FP64
-    enqueue_residual(ctx, ctx->tmp.p);
-    BHIP_TRY(ctx, hipGetLastError());
-    BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    BHIP_TRY(ctx, hipStreamSynchronize(s));
-    S.rel_resid = ctx->h_state->bb > 0 ? sqrt(ctx->h_state->rr / ctx->h_state->bb) : sqrt(ctx->h_state->rr);
+    if (final_resid) {
+        enqueue_residual(ctx, ctx->tmp.p, sell_resid);
+        BHIP_TRY(ctx, hipGetLastError());
+        BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        BHIP_TRY(ctx, hipStreamSynchronize(s));
+        S.rel_resid = ctx->h_state->bb > 0 ? sqrt(ctx->h_state->rr / ctx->h_state->bb) : sqrt(ctx->h_state->rr);
+    }
     if (stats) *stats = S;
     if (st_code == BHIP_ERR_NOT_CONVERGED)
         return bhip_fail(ctx, st_code, "Solution not converged within the iteration cap");
