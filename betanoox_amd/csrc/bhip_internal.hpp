@@ -139,11 +139,15 @@ struct bhip_ctx {
     dvec<int> sell_ptr, sell_col;  // slice starts; block columns (node index)
     dvec<short> sell_col16;        // block columns as offsets from the row's node (narrow form)
     bool sell_narrow = false;      // 16-bit offsets in use
-    dvec<float> bval32;      // SELL block values, SoA: B*B x n_sell
+    // Column runs (one DOF per node only): an entry is a run of three
+    // consecutive columns (16-bit offset of the first from the row's node) with
+    // three values, so a 27-column hex8 row takes 9 entries.
+    bool sell_runs = false;
+    dvec<float> bval32;      // SELL block values, SoA: sell_vals(ctx) x n_sell
     // ---- FP64 values in the same SELL layout (BHIP_PREC_FP64_BLOCKED)
     bool f64b_ready = false;       // bval64 holds the current matrix
     bool bval64_layout_ok = false; // bval64 sized and zero-padded for the current layout
-    dvec<double> bval64;           // SELL block values, SoA: B*B x n_sell
+    dvec<double> bval64;           // SELL block values, SoA: sell_vals(ctx) x n_sell
     bool sell_layout_ok = false;  // SELL pointers/columns built for the current pattern
     int sell_shape = 0;      // tile shape id of the FP32 SELL SpMV
     bool sell_shape_set = false;  // chosen by bhip_tune_spmv / bhip_set_spmv_shape (else the
@@ -194,6 +198,9 @@ struct bhip_ctx {
     dvec<unsigned long long> tmp_k1, tmp_k2;  // node-pair keys
     dvec<long long> tmp_l1, tmp_l2;    // per-DOF lengths and scans
 };
+
+// Values per stored SELL entry: B*B for node blocks, 3 for column runs.
+inline int sell_vals(const bhip_ctx *ctx) { return ctx->sell_runs ? 3 : ctx->sell_b * ctx->sell_b; }
 
 // Blocking copy ordered on the context's stream.  Context streams are
 // non-blocking, so the legacy-stream hipMemcpy would neither wait for the

@@ -182,7 +182,10 @@ __device__ __forceinline__ T ld_stream(const T *p) {
     else return *p;
 }
 
-template <int B, int MODE, int SPB, int UNR, int NT, class CT, class VT>
+// RUN (B = 1, CT = short): column runs; entry k holds the offset of the first
+// of three consecutive columns and their values at sval[d * nsell + k], d =
+// 0..2, summed as one 1 x 3 block: v0 x[m] + v1 x[m+1] + v2 x[m+2].
+template <int B, int MODE, int SPB, int UNR, int NT, class CT, class VT, bool RUN = false>
 __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slices,
                                                       const int *__restrict__ sptr,
                                                       const CT *__restrict__ scol,
@@ -201,11 +204,19 @@ __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slice
         VT a[B];
 #pragma unroll
         for (int c = 0; c < B; c++) a[c] = 0;
+        static_assert(!RUN || (B == 1 && sizeof(CT) == 2), "column runs are a one-DOF, 16-bit form");
 #pragma unroll UNR
         for (int k = kb + lane; k < ke; k += 64) {
             long long m;
             if constexpr (sizeof(CT) == 2) m = node + (long long)ld_stream<NT>(scol + k);
             else m = ld_stream<NT>(scol + k);
+            if constexpr (RUN) {
+                VT t = ld_stream<NT>(sval + k) * x[m];
+                t += ld_stream<NT>(sval + nsell + k) * x[m + 1];
+                t += ld_stream<NT>(sval + 2 * nsell + k) * x[m + 2];
+                a[0] += t;
+                continue;
+            }
             VT xv[B];
 #pragma unroll
             for (int d = 0; d < B; d++) xv[d] = x[B * m + d];
@@ -595,6 +606,86 @@ __global__ void k_max_offset(long long n_pairs, const int *pair_row, const int *
     if ((threadIdx.x & 63) == 0) atomicMax(out, v);
 }
 
+// Column runs of a row: its sorted columns covered greedily by windows of
+// three consecutive columns, each window starting at the smallest column not
+// yet covered.
+__device__ __forceinline__ int run_count(const int *nb, int deg) {
+    int t = 0;
+    for (int i = 0; i < deg;) {
+        const int c = nb[i];
+        t++;
+        while (i < deg && nb[i] <= c + 2) i++;
+    }
+    return t;
+}
+
+// Slice widths of the column-run form: one thread per node, one slice per
+// wave (blockDim a multiple of 64), width = 64 x the slice's largest count.
+__global__ void k_run_width(int n_nodes, int n_slices, const int *nbr_ptr, const int *nbr, int *width) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long s = node / 64;
+    int t = 0;
+    if (node < n_nodes) t = run_count(nbr + nbr_ptr[node], nbr_ptr[node + 1] - nbr_ptr[node]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t = max(t, __shfl_xor(t, off, 64));
+    if ((threadIdx.x & 63) == 0 && s < n_slices) width[s] = t * 64;
+}
+
+// FP64 CSR -> column runs of VT (one DOF per node), one thread per node:
+// window w of node n is entry sptr[n/64] + 64 w + n%64, its values at
+// d * nsell + entry for columns base + d.  A window whose three columns would
+// pass the last node starts at n_nodes - 3 instead (n_nodes >= 3); padded
+// windows read the node's own columns (from n - 2 for the last two nodes)
+// with zero values, so every gather stays inside the vector.  Restrained
+// entries zero, restrained diagonal 1, as in k_csr_to_sell; the CSR row of a
+// free node lists its free neighbours in order.  sval64, when given,
+// receives the same values in FP64.
+template <class VT>
+__global__ void k_csr_to_runs(int n_nodes, const int *nbr_ptr, const int *nbr, const int *rowptr,
+                              const unsigned char *rmask, const double *val, const int *sptr, long long nsell,
+                              short *scol, VT *sval, double *sval64) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const long long s = n / 64, l = n % 64;
+    const int w_end = (sptr[s + 1] - sptr[s]) / 64;
+    const int *nb = nbr + nbr_ptr[n];
+    const int deg = nbr_ptr[n + 1] - nbr_ptr[n];
+    const bool rn = rmask[n] & 1u;
+    long long pos = rowptr[n];
+    int w = 0;
+    for (int i = 0; i < deg; w++) {
+        const int c0 = nb[i];
+        const long long base = c0 + 2 >= n_nodes ? n_nodes - 3 : c0;
+        double v[3] = {0, 0, 0};
+        while (i < deg && nb[i] <= c0 + 2) {
+            const int m = nb[i];
+            const bool rm = rmask[m] & 1u;
+            double a = 0;
+            if (!rn && !rm) a = val[pos];
+            else if (rn && m == n) a = 1;
+            if (!rm) pos++;
+            v[m - base] = a;
+            i++;
+        }
+        const long long k = (long long)sptr[s] + 64LL * w + l;
+        scol[k] = (short)(base - n);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            sval[d * nsell + k] = (VT)v[d];
+            if (sval64) sval64[d * nsell + k] = v[d];
+        }
+    }
+    for (; w < w_end; w++) {
+        const long long k = (long long)sptr[s] + 64LL * w + l;
+        scol[k] = (short)(n + 2 >= n_nodes ? -2 : 0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            sval[d * nsell + k] = 0;
+            if (sval64) sval64[d * nsell + k] = 0;
+        }
+    }
+}
+
 // FP64 CSR -> SELL-64 B x B blocks of VT (SoA); restrained entries zero,
 // restrained diagonal 1.  Pair p = (node n, k-th neighbour m).  sval64, when
 // given, receives the same blocks in FP64 in the same pass.
@@ -701,6 +792,14 @@ void launch_sell_t(bhip_ctx *ctx, const VT *sval, const VT *x, VT *y, const PcgS
                    const VT *b = nullptr) {
     long long g = (ctx->n_slices + SPB - 1) / SPB;
     const int grid = (int)(g > 0 ? g : 1);
+    if constexpr (B == 1) {
+        if (ctx->sell_runs) {
+            k_sell_spmv<1, MODE, SPB, UNR, NT, short, VT, true><<<grid, 64 * SPB, 0, ctx->stream>>>(
+                ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col16.p, sval, ctx->n_sell, x, y, b, st,
+                ctx->partials.p);
+            return;
+        }
+    }
     if (ctx->sell_narrow)
         k_sell_spmv<B, MODE, SPB, UNR, NT, short, VT><<<grid, 64 * SPB, 0, ctx->stream>>>(
             ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_col16.p, sval, ctx->n_sell, x, y, b, st,
@@ -766,7 +865,7 @@ void launch_sell64_resid(bhip_ctx *ctx, const double *x, double *r, const double
 // Algorithmic bytes of one SpMV (values + indices + row pointers + x + y);
 // for the blocked form the stored (padded) blocks are counted.
 double sell_bytes(bhip_ctx *ctx, int value_bytes) {
-    return (double)ctx->n_sell * (ctx->sell_b * ctx->sell_b * value_bytes + (ctx->sell_narrow ? 2 : 4)) +
+    return (double)ctx->n_sell * (sell_vals(ctx) * value_bytes + (ctx->sell_narrow ? 2 : 4)) +
            (ctx->n_slices + 1.0) * 4 + 2.0 * ctx->n_dof * value_bytes;
 }
 
@@ -819,30 +918,37 @@ int tuned_default_shape(int b, long long n_slices) {
     return 0;
 }
 
-int build_sell(bhip_ctx *ctx) {
+// Slice widths (x64) into ctx->tmp_i by the given kernel, returned on the host.
+int slice_widths(bhip_ctx *ctx, bool runs, std::vector<int> &hw) {
     hipStream_t s = ctx->stream;
-    ctx->n_slices = (ctx->n_nodes + 63) / 64;
     const int ns = ctx->n_slices;
     dvec<int> &width = ctx->tmp_i;
     BHIP_TRY(ctx, width.resize(ns + 1));
     BHIP_TRY(ctx, hipMemsetAsync(width.p, 0, sizeof(int) * (ns + 1), s));
-    k_sell_width<<<grid_for(ns, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, width.p);
+    if (runs)
+        k_run_width<<<grid_for((long long)ns * 64, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, ctx->nbr.p,
+                                                                      width.p);
+    else
+        k_sell_width<<<grid_for(ns, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, width.p);
     BHIP_TRY(ctx, hipGetLastError());
-    std::vector<int> hw(ns + 1);
+    hw.assign(ns + 1, 0);
     BHIP_TRY(ctx, hipMemcpyAsync(hw.data(), width.p, sizeof(int) * (ns + 1), hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipStreamSynchronize(s));
-    std::vector<int> hp(ns + 1);
-    long long tot = 0;
-    for (int i = 0; i < ns; i++) {
-        hp[i] = (int)tot;
-        tot += hw[i];
-    }
-    if (tot >= (1LL << 31)) return bhip_fail(ctx, BHIP_ERR_CAPACITY, "blocked matrix too large for int32 offsets");
-    hp[ns] = (int)tot;
-    ctx->n_sell = tot;
-    bhip_invalidate_graphs(ctx);
-    BHIP_TRY(ctx, ctx->sell_ptr.resize(ns + 1));
-    BHIP_TRY(ctx, hipMemcpyAsync(ctx->sell_ptr.p, hp.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
+    return BHIP_OK;
+}
+
+// Column runs replace the scalar form of a one-DOF layout when they store at
+// least 10% fewer bytes (3 x 4 + 2 B per run against 4 + 2 B per entry) and
+// every run offset fits 16 bits; BHIP_SELL_RUNS=0 keeps the scalar form.
+bool runs_enabled() {
+    const char *e = getenv("BHIP_SELL_RUNS");
+    return !(e && e[0] == '0');
+}
+
+int build_sell(bhip_ctx *ctx) {
+    hipStream_t s = ctx->stream;
+    ctx->n_slices = (ctx->n_nodes + 63) / 64;
+    const int ns = ctx->n_slices;
     // 16-bit column offsets when every neighbour lies within 32767 nodes of its row's node
     BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p + 3, 0, sizeof(int), s));
     k_max_offset<<<grid_for(ctx->n_pairs, 256, 4096), 256, 0, s>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p,
@@ -852,17 +958,49 @@ int build_sell(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipMemcpyAsync(&max_off, ctx->flags.p + 3, sizeof(int), hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipStreamSynchronize(s));
     const char *wide = getenv("BHIP_SELL_WIDE_COLUMNS");
-    ctx->sell_narrow = max_off <= 32767 && !(wide && wide[0] == '1');
+    const bool narrow = max_off <= 32767 && !(wide && wide[0] == '1');
+    std::vector<int> hw;
+    BHIP_PROPAGATE(slice_widths(ctx, false, hw));
+    long long tot = 0;
+    for (int i = 0; i < ns; i++) tot += hw[i];
+    bool runs = false;
+    if (ctx->sell_b == 1 && narrow && max_off + 2 <= 32767 && ctx->n_nodes >= 3 && runs_enabled()) {
+        std::vector<int> hr;
+        BHIP_PROPAGATE(slice_widths(ctx, true, hr));
+        long long tr = 0;
+        for (int i = 0; i < ns; i++) tr += hr[i];
+        if (14.0 * (double)tr <= 0.9 * 6.0 * (double)tot) {
+            runs = true;
+            hw.swap(hr);
+            tot = tr;
+        }
+    }
+    std::vector<int> hp(ns + 1);
+    tot = 0;
+    for (int i = 0; i < ns; i++) {
+        hp[i] = (int)tot;
+        tot += hw[i];
+    }
+    if (tot >= (1LL << 31)) return bhip_fail(ctx, BHIP_ERR_CAPACITY, "blocked matrix too large for int32 offsets");
+    hp[ns] = (int)tot;
+    ctx->n_sell = tot;
+    ctx->sell_runs = runs;
+    ctx->sell_narrow = narrow;
+    bhip_invalidate_graphs(ctx);
+    BHIP_TRY(ctx, ctx->sell_ptr.resize(ns + 1));
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->sell_ptr.p, hp.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
     if (ctx->sell_narrow) BHIP_TRY(ctx, ctx->sell_col16.resize(tot > 0 ? tot : 1));
     else BHIP_TRY(ctx, ctx->sell_col.resize(tot > 0 ? tot : 1));
-    BHIP_TRY(ctx, ctx->bval32.resize((size_t)ctx->sell_b * ctx->sell_b * (tot > 0 ? tot : 1)));
+    BHIP_TRY(ctx, ctx->bval32.resize((size_t)sell_vals(ctx) * (tot > 0 ? tot : 1)));
+    // padding of the block form (the run conversion writes its own, for every slot of a node)
     if (ctx->sell_narrow)
         k_sell_pad<short><<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col16.p);
     else
         k_sell_pad<int><<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
     BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
     BHIP_TRY(ctx, hipGetLastError());
-    if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
+    // column runs: 4 slices per workgroup, 4 runs per trip, streaming loads
+    if (!ctx->sell_shape_set) ctx->sell_shape = runs ? 9 : tuned_default_shape(ctx->sell_b, ctx->n_slices);
     ctx->sell_layout_ok = true;
     ctx->bval64_layout_ok = false;
     ctx->f32_ready = ctx->f64b_ready = false;
@@ -889,6 +1027,12 @@ int ensure_sell_layout(bhip_ctx *ctx) {
 template <class VT>
 void enqueue_to_sell(bhip_ctx *ctx, VT *sval, double *sval64 = nullptr) {
     const int B = 256, nb = ctx->sell_b;
+    if (ctx->sell_runs) {
+        k_csr_to_runs<VT><<<grid_for(ctx->n_nodes, B), B, 0, ctx->stream>>>(
+            ctx->n_nodes, ctx->nbr_ptr.p, ctx->nbr.p, ctx->rowptr.p, ctx->node_rmask.p, ctx->val.p, ctx->sell_ptr.p,
+            ctx->n_sell, ctx->sell_col16.p, sval, sval64);
+        return;
+    }
     const int grid = grid_for(ctx->n_pairs, B);
 #define BHIP_TO_SELL(BB, CT, COL)                                                                                   \
     k_csr_to_sell<BB, CT, VT><<<grid, B, 0, ctx->stream>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, \
@@ -911,7 +1055,7 @@ void enqueue_to_sell(bhip_ctx *ctx, VT *sval, double *sval64 = nullptr) {
 int ensure_bval64(bhip_ctx *ctx) {
     if (!ctx->bval64_layout_ok) {
         bhip_invalidate_graphs(ctx);
-        BHIP_TRY(ctx, ctx->bval64.resize((size_t)ctx->sell_b * ctx->sell_b * (ctx->n_sell > 0 ? ctx->n_sell : 1)));
+        BHIP_TRY(ctx, ctx->bval64.resize((size_t)sell_vals(ctx) * (ctx->n_sell > 0 ? ctx->n_sell : 1)));
         BHIP_TRY(ctx, hipMemsetAsync(ctx->bval64.p, 0, ctx->bval64.bytes(), ctx->stream));
         ctx->bval64_layout_ok = true;
     }
@@ -1320,30 +1464,74 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
     hipStream_t s = ctx->stream;
     PcgState *st = ctx->state.p;
     const long long n = ctx->n_dof;
-    k_to_f32<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->dinv.p, ctx->p32.p);
-    k_state_init<<<1, 1, 0, s>>>(st, 1.0, 1, 0);
+    // Each SpMV is timed inside inner PCG iterations (between the fused vector
+    // kernels, which leave the caches as a solve does), not back to back: a
+    // repeated SpMV keeps part of its matrix in the caches and favours plain
+    // loads, which lose to streaming loads in a solve.  The inner recurrence
+    // restarts every kIterChunk iterations from r = dinv (tolerance 0, so it
+    // never stops on its own) to stay far from convergence.
+    constexpr int kIterChunk = 8;
+    auto restart = [&]() {
+        k_scale_to_f32<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->dinv.p, 1.0, ctx->r32.p, ctx->d32.p);
+        k_state_init<<<1, 1, 0, s>>>(st, 0.0, 1 << 30, 2);
+        enqueue_init<float>(ctx, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->dinv32.p);
+    };
+    std::vector<hipEvent_t> evs(2 * (size_t)reps, nullptr);
+    struct EventsGuard {
+        std::vector<hipEvent_t> &v;
+        ~EventsGuard() {
+            for (hipEvent_t e : v)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } guard{evs};
+    for (hipEvent_t &e : evs) BHIP_TRY(ctx, hipEventCreate(&e));
+    const int gv = vec_grid(ctx, n);
+    double *p2 = ctx->partials.p + fused_offset(ctx);
+    int it = 0;
+    auto iteration = [&](hipEvent_t e0, hipEvent_t e1) -> hipError_t {
+        if (it % kIterChunk == 0) restart();
+        const int par = it & 1;
+        it++;
+        if (e0) {
+            hipError_t e = hipEventRecord(e0, s);
+            if (e != hipSuccess) return e;
+        }
+        launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
+        if (e1) {
+            hipError_t e = hipEventRecord(e1, s);
+            if (e != hipSuccess) return e;
+        }
+        k_pcg_update_fa<float><<<gv, kThreads, 0, s>>>(n, par, sell_groups(ctx), ctx->partials.p, ctx->d32.p,
+                                                       ctx->r32.p, ctx->p32.p, ctx->q32.p, ctx->dinv32.p, st, p2);
+        k_pcg_pupdate_fb<float><<<gv, kThreads, 0, s>>>(n, par, gv, p2, ctx->p32.p, ctx->r32.p, ctx->dinv32.p, st);
+        return hipSuccess;
+    };
     const int ns = kNumSellShapes;
     std::vector<double> best(ns, 1e300);
     const int keep = ctx->sell_shape;
     for (int id = 0; id < ns; id++) {
         ctx->sell_shape = id;
-        launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
+        it = 0;
+        BHIP_TRY(ctx, iteration(nullptr, nullptr));
     }
     for (int round = 0; round < 3; round++)
         for (int id = 0; id < ns; id++) {
             ctx->sell_shape = id;
-            BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-            for (int k = 0; k < reps; k++) launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
-            BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
-            BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+            it = 0;
+            for (int k = 0; k < reps; k++) BHIP_TRY(ctx, iteration(evs[2 * k], evs[2 * k + 1]));
+            BHIP_TRY(ctx, hipEventSynchronize(evs[2 * reps - 1]));
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) {
                 ctx->sell_shape = keep;
                 BHIP_TRY(ctx, e);
             }
-            float ms = 0;
-            BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            best[id] = std::min(best[id], (double)ms / reps);
+            double tot = 0;
+            for (int k = 0; k < reps; k++) {
+                float ms = 0;
+                BHIP_TRY(ctx, hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
+                tot += ms;
+            }
+            best[id] = std::min(best[id], tot / reps);
         }
     int pick = 0;
     for (int id = 1; id < ns; id++)

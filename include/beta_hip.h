@@ -216,11 +216,14 @@ typedef struct bhip_spmv_shape {
     int    column_bytes;     /* bytes per stored block column of the tuned matrix: 2 (16-bit
                                 offsets from the row's node) or 4 (node indices); 0 from
                                 bhip_spmv_shape_describe */
-    double ms;               /* best mean kernel time of the tuning rounds (HIP events) */
+    double ms;               /* best mean SpMV kernel time of the tuning rounds, timed with HIP
+                                events around each launch inside inner PCG iterations */
     double gbps;             /* algorithmic bytes / ms */
 } bhip_spmv_shape;
 /* Autotuner: times every shape on the context's assembled matrix (three
- * interleaved rounds of `reps` launches per shape), keeps the fastest for the
+ * interleaved rounds of `reps` SpMV launches per shape, each launch inside an
+ * inner PCG iteration so that the caches hold what a solve leaves in them; the
+ * iterations leave the context's solution untouched), keeps the fastest for the
  * context's later solves and reports the table (up to max_entries rows).
  * n_entries = 0 and chosen = -1 when the matrix has no blocked FP32 form
  * (DOFs per node not uniform or above 3). */
