@@ -12,7 +12,7 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
 --workload cases (RVE request, --n 64, six load cases):
   one six-case solve in the packed solver (k_pk_spmv<6, ...>, k_pk_update<6>, k_pk_pupdate<6>)
 --workload heat (diffusion, --n 128, 1 DOF per node):
-  reps x FP32 SELL-64 scalar SpMV         (k_sell_spmv<1, 1>)
+  reps x FP32 SELL-64 one-DOF SpMV        (k_sell_spmv<1, 1>, column runs)
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
   10 backward-Euler steps                 (unless --no-solve)
 
