@@ -76,8 +76,9 @@ def sweep(quick=False, reps=10):
                        "chosen": chosen, "table": table}
                 out["single"].append(rec)
                 best = table[chosen]
-                log("%s %d (B=%d, %d slices): shape %d %.1f us %.0f GB/s; shape 0 %.1f us"
-                    % (gen, n, b, n_slices, chosen, best["ms"] * 1e3, best["gbps"], table[0]["ms"] * 1e3))
+                log("%s %d (B=%d, %d slices): shape %d, iteration %.1f us, SpMV %.1f us %.0f GB/s; shape 0 %.1f / %.1f us"
+                    % (gen, n, b, n_slices, chosen, best["iter_ms"] * 1e3, best["ms"] * 1e3, best["gbps"],
+                       table[0]["iter_ms"] * 1e3, table[0]["ms"] * 1e3))
     o = capi.Context.opts(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
     for n in ([24, 64] if quick else RVE_SIZES):
         prob = meshgen.rve_request(n, 7)
@@ -112,7 +113,7 @@ def _rows(recs):
     rows = []
     cur = 0
     for r in recs:
-        ms = {t["id"]: t["ms"] for t in r["table"]}
+        ms = {t["id"]: t["iter_ms"] for t in r["table"]}
         best = min(ms.values())
         if ms[cur] > best * (1 + TIE) + TIE_US * 1e-3:
             cur = r["chosen"]
@@ -167,17 +168,19 @@ def emit_md(data, path):
     L = ["# SpMV tile-shape sweep (bhip_tune_spmv), gfx950", "",
          "Command (GPU): `python -m betanoox_amd.autotune --sweep autotune.json`; "
          "summary: `python -m betanoox_amd.autotune --emit-md autotune.json <this file>`.", "",
-         "Each shape: best of three interleaved rounds of %d launches (HIP events). "
-         "GB/s = algorithmic bytes (stored blocks x (4 B^2 + 4) + slice pointers + x + y) / time." % data["reps"], "",
-         "| generator | n | B | slices | fastest shape | us | GB/s | shape 0 us | shape 0 GB/s |",
+         "Each shape: best of three interleaved rounds of %d inner PCG iterations (HIP events around "
+         "the SpMV and around the whole iteration); the fastest iteration picks the shape. GB/s = "
+         "algorithmic bytes of the stored form (B x B blocks, or column runs of three for the one-DOF "
+         "diffusion rows: values + 16-bit offsets + slice pointers + x + y) / SpMV time." % data["reps"], "",
+         "| generator | n | B | slices | fastest shape | iteration us | SpMV us | GB/s | shape 0 iteration / SpMV us |",
          "|---|---|---|---|---|---|---|---|---|"]
     for r in data["single"]:
         t = r["table"][r["chosen"]]
         t0 = r["table"][0]
         desc = "%d (spb %d, unroll %d, nt %d)" % (t["id"], t["slices_per_block"], t["unroll"], t["nontemporal"])
-        L.append("| %s | %d | %d | %d | %s | %.1f | %.0f | %.1f | %.0f |"
-                 % (r["gen"], r["n"], r["b"], r["n_slices"], desc, t["ms"] * 1e3, t["gbps"], t0["ms"] * 1e3,
-                    t0["gbps"]))
+        L.append("| %s | %d | %d | %d | %s | %.1f | %.1f | %.0f | %.1f / %.1f |"
+                 % (r["gen"], r["n"], r["b"], r["n_slices"], desc, t["iter_ms"] * 1e3, t["ms"] * 1e3, t["gbps"],
+                    t0["iter_ms"] * 1e3, t0["ms"] * 1e3))
     if data["multi"]:
         L += ["", "Six-case RVE solves (bhip_solve_cases, FP32 inner + FP64 refinement to 1e-8), "
               "multi-case SpMV with plain or streaming loads (best of 3 solves):", "",

@@ -133,7 +133,7 @@ class PlasticStats(C.Structure):
 
 class SpmvShape(C.Structure):
     _fields_ = [("id", C.c_int), ("slices_per_block", C.c_int), ("unroll", C.c_int), ("nontemporal", C.c_int),
-                ("column_bytes", C.c_int), ("ms", C.c_double), ("gbps", C.c_double)]
+                ("column_bytes", C.c_int), ("ms", C.c_double), ("gbps", C.c_double), ("iter_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

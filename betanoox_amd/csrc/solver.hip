@@ -999,8 +999,8 @@ int build_sell(bhip_ctx *ctx) {
         k_sell_pad<int><<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
     BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
     BHIP_TRY(ctx, hipGetLastError());
-    // column runs: 4 slices per workgroup, 4 runs per trip, streaming loads
-    if (!ctx->sell_shape_set) ctx->sell_shape = runs ? 9 : tuned_default_shape(ctx->sell_b, ctx->n_slices);
+    // (the B = 1 rows of the table are measured on column runs, the form hex8 diffusion takes)
+    if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
     ctx->sell_layout_ok = true;
     ctx->bval64_layout_ok = false;
     ctx->f32_ready = ctx->f64b_ready = false;
@@ -1446,11 +1446,12 @@ int bhip_spmv_shape_info(int id, bhip_spmv_shape *info) {
 }
 
 // Times every SELL tile shape on the resident matrix.  Each shape gets one
-// warm-up launch, then three rounds over all shapes of `reps` launches each
-// (interleaved, so that clock ramps and neighbours affect all shapes alike);
-// a shape's time is its best round.  The fastest shape is kept for later
-// solves.  Shapes do not change results (see k_sell_spmv), so tuning never
-// changes a solve's bits.
+// warm-up iteration, then three rounds over all shapes of `reps` inner PCG
+// iterations each (interleaved, so that clock ramps and neighbours affect all
+// shapes alike); a shape's times are its best round.  The shape with the
+// fastest iteration (SpMV + fused vector kernels) is kept for later solves.
+// Shapes do not change results (see k_sell_spmv), so tuning never changes a
+// solve's bits.
 int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max_entries, int *n_entries,
                         int *chosen) {
     if (!ctx->assembled) return bhip_fail(ctx, BHIP_ERR_STATE, "matrix not assembled");
@@ -1464,19 +1465,21 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
     hipStream_t s = ctx->stream;
     PcgState *st = ctx->state.p;
     const long long n = ctx->n_dof;
-    // Each SpMV is timed inside inner PCG iterations (between the fused vector
-    // kernels, which leave the caches as a solve does), not back to back: a
-    // repeated SpMV keeps part of its matrix in the caches and favours plain
-    // loads, which lose to streaming loads in a solve.  The inner recurrence
-    // restarts every kIterChunk iterations from r = dinv (tolerance 0, so it
-    // never stops on its own) to stay far from convergence.
+    // Shapes are timed inside inner PCG iterations, and chosen by the time of
+    // the whole iteration: back-to-back SpMVs keep part of a matrix near the
+    // last-level cache size in that cache and favour plain loads, whose
+    // matrix lines then evict the vectors the fused kernels read (heat 128^3:
+    // the SpMV alone looks 2% faster with plain loads, the iteration is 6%
+    // slower).  The inner recurrence restarts every kIterChunk iterations from
+    // r = dinv (tolerance 0, so it never stops on its own) to stay far from
+    // convergence.
     constexpr int kIterChunk = 8;
     auto restart = [&]() {
         k_scale_to_f32<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->dinv.p, 1.0, ctx->r32.p, ctx->d32.p);
         k_state_init<<<1, 1, 0, s>>>(st, 0.0, 1 << 30, 2);
         enqueue_init<float>(ctx, ctx->d32.p, ctx->r32.p, ctx->p32.p, ctx->dinv32.p);
     };
-    std::vector<hipEvent_t> evs(2 * (size_t)reps, nullptr);
+    std::vector<hipEvent_t> evs(3 * (size_t)reps, nullptr);
     struct EventsGuard {
         std::vector<hipEvent_t> &v;
         ~EventsGuard() {
@@ -1488,60 +1491,61 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
     const int gv = vec_grid(ctx, n);
     double *p2 = ctx->partials.p + fused_offset(ctx);
     int it = 0;
-    auto iteration = [&](hipEvent_t e0, hipEvent_t e1) -> hipError_t {
+    // events a (before the SpMV), b (after it) and c (after the vector kernels)
+    auto iteration = [&](hipEvent_t a, hipEvent_t b, hipEvent_t c) -> hipError_t {
         if (it % kIterChunk == 0) restart();
         const int par = it & 1;
         it++;
-        if (e0) {
-            hipError_t e = hipEventRecord(e0, s);
-            if (e != hipSuccess) return e;
-        }
+        hipError_t e = hipSuccess;
+        if (a && (e = hipEventRecord(a, s)) != hipSuccess) return e;
         launch_sell(ctx, 1, ctx->p32.p, ctx->q32.p, st);
-        if (e1) {
-            hipError_t e = hipEventRecord(e1, s);
-            if (e != hipSuccess) return e;
-        }
+        if (b && (e = hipEventRecord(b, s)) != hipSuccess) return e;
         k_pcg_update_fa<float><<<gv, kThreads, 0, s>>>(n, par, sell_groups(ctx), ctx->partials.p, ctx->d32.p,
                                                        ctx->r32.p, ctx->p32.p, ctx->q32.p, ctx->dinv32.p, st, p2);
         k_pcg_pupdate_fb<float><<<gv, kThreads, 0, s>>>(n, par, gv, p2, ctx->p32.p, ctx->r32.p, ctx->dinv32.p, st);
+        if (c && (e = hipEventRecord(c, s)) != hipSuccess) return e;
         return hipSuccess;
     };
     const int ns = kNumSellShapes;
-    std::vector<double> best(ns, 1e300);
+    std::vector<double> best(ns, 1e300), best_it(ns, 1e300);
     const int keep = ctx->sell_shape;
     for (int id = 0; id < ns; id++) {
         ctx->sell_shape = id;
         it = 0;
-        BHIP_TRY(ctx, iteration(nullptr, nullptr));
+        BHIP_TRY(ctx, iteration(nullptr, nullptr, nullptr));
     }
     for (int round = 0; round < 3; round++)
         for (int id = 0; id < ns; id++) {
             ctx->sell_shape = id;
             it = 0;
-            for (int k = 0; k < reps; k++) BHIP_TRY(ctx, iteration(evs[2 * k], evs[2 * k + 1]));
-            BHIP_TRY(ctx, hipEventSynchronize(evs[2 * reps - 1]));
+            for (int k = 0; k < reps; k++) BHIP_TRY(ctx, iteration(evs[3 * k], evs[3 * k + 1], evs[3 * k + 2]));
+            BHIP_TRY(ctx, hipEventSynchronize(evs[3 * reps - 1]));
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) {
                 ctx->sell_shape = keep;
                 BHIP_TRY(ctx, e);
             }
-            double tot = 0;
+            double t_spmv = 0, t_it = 0;
             for (int k = 0; k < reps; k++) {
                 float ms = 0;
-                BHIP_TRY(ctx, hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
-                tot += ms;
+                BHIP_TRY(ctx, hipEventElapsedTime(&ms, evs[3 * k], evs[3 * k + 1]));
+                t_spmv += ms;
+                BHIP_TRY(ctx, hipEventElapsedTime(&ms, evs[3 * k], evs[3 * k + 2]));
+                t_it += ms;
             }
-            best[id] = std::min(best[id], tot / reps);
+            best[id] = std::min(best[id], t_spmv / reps);
+            best_it[id] = std::min(best_it[id], t_it / reps);
         }
     int pick = 0;
     for (int id = 1; id < ns; id++)
-        if (best[id] < best[pick]) pick = id;
+        if (best_it[id] < best_it[pick]) pick = id;
     const double by = spmv_bytes(ctx, true, true);
     for (int id = 0; id < ns && id < max_entries; id++) {
         bhip_spmv_shape_info(id, &table[id]);
         table[id].column_bytes = ctx->sell_narrow ? 2 : 4;
         table[id].ms = best[id];
         table[id].gbps = by / best[id] / 1e6;
+        table[id].iter_ms = best_it[id];
     }
     *n_entries = std::min(ns, std::max(max_entries, 0));
     *chosen = pick;

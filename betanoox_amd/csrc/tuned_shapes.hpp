@@ -24,8 +24,6 @@ constexpr TunedSellRow kTunedSell[] = {
     {2, 24617, 3},
     {2, 1LL << 40, 9},
     // B = 1, measured at n = 32, 48, 64, 80, 96, 112, 128, 160 (diffusion_cube)
-    {1, 2809, 11},
-    {1, 5969, 0},
-    {1, 27500, 2},
-    {1, 1LL << 40, 11},
+    {1, 27500, 0},
+    {1, 1LL << 40, 9},
 };

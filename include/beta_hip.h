@@ -219,11 +219,13 @@ typedef struct bhip_spmv_shape {
     double ms;               /* best mean SpMV kernel time of the tuning rounds, timed with HIP
                                 events around each launch inside inner PCG iterations */
     double gbps;             /* algorithmic bytes / ms */
+    double iter_ms;          /* best mean time of one inner PCG iteration (SpMV + fused vector
+                                kernels) with this shape; the tuner keeps the smallest */
 } bhip_spmv_shape;
 /* Autotuner: times every shape on the context's assembled matrix (three
- * interleaved rounds of `reps` SpMV launches per shape, each launch inside an
- * inner PCG iteration so that the caches hold what a solve leaves in them; the
- * iterations leave the context's solution untouched), keeps the fastest for the
+ * interleaved rounds of `reps` inner PCG iterations per shape, so that the
+ * caches hold what a solve leaves in them; the iterations leave the context's
+ * solution untouched), keeps the shape with the fastest iteration for the
  * context's later solves and reports the table (up to max_entries rows).
  * n_entries = 0 and chosen = -1 when the matrix has no blocked FP32 form
  * (DOFs per node not uniform or above 3). */

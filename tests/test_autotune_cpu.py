@@ -6,8 +6,9 @@ from betanoox_amd import autotune
 
 
 def _rec(b, n_slices, ms_by_id, gen="elastic_cube", n=0):
+    # iteration times: the SpMV time plus the same vector-kernel time for every shape
     table = [{"id": i, "slices_per_block": 4, "unroll": 4, "nontemporal": int(i >= 9), "column_bytes": 2,
-              "ms": ms, "gbps": 1.0} for i, ms in enumerate(ms_by_id)]
+              "ms": ms, "gbps": 1.0, "iter_ms": ms + 0.001} for i, ms in enumerate(ms_by_id)]
     chosen = min(range(len(ms_by_id)), key=lambda i: ms_by_id[i])
     return {"gen": gen, "b": b, "n": n or n_slices, "n_nodes": 64 * n_slices, "n_slices": n_slices,
             "chosen": chosen, "table": table}
