@@ -2,7 +2,8 @@
 
 Two levels:
   * at run time, bhip_tune_spmv (Context.tune_spmv) times every SELL tile
-    shape on a context's own matrix and keeps the fastest;
+    shape inside inner PCG iterations on a context's own matrix and keeps the
+    shape with the fastest iteration;
   * this module sweeps that tuner over the seeded workload generators (1, 2
     and 3 DOFs per node, a range of mesh sizes) and, for the packed SpMV of
     the multi-case and served path, times six-case solves with streaming
