@@ -1509,6 +1509,15 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
     const int ns = kNumSellShapes;
     std::vector<double> best(ns, 1e300), best_it(ns, 1e300);
     const int keep = ctx->sell_shape;
+    // an error return leaves the context on the shape it had
+    struct ShapeGuard {
+        bhip_ctx *c;
+        int shape;
+        bool armed = true;
+        ~ShapeGuard() {
+            if (armed) c->sell_shape = shape;
+        }
+    } shape_guard{ctx, keep};
     for (int id = 0; id < ns; id++) {
         ctx->sell_shape = id;
         it = 0;
@@ -1520,11 +1529,7 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
             it = 0;
             for (int k = 0; k < reps; k++) BHIP_TRY(ctx, iteration(evs[3 * k], evs[3 * k + 1], evs[3 * k + 2]));
             BHIP_TRY(ctx, hipEventSynchronize(evs[3 * reps - 1]));
-            hipError_t e = hipGetLastError();
-            if (e != hipSuccess) {
-                ctx->sell_shape = keep;
-                BHIP_TRY(ctx, e);
-            }
+            BHIP_TRY(ctx, hipGetLastError());
             double t_spmv = 0, t_it = 0;
             for (int k = 0; k < reps; k++) {
                 float ms = 0;
@@ -1550,6 +1555,7 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
     *n_entries = std::min(ns, std::max(max_entries, 0));
     *chosen = pick;
     if (pick != keep) bhip_invalidate_graphs(ctx);
+    shape_guard.armed = false;
     ctx->sell_shape = pick;
     ctx->sell_shape_set = true;
     return BHIP_OK;
