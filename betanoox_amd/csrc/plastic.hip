@@ -34,6 +34,7 @@ struct PlasticData {
     dvec<double> corr;    // Newton correction, n_dof
     dvec<double> red;     // reductions
     dvec<unsigned> cnt;
+    dvec<double> fe;      // element internal forces of the residual, 24 x n_el (entry-major)
 };
 
 namespace {
@@ -224,35 +225,55 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
     tr[7] = plastic;
 }
 
-// "F" + calculateResidualNorm: per node, resultant of B^T dstress over the
-// incident elements (ascending), residual = load - resultant on free DOFs.
-__global__ void k_plastic_residual(int n_nodes, int nip, const double *__restrict__ gtab, const int *n2e_ptr, const int *n2e,
-                                   const int *conn,
-                                   const double *xyz, const double *isv, const unsigned char *rmask,
-                                   const double *load, double *resid) {
+// "F", element stage: the internal force vector of element e,
+// fe = sum over Gauss points of B^T dstress detJ w (ElasticityElement3D::addToFe:
+// force += sum_j b[j][i] stress[j] * f), one thread per element, entry k of
+// element e at fe[k * n_el + e].
+__global__ void k_plastic_fe(int n_el, int nip, const double *__restrict__ gtab, const int *conn, const double *xyz,
+                             const double *isv, double *fe) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    int ec[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) ec[i] = conn[e * 8 + i];
+    double x[8], y[8], z[8];
+    load_coords(xyz, ec, x, y, z);
+    double f[24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) f[k] = 0;
+    for (int ip = 0; ip < nip; ip++) {
+        double g[8][3];
+        const double detJ = hex8_all_grads(gtab + kGaussDN, x, y, z, ip, g);
+        const double w = detJ * gtab[ip];
+        const double *s = isv + (e * nip + ip) * kIsv + 7;
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            f[3 * a + 0] += (g[a][0] * s[0] + g[a][1] * s[3] + g[a][2] * s[5]) * w;
+            f[3 * a + 1] += (g[a][1] * s[1] + g[a][0] * s[3] + g[a][2] * s[4]) * w;
+            f[3 * a + 2] += (g[a][2] * s[2] + g[a][1] * s[4] + g[a][0] * s[5]) * w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 24; k++) fe[(long long)k * n_el + e] = f[k];
+}
+
+// "F" assembled + calculateResidualNorm: per node, the resultant of its
+// incident elements' force vectors added in ascending element order (the
+// reference's element loop), residual = load - resultant on free DOFs.
+__global__ void k_plastic_residual(int n_nodes, int n_el, const int *n2e_ptr, const int *n2e, const int *conn,
+                                   const double *__restrict__ fe, const unsigned char *rmask, const double *load,
+                                   double *resid) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     double acc[3] = {0, 0, 0};
     for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
-        int e = n2e[k];
-        int ec[8], a = 0;
+        const long long e = n2e[k];
+        int a = 0;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            ec[i] = conn[(long long)e * 8 + i];
-            if (ec[i] == n) a = i;
-        }
-        double x[8], y[8], z[8];
-        load_coords(xyz, ec, x, y, z);
-        for (int ip = 0; ip < nip; ip++) {
-            double g[8][3];
-            double detJ = hex8_all_grads(gtab + kGaussDN, x, y, z, ip, g);
-            double f = detJ * gtab[ip];
-            const double *s = isv + ((long long)e * nip + ip) * kIsv + 7;
-            // ElasticityElement3D::addToFe: force += sum_j b[j][i] stress[j] * f
-            acc[0] += (g[a][0] * s[0] + g[a][1] * s[3] + g[a][2] * s[5]) * f;
-            acc[1] += (g[a][1] * s[1] + g[a][0] * s[3] + g[a][2] * s[4]) * f;
-            acc[2] += (g[a][2] * s[2] + g[a][1] * s[4] + g[a][0] * s[5]) * f;
-        }
+        for (int i = 0; i < 8; i++)
+            if (conn[e * 8 + i] == n) a = i;
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] += fe[(long long)(3 * a + c) * n_el + e];
     }
     unsigned rm = rmask[n];
     for (int c = 0; c < 3; c++) {
@@ -363,9 +384,11 @@ int plastic_update_dev(bhip_ctx *ctx, const double *du) {
 
 int plastic_residual_dev(bhip_ctx *ctx, double *max_abs) {
     PlasticData *P = ctx->plastic;
+    k_plastic_fe<<<grid_for(ctx->n_el, 128), 128, 0, ctx->stream>>>(ctx->n_el, P->nip, ctx->gauss.p, ctx->conn.p,
+                                                                     ctx->xyz.p, P->isv.p, P->fe.p);
     k_plastic_residual<<<grid_for(ctx->n_nodes, 128), 128, 0, ctx->stream>>>(
-        ctx->n_nodes, P->nip, ctx->gauss.p, ctx->n2e_ptr.p, ctx->n2e.p, ctx->conn.p, ctx->xyz.p, P->isv.p, ctx->node_rmask.p,
-        ctx->load.p, ctx->rhs.p);
+        ctx->n_nodes, ctx->n_el, ctx->n2e_ptr.p, ctx->n2e.p, ctx->conn.p, P->fe.p, ctx->node_rmask.p, ctx->load.p,
+        ctx->rhs.p);
     int g = std::min(1024, grid_for(ctx->n_dof, 256));
     k_max_abs<<<g, 256, 0, ctx->stream>>>(ctx->n_dof, ctx->rhs.p, P->red.p, P->cnt.p, P->red.p + 1024);
     BHIP_TRY(ctx, hipGetLastError());
@@ -386,6 +409,7 @@ void bhip_plastic_release(bhip_ctx *ctx) {
     P->du.release();
     P->utot.release();
     P->corr.release();
+    P->fe.release();
     P->red.release();
     P->cnt.release();
     delete P;
@@ -412,6 +436,7 @@ int bhip_plastic_init(bhip_ctx *ctx) {
     BHIP_TRY(ctx, P->utot.resize(ctx->n_dof));
     BHIP_TRY(ctx, P->corr.resize(ctx->n_dof));
     BHIP_TRY(ctx, P->red.resize(1025));
+    BHIP_TRY(ctx, P->fe.resize((size_t)24 * (ctx->n_el > 0 ? ctx->n_el : 1)));
     BHIP_TRY(ctx, P->cnt.resize(3));
     BHIP_TRY(ctx, hipMemsetAsync(P->isv.p, 0, P->isv.bytes(), ctx->stream));
     BHIP_TRY(ctx, hipMemsetAsync(P->tan.p, 0, P->tan.bytes(), ctx->stream));
