@@ -332,5 +332,8 @@ int bhip_scan_i64(bhip_ctx *ctx, const long long *in, long long *out, long long 
 int bhip_lift_build(bhip_ctx *ctx);
 int bhip_lift_rhs(bhip_ctx *ctx, int R, const double *presc, const double *load, double *b);
 int bhip_prepare_solver(bhip_ctx *ctx, bool fp32);
+int bhip_prepare_refined(bhip_ctx *ctx);
+void bhip_runs_from_csr64(bhip_ctx *ctx, const double *csr_vals, double *out);
+void bhip_runs_spmv64(bhip_ctx *ctx, const double *vals, const double *x, double *y);
 int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb);
 const double *bhip_plastic_tangent(bhip_ctx *ctx);

@@ -1261,6 +1261,29 @@ int bhip_prepare_solver(bhip_ctx *ctx, bool fp32) {
     return BHIP_OK;
 }
 
+// What bhip_solve prepares for BHIP_PREC_FP32_IR: Jacobi diagonal, FP32
+// values and, unless BHIP_SELL_RESIDUAL=0, the FP64 values of the residual.
+int bhip_prepare_refined(bhip_ctx *ctx) {
+    BHIP_PROPAGATE(prepare_diag(ctx));
+    const bool want64 = sell_residual_enabled();
+    BHIP_PROPAGATE(prepare_f32(ctx, want64));
+    if (want64 && ctx->use_bsr) BHIP_PROPAGATE(prepare_f64b(ctx));
+    return BHIP_OK;
+}
+
+// FP64 column-run values (3 x n_sell) of another CSR value array on the
+// context's pattern (the layout must be the run form).
+void bhip_runs_from_csr64(bhip_ctx *ctx, const double *csr_vals, double *out) {
+    k_csr_to_runs<double><<<grid_for(ctx->n_nodes, 256), 256, 0, ctx->stream>>>(
+        ctx->n_nodes, ctx->nbr_ptr.p, ctx->nbr.p, ctx->rowptr.p, ctx->node_rmask.p, csr_vals, ctx->sell_ptr.p,
+        ctx->n_sell, ctx->sell_col16.p, out, nullptr);
+}
+
+// y = V x on FP64 column-run values V (restrained rows: identity).
+void bhip_runs_spmv64(bhip_ctx *ctx, const double *vals, const double *x, double *y) {
+    launch_sell_t<1, 0, 4, 4, 1>(ctx, vals, x, y, nullptr);
+}
+
 // r = b - K x (FP64 CSR), returns |r|^2 and |b|^2 (synchronises the stream).
 int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb) {
     PcgState *st = ctx->state.p;
