@@ -5,7 +5,9 @@ columns, like the 27-column rows of a hex8 diffusion mesh in its natural node
 order, is stored as runs of three columns: one 16-bit offset and three values
 per run instead of an offset per value.  Checked here:
   - the structured diffusion mesh gets the run form: its FP32 SpMV moves
-    fewer bytes than the scalar form (BHIP_SELL_RUNS=0);
+    fewer bytes than the scalar form (BHIP_SELL_RUNS=0), and both byte counts
+    equal a restatement of the two layouts (greedy runs over each node's
+    sorted neighbours, slices as wide as their widest node);
   - the refined solve on the run form and on the scalar form both reach the
     plain FP64 solution (the FP32 inner products differ in the order of their
     sums, so the iterates differ in the last bits);
@@ -58,6 +60,37 @@ def _context(factory, prob, runs):
     return ctx, by
 
 
+def _layout_bytes(prob):
+    """Bytes of one FP32 SpMV in the run form and in the scalar form, from a
+    restatement of the layout: a node's sorted neighbour columns covered
+    greedily by runs of three consecutive columns; a 64-node slice is as wide
+    as its widest node; stored entries + 16-bit offsets + slice pointers + x, y."""
+    n = prob.n_nodes
+    nbr = [set() for _ in range(n)]
+    for el in prob.conn:
+        for a in el:
+            nbr[a].update(int(b) for b in el)
+    runs = np.zeros(n, np.int64)
+    deg = np.zeros(n, np.int64)
+    for i, cols in enumerate(nbr):
+        cols = sorted(cols)
+        deg[i] = len(cols)
+        k = 0
+        while k < len(cols):
+            c0 = cols[k]
+            runs[i] += 1
+            while k < len(cols) and cols[k] <= c0 + 2:
+                k += 1
+    ns = (n + 63) // 64
+    pad = np.zeros(ns * 64, np.int64)
+    def slots(per_node):
+        pad[:] = 0
+        pad[:n] = per_node
+        return int(pad.reshape(ns, 64).max(axis=1).sum()) * 64
+    tail = (ns + 1) * 4 + 2 * n * 4
+    return slots(runs) * 14 + tail, slots(deg) * 6 + tail
+
+
 def _solve(ctx, precision, tol=1e-8):
     st = ctx.solve(tol=tol, precision=precision, inner_tol=1e-3, maxit=20000)
     return st, ctx.solution()
@@ -73,8 +106,9 @@ def test_runs_chosen_and_solve(hip_ctx_factory, n):
     prob = meshgen.diffusion_cube(n)
     ctx_r, by_r = _context(hip_ctx_factory, prob, True)
     ctx_s, by_s = _context(hip_ctx_factory, prob, False)
-    # interior rows: 9 runs x 14 B against 27 entries x 6 B
+    # interior rows: 9 runs x 14 B against 27 entries x 6 B; both layouts as restated
     assert by_r < 0.85 * by_s, (by_r, by_s)
+    assert (by_r, by_s) == _layout_bytes(prob)
     st64, x64 = _solve(ctx_s, capi.PREC_FP64, tol=1e-12)
     for ctx in (ctx_r, ctx_s):
         st, x = _solve(ctx, capi.PREC_FP32_IR)
