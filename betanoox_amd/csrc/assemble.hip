@@ -583,7 +583,8 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     if (!ctx->pattern_ready) return bhip_fail(ctx, BHIP_ERR_STATE, "pattern not built");
     if (!ctx->have_mat) return bhip_fail(ctx, BHIP_ERR_STATE, "materials not set");
     BHIP_PROPAGATE(bhip_ensure_vectors(ctx));
-    bhip_invalidate_graphs(ctx);
+    // captured iteration graphs stay: new values go into the same buffers, and
+    // a graph whose buffers moved is re-captured by its key (solver.hip)
     ctx->lift_ready = false;
     int nip = 0;
     BHIP_PROPAGATE(upload_gauss(ctx, ctx->nnpe, ctx->order, &nip));

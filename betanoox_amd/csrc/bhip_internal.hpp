@@ -96,6 +96,7 @@ struct PcgState {
 constexpr int kMaxNnpe = 8;
 constexpr int kMaxIp = 64;    // order <= 4 in 3D
 constexpr int kRedBlocks = 2048;  // upper bound of grid-stride reduction grids
+constexpr int kGraphKeyWords = 40;  // words of a captured iteration graph's key
 
 struct bhip_ctx {
     int device = 0;
@@ -184,9 +185,13 @@ struct bhip_ctx {
     dvec<PcgState> state;
     dvec<int> flags;        // assembly / diagonal check flags
     PcgState *h_state = nullptr;  // pinned mirror
-    // ---- graphs
+    // ---- graphs: one captured chunk of iterations per iteration kind, with
+    // the addresses, sizes and shape it was captured with (graph_key in
+    // solver.hip); a chunk whose key still matches is replayed, so a
+    // reassembly into the same buffers keeps its graphs
     hipGraphExec_t graph64 = nullptr, graph32 = nullptr, graph64b = nullptr;
     int graph64_iters = 0, graph32_iters = 0, graph64b_iters = 0;
+    uint64_t graph64_key[kGraphKeyWords] = {}, graph32_key[kGraphKeyWords] = {}, graph64b_key[kGraphKeyWords] = {};
     double spmv_bytes64 = 0, spmv_bytes32 = 0;
     // ---- J2 plasticity state (plastic.hip)
     struct PlasticData *plastic = nullptr;

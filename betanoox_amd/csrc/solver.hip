@@ -1166,11 +1166,57 @@ bool graphs_enabled() {
     return !(e && e[0] == '1');
 }
 
+// Everything enqueue_iteration bakes into a captured graph: the stream, every
+// buffer address its kernels take, the sizes that set their grids and the
+// SpMV form and tile shape.  A graph is replayed only while its key matches,
+// so a buffer that moves (a grow-only dvec that had to reallocate) or a new
+// layout re-captures it, and a reassembly that writes new values into the
+// same buffers keeps it.
+void graph_key(bhip_ctx *ctx, uint64_t *k) {
+    int i = 0;
+    auto put = [&](uint64_t v) { k[i++] = v; };
+    auto ptr = [&](const void *p) { put((uint64_t)(uintptr_t)p); };
+    ptr(ctx->stream);
+    put((uint64_t)ctx->n_dof);
+    put((uint64_t)ctx->n_nodes);
+    put((uint64_t)ctx->n_cu);
+    put((uint64_t)ctx->nnz);
+    put((uint64_t)ctx->n_slices);
+    put((uint64_t)ctx->n_sell);
+    put((uint64_t)ctx->sell_b | (uint64_t)ctx->sell_shape << 8 | (uint64_t)ctx->use_bsr << 16 |
+        (uint64_t)ctx->sell_narrow << 17 | (uint64_t)ctx->sell_runs << 18);
+    ptr(ctx->state.p);
+    ptr(ctx->partials.p);
+    ptr(ctx->x.p);
+    ptr(ctx->r.p);
+    ptr(ctx->p.p);
+    ptr(ctx->q.p);
+    ptr(ctx->dinv.p);
+    ptr(ctx->d32.p);
+    ptr(ctx->r32.p);
+    ptr(ctx->p32.p);
+    ptr(ctx->q32.p);
+    ptr(ctx->dinv32.p);
+    ptr(ctx->val.p);
+    ptr(ctx->val32.p);
+    ptr(ctx->bval32.p);
+    ptr(ctx->bval64.p);
+    ptr(ctx->rowptr.p);
+    ptr(ctx->colind.p);
+    ptr(ctx->sell_ptr.p);
+    ptr(ctx->sell_col.p);
+    ptr(ctx->sell_col16.p);
+    while (i < kGraphKeyWords) put(0);
+}
+
 // A captured chunk of `chunk` iterations (kernel nodes only).
 int get_graph(bhip_ctx *ctx, int kind, int chunk, hipGraphExec_t *out) {
     hipGraphExec_t &g = kind == kIterInner32 ? ctx->graph32 : kind == kIterSell64 ? ctx->graph64b : ctx->graph64;
     int &gi = kind == kIterInner32 ? ctx->graph32_iters : kind == kIterSell64 ? ctx->graph64b_iters : ctx->graph64_iters;
-    if (g && gi == chunk) {
+    uint64_t *gk = kind == kIterInner32 ? ctx->graph32_key : kind == kIterSell64 ? ctx->graph64b_key : ctx->graph64_key;
+    uint64_t key[kGraphKeyWords];
+    graph_key(ctx, key);
+    if (g && gi == chunk && memcmp(key, gk, sizeof(key)) == 0) {
         *out = g;
         return BHIP_OK;
     }
@@ -1178,6 +1224,7 @@ int get_graph(bhip_ctx *ctx, int kind, int chunk, hipGraphExec_t *out) {
         (void)hipGraphExecDestroy(g);
         g = nullptr;
     }
+    gi = 0;
     hipGraph_t graph;
     BHIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < chunk; k++) enqueue_iteration(ctx, kind, k & 1);
@@ -1185,8 +1232,10 @@ int get_graph(bhip_ctx *ctx, int kind, int chunk, hipGraphExec_t *out) {
     BHIP_TRY(ctx, e);
     e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) g = nullptr;
     BHIP_TRY(ctx, e);
     gi = chunk;
+    memcpy(gk, key, sizeof(key));
     *out = g;
     return BHIP_OK;
 }

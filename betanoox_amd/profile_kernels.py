@@ -15,12 +15,17 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   reps x FP32 SELL-64 one-DOF SpMV        (k_sell_spmv<1, 1>, column runs)
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
   10 backward-Euler steps                 (unless --no-solve)
+--workload plastic (J2 bar, --n 48, the first --increments of the
+  20-increment path to 0.5% strain): per Newton iteration the FP64 return map
+  (k_plastic_update), the tangent assembly (k_elem_blocks / k_asm_pairs of
+  ElasHex8<J2>), the device residual (k_plastic_fe, k_plastic_residual) and an
+  FP32-inner refined solve to 1e-8
 
 The SELL SpMV runs in the shape the library's tuned table gives the matrix,
 or in --shape ID (see bhip_tune_spmv); the shape is printed.
 
-usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat] [--n N] [--reps 10] [--no-solve]
-                                              [--shape ID]
+usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat|plastic] [--n N] [--reps 10]
+                                              [--no-solve] [--shape ID] [--increments K]
 """
 import argparse
 import json
@@ -33,11 +38,12 @@ from . import capi, meshgen, setup_problem
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("elastic", "cases", "heat"), default="elastic")
+    ap.add_argument("--workload", choices=("elastic", "cases", "heat", "plastic"), default="elastic")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--shape", type=int, default=-1)
+    ap.add_argument("--increments", type=int, default=6, help="plastic: increments of the 20-increment path run")
     a = ap.parse_args(argv)
     out = {"workload": a.workload}
     with capi.Context(0) as ctx:
@@ -72,6 +78,20 @@ def main(argv=None):
             o = capi.Context.opts(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
             X, s = ctx.solve_cases(P, opts=o)
             out["cases_fp32_ir"] = {k: s[k] for k in ("iters", "outer_iters", "rel_resid", "ms_total")}
+        elif a.workload == "plastic":
+            n = a.n or 48
+            prob = meshgen.plastic_bar(n)
+            setup_problem(ctx, prob)
+            ctx.plastic_init()
+            ctx.set_load(prob.load)
+            o = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
+                                  inner_tol=1e-3, maxit=40000)
+            hist = []
+            for inc in range(min(a.increments, prob.extra["increments"])):
+                st = ctx.plastic_increment(prob.prescribed, 1e-9, 300, opts=o)
+                hist.append(st)
+                print("increment %d: %s" % (inc, json.dumps(st)), file=sys.stderr, flush=True)
+            out["plastic_increments"] = hist
         else:
             n = a.n or 128
             prob = meshgen.diffusion_cube(n)
