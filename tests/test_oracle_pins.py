@@ -414,3 +414,154 @@ def test_j2_uniaxial_stress_closed_form():
     sig = s0 + E * H / (E + H) * (0.005 - s0 / E)
     assert abs(stress[2] - sig) < 1e-9
     assert abs(stress[2] - 0.257425743) < 1e-9
+
+
+# ------------------------------------------------ plastic and elastic elements
+def _jittered_hex(rng, scale=1.0, jit=0.12):
+    """A non-affine (warped) hex8 in the reference node order."""
+    X = (HEX_NODES + 1) / 2 * scale + jit * scale * rng.uniform(-1, 1, (8, 3))
+    return X + rng.standard_normal(3)
+
+
+def _plastic_call(X, params, ue, isv):
+    L = O._plastic_sigs()
+    L.orc_plastic_update(O.d(O.f64(X).ravel().copy()), O.d(O.f64(params)), 2, O.d(O.f64(ue).copy()), O.d(isv))
+    Ke, Fe = np.zeros(576), np.zeros(24)
+    L.orc_plastic_ke_fe(O.d(O.f64(X).ravel().copy()), O.d(O.f64(params)), 2, O.d(isv), O.d(Ke), O.d(Fe))
+    K = Ke.reshape(24, 24)
+    return np.triu(K) + np.triu(K, 1).T, Fe
+
+
+def _fe_elastic(X, Cm, ue, order=2):
+    L = O.lib()
+    L.orc_fe_hex8_elastic.restype = C.c_int
+    L.orc_fe_hex8_elastic.argtypes = [O._P_D, O._P_D, C.c_int, O._P_D, O._P_D]
+    Fe = np.zeros(24)
+    st = L.orc_fe_hex8_elastic(O.d(O.f64(X).ravel().copy()), O.d(O.f64(Cm).ravel().copy()), order,
+                               O.d(O.f64(ue).copy()), O.d(Fe))
+    return st, Fe
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_fe_hex8_elastic_is_ke_times_ue(order):
+    """orc_fe_hex8_elastic (addToFe with stress = C strain) on warped bricks
+    equals K_e u_e with the pinned K_e (rigid modes, uniform-strain energy,
+    patch test) for random u_e; a wrong B row, shear factor or stress
+    component in the force path fails it."""
+    rng = np.random.default_rng(31)
+    for _ in range(4):
+        X = _jittered_hex(rng, scale=rng.uniform(0.2, 2.0))
+        Cm = O.iso_cmat(rng.uniform(1, 100), rng.uniform(0.15, 0.4))
+        st, K = O.ke_hex8(X, Cm, order)
+        assert st == 0
+        ue = rng.standard_normal(24)
+        st, Fe = _fe_elastic(X, Cm, ue, order)
+        assert st == 0
+        assert np.abs(Fe - K @ ue).max() <= 1e-13 * np.abs(K).max() * np.abs(ue).max()
+
+
+def test_plastic_element_elastic_increment_warped():
+    """A purely elastic increment with shear on a warped hex8, from a nonzero
+    stress below yield: the plastic element's "KI" tangent equals the pinned
+    elastic K_e and its "F" force equals K_e u_e (the force of the stress
+    increment), at every Gauss point the return map stays elastic."""
+    rng = np.random.default_rng(32)
+    E, nu, s0, H = 200.0, 0.3, 0.25, 2.0
+    params = np.array([E, nu, s0, H])
+    for _ in range(4):
+        X = _jittered_hex(rng)
+        ue = 1e-5 * rng.standard_normal(24)
+        isv = np.zeros((8, O.PISV))
+        isv[:, :6] = 0.02 * rng.standard_normal(6)  # prior stress, well inside the yield surface
+        isv[:, 6] = 0.001
+        Kp, Fp = _plastic_call(X, params, ue, isv)
+        _, K = O.ke_hex8(X, O.iso_cmat(E, nu))
+        assert np.all(isv[:, 14] == 0)
+        assert np.abs(Kp - K).max() <= 1e-13 * np.abs(K).max()
+        assert np.abs(Fp - K @ ue).max() <= 1e-13 * np.abs(K).max() * np.abs(ue).max()
+
+
+def test_plastic_pure_shear_path_closed_form():
+    """Homogeneous simple shear u = gamma*y on an affine (sheared, scaled)
+    hex8, driven in increments past yield, "V" + "KI"/"F" + "U" per increment.
+
+    Closed form of the reference's return map (getStressIncrement version
+    2.0, plastic.cpp:368-465) in pure shear tau = sigma_xy, J2 with linear
+    hardening: the Hill function is tau^2, the yield function
+    3 tau^2 / sigma_y^2 - 1 with sigma_y = sigma0 + H epbar, the flow vector
+    (0,0,0, 2 tau, 0,0).  An increment dgamma is elastic (tau += G dgamma)
+    when the trial 3 (tau + G dgamma)^2 <= sigma_y^2; otherwise the whole
+    increment is plastic and one return-map pass gives
+      tau += G dgamma H / (3G + H)            (the J2 elastoplastic shear modulus)
+      epbar += sqrt(3) tau_new G dgamma / ((H + 3G) tau_trial)
+    (stop test met after that pass), and the tangent's shear entry is
+    C_ep[3][3] = G H / (3G + H) with every other entry elastic.  The element
+    force of the increment obeys the virial identity sum_a x_a (x) F_a =
+    V dsigma (exact for an affine brick).  A dropped shear factor in the Hill
+    function or the flow vector, tensor instead of engineering shear, or a
+    wrong hardening modulus changes these numbers."""
+    E, nu, s0, H = 200.0, 0.3, 0.25, 2.0
+    G = E / (2 * (1 + nu))
+    params = np.array([E, nu, s0, H])
+    rng = np.random.default_rng(33)
+    X, V = _affine_hex(rng, scale=0.7)
+    isv = np.zeros((8, O.PISV))
+    tau, ep = 0.0, 0.0
+    g_y = s0 / math.sqrt(3) / G
+    steps = [0.6 * g_y, 0.3 * g_y, 0.5 * g_y, 0.5 * g_y, 1.0 * g_y, 2.0 * g_y, 0.02 * g_y]
+    flags = []
+    for dg in steps:
+        ue = np.zeros((8, 3))
+        ue[:, 0] = dg * X[:, 1]
+        Kp, Fp = _plastic_call(X, params, ue.ravel(), isv)
+        tr = tau + G * dg
+        plastic = 3 * tr ** 2 > (s0 + H * ep) ** 2
+        flags.append(bool(plastic))
+        if plastic:
+            tau_new = tau + G * dg * H / (3 * G + H)
+            dep = math.sqrt(3) * tau_new * G * dg / ((H + 3 * G) * tr)
+        else:
+            tau_new, dep = tr, 0.0
+        dtau = tau_new - tau
+        for ip in range(8):
+            r = isv[ip]
+            assert r[14] == (1 if plastic else 0)
+            assert abs(r[7 + 3] - dtau) <= 1e-12 * abs(dtau)
+            assert np.abs(r[[7, 8, 9, 11, 12]]).max() <= 1e-12 * abs(dtau)
+            assert abs(r[13] - dep) <= 1e-12 * max(dep, 1e-30)
+            Cep = r[16:52].reshape(6, 6)
+            Ce = O.iso_cmat(E, nu)
+            want = Ce.copy()
+            if plastic:
+                want[3, 3] = G * H / (3 * G + H)
+            assert np.abs(Cep - want).max() <= 1e-12 * E
+        # virial identity on the force of the stress increment
+        Fa = Fp.reshape(8, 3)
+        M = Fa.T @ X  # M[i, j] = sum_a F_a(i) x_a(j)
+        dsig = np.zeros((3, 3))
+        dsig[0, 1] = dsig[1, 0] = dtau
+        assert np.abs(M - V * dsig).max() <= 1e-12 * V * abs(dtau)
+        L = O._plastic_sigs()
+        L.orc_plastic_commit(O.d(isv), 8)
+        tau, ep = tau_new, ep + dep
+        assert np.allclose(isv[:, 3], tau, rtol=1e-12, atol=0)
+    # the first crossing makes its whole increment plastic and leaves the
+    # state inside the surface (tau ~ 0.9 tau_y); later large steps cross
+    # again, the last small one stays elastic
+    assert flags == [False, False, True, True, True, True, False]
+
+
+def test_heat_consistent_capacity_closed_form():
+    """Consistent capacity matrix of an affine brick (parallelepiped):
+    M_ab = c V / 216 * {8, 4, 2, 1} for nodes a, b that differ in 0, 1, 2, 3
+    reference coordinates (the tensor product of the 1D h/6 [2 1; 1 2]).  A
+    lumped or row-summed M, a missing detJ or a wrong shape function fails it."""
+    rng = np.random.default_rng(34)
+    for _ in range(3):
+        X, V = _affine_hex(rng, scale=rng.uniform(0.3, 2.0))
+        cap = rng.uniform(0.5, 3.0)
+        st, K, M = O.ke_heat(X, np.array([1.0, 1.0, 1.0]), cap)
+        assert st == 0
+        ndiff = (HEX_NODES[:, None, :] != HEX_NODES[None, :, :]).sum(axis=2)
+        want = cap * V / 216.0 * np.array([8.0, 4.0, 2.0, 1.0])[ndiff]
+        assert np.abs(M - want).max() <= 1e-14 * cap * V
