@@ -63,7 +63,8 @@ EXPORTS = [
     "bhip_build_pattern", "bhip_pattern_nnz", "bhip_get_pattern_lower", "bhip_get_values_lower",
     "bhip_get_csr", "bhip_set_materials", "bhip_set_integration", "bhip_assemble",
     "bhip_get_pseudo_load", "bhip_set_load", "bhip_build_rhs", "bhip_get_rhs", "bhip_set_rhs",
-    "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_spmv", "bhip_solve_cases",
+    "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_pcg_solve_noalloc", "bhip_spmv",
+    "bhip_solve_cases",
     "bhip_bench_spmv", "bhip_bench_stream", "bhip_bench_read", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
     "bhip_server_create", "bhip_server_destroy", "bhip_server_submit", "bhip_server_poll", "bhip_server_wait",
@@ -189,6 +190,7 @@ def load_library(path=LIB_PATH):
         "bhip_solve": (C.c_int, [vp, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_get_solution": (C.c_int, [vp, _P_D]),
         "bhip_pcg_solve": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
+        "bhip_pcg_solve_noalloc": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_spmv": (C.c_int, [vp, _P_D, _P_D]),
         "bhip_solve_cases": (C.c_int, [vp, C.c_int, _P_D, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),

@@ -185,6 +185,7 @@ struct bhip_ctx {
     dvec<PcgState> state;
     dvec<int> flags;        // assembly / diagonal check flags
     PcgState *h_state = nullptr;  // pinned mirror
+    unsigned char *scratch = nullptr;  // device address of this GPU's allocation-free scratch (solver.hip)
     // ---- graphs: one captured chunk of iterations per iteration kind, with
     // the addresses, sizes and shape it was captured with (graph_key in
     // solver.hip); a chunk whose key still matches is replayed, so a
@@ -292,6 +293,7 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
 int bhip_spmv_shape_info(int id, bhip_spmv_shape *info);
 int bhip_set_spmv_shape_impl(bhip_ctx *ctx, int id);
 void bhip_invalidate_graphs(bhip_ctx *ctx);
+int bhip_scratch_setup(bhip_ctx *ctx);  // once per device: address of the allocation-free scratch
 void bhip_plastic_release(bhip_ctx *ctx);
 void bhip_multi_invalidate(bhip_ctx *ctx);
 void bhip_multi_release(bhip_ctx *ctx);

@@ -62,6 +62,10 @@ int bhip_create(int device, bhip_ctx **out) {
         return BHIP_ERR_HIP;
     }
     memset(ctx->h_state, 0, sizeof(PcgState));
+    if (bhip_scratch_setup(ctx) != BHIP_OK) {
+        bhip_destroy(ctx);
+        return BHIP_ERR_HIP;
+    }
     *out = ctx;
     return BHIP_OK;
 }

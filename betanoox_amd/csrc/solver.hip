@@ -32,6 +32,8 @@
 #include <cstdlib>
 #include <vector>
 
+#include <mutex>
+
 #include "bhip_internal.hpp"
 #include "tuned_shapes.hpp"
 
@@ -768,15 +770,19 @@ int sell_groups(bhip_ctx *ctx) {
     return (int)(g > 0 ? g : 1);
 }
 
+// CSR SpMV writing its block records into `part` (the context's records
+// unless the caller brings its own).
 template <class VT, class XT, int MODE>
-void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *b, const PcgState *st) {
+void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *b, const PcgState *st,
+                double *part = nullptr) {
     hipStream_t s = ctx->stream;
     const int n = ctx->n_dof;
     const int grid = csr_grid(ctx);
+    if (!part) part = ctx->partials.p;
 #define BHIP_CSR_CASE(L)                                                                           \
     case L:                                                                                        \
         k_csr_spmv<L, VT, XT, MODE><<<grid, kThreads, 0, s>>>(n, ctx->rowptr.p, ctx->colind.p, val, \
-                                                              x, y, b, st, ctx->partials.p);       \
+                                                              x, y, b, st, part);                  \
         break;
     switch (pick_lpr(ctx->nnz, ctx->n_dof)) {
         BHIP_CSR_CASE(2)
@@ -1118,6 +1124,19 @@ int prepare_f32(bhip_ctx *ctx, bool with64 = false) {
 // (SELL or CSR values), FP64 on SELL block values.
 enum IterKind { kIterCsr64 = 0, kIterInner32 = 1, kIterSell64 = 2 };
 
+// One plain FP64 PCG iteration on the CSR values, five kernels, with the
+// block records in `part` and the solver state in `st`.
+void enqueue_csr64_iteration(bhip_ctx *ctx, double *part, PcgState *st) {
+    hipStream_t s = ctx->stream;
+    const long long n = ctx->n_dof;
+    const int gv = vec_grid(ctx, n);
+    launch_csr<double, double, 1>(ctx, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st, part);
+    k_fin_alpha<<<1, kFinThreads, 0, s>>>(csr_grid(ctx), part, st);
+    k_pcg_update<double><<<gv, kThreads, 0, s>>>(n, ctx->x.p, ctx->r.p, ctx->p.p, ctx->q.p, ctx->dinv.p, st, part);
+    k_fin_update<<<1, kFinThreads, 0, s>>>(gv, part, st);
+    k_pcg_pupdate<double><<<gv, kThreads, 0, s>>>(n, ctx->p.p, ctx->r.p, ctx->dinv.p, st);
+}
+
 // Enqueue one PCG iteration: three kernels on a blocked form (par = parity of
 // the iteration, which selects the r.z buffer), five on CSR.
 void enqueue_iteration(bhip_ctx *ctx, int kind, int par) {
@@ -1134,12 +1153,7 @@ void enqueue_iteration(bhip_ctx *ctx, int kind, int par) {
         return;
     }
     if (kind == kIterCsr64) {
-        launch_csr<double, double, 1>(ctx, ctx->val.p, ctx->p.p, ctx->q.p, nullptr, st);
-        k_fin_alpha<<<1, kFinThreads, 0, s>>>(csr_grid(ctx), ctx->partials.p, st);
-        k_pcg_update<double><<<gv, kThreads, 0, s>>>(n, ctx->x.p, ctx->r.p, ctx->p.p, ctx->q.p, ctx->dinv.p, st,
-                                                     ctx->partials.p);
-        k_fin_update<<<1, kFinThreads, 0, s>>>(gv, ctx->partials.p, st);
-        k_pcg_pupdate<double><<<gv, kThreads, 0, s>>>(n, ctx->p.p, ctx->r.p, ctx->dinv.p, st);
+        enqueue_csr64_iteration(ctx, ctx->partials.p, st);
     } else {
         if (ctx->use_bsr) {
             double *p2 = ctx->partials.p + fused_offset(ctx);
@@ -1462,6 +1476,152 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
     if (stats) *stats = S;
     if (st_code == BHIP_ERR_NOT_CONVERGED)
         return bhip_fail(ctx, st_code, "Solution not converged within the iteration cap");
+    return BHIP_OK;
+}
+
+// ------------------------------------------------ allocation-free FP64 solve
+// bhip_pcg_solve_noalloc runs the plain FP64 Jacobi-PCG of bhip_pcg_solve
+// (same kernels, same order, same bits) for callers that must not allocate.
+// The device memory a solve needs beyond the context's matrix and vectors --
+// the solver state, the diagonal flags and the block reduction records --
+// lives in this file-scope array, one copy per GPU, held until the process
+// ends.  In host code `scratch` only names the symbol: kernels receive the
+// device address hipGetSymbolAddress returned for the current device
+// (bhip_scratch_setup, called by bhip_create).  Every call on one GPU shares
+// that GPU's copy, so calls that overlap on one GPU (two host threads, two
+// contexts) race; each call clears what must start at zero with fill kernels.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)4 << 20;  // 4 MiB per GPU
+constexpr size_t kScratchStateOff = 0;             // PcgState
+constexpr size_t kScratchFlagsOff = 256;           // 4 ints: diagonal flags
+constexpr size_t kScratchRecOff = 512;             // reduction records (doubles)
+constexpr size_t kScratchRecords = (kScratchBytes - kScratchRecOff) / sizeof(double);
+static_assert(sizeof(PcgState) <= kScratchFlagsOff, "PcgState does not fit its scratch slot");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kMaxScratchDevices] = {};
+
+__global__ void k_fill_int(int n, int *p, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_double(long long n, double *p, double v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Records a five-kernel FP64 CSR iteration writes: 2 per SpMV workgroup (the
+// residual form), 3 per vector workgroup.
+size_t noalloc_records(bhip_ctx *ctx) {
+    return std::max((size_t)kRedBlocks * 3, (size_t)csr_grid(ctx) * 2);
+}
+
+}  // namespace
+
+int bhip_scratch_setup(bhip_ctx *ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxScratchDevices)
+        return bhip_fail(ctx, BHIP_ERR_CAPACITY, "device index beyond the allocation-free scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return bhip_fail(ctx, BHIP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[ctx->device] = static_cast<unsigned char *>(p);
+    }
+    ctx->scratch = g_scratch_dev[ctx->device];
+    return BHIP_OK;
+}
+
+extern "C" int bhip_pcg_solve_noalloc(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_opts *o,
+                                      bhip_stats *stats) {
+    if (!ctx || !b || !o) return BHIP_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (!ctx->scratch) return bhip_fail(ctx, BHIP_ERR_STATE, "allocation-free scratch not set up");
+    if (!ctx->assembled) return bhip_fail(ctx, BHIP_ERR_STATE, "matrix not assembled");
+    if (ctx->x.n != (size_t)ctx->n_dof || ctx->tmp.n != (size_t)ctx->n_dof)
+        return bhip_fail(ctx, BHIP_ERR_STATE, "solver vectors not allocated");
+    if (o->precision != BHIP_PREC_FP64)
+        return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "the allocation-free solve runs BHIP_PREC_FP64 only");
+    if (!(o->tol > 0)) return bhip_fail(ctx, BHIP_ERR_ARG, "tol must be positive");
+    if (noalloc_records(ctx) > kScratchRecords)
+        return bhip_fail(ctx, BHIP_ERR_CAPACITY,
+                         "system too large for the allocation-free scratch (" + std::to_string(noalloc_records(ctx)) +
+                             " reduction records needed, " + std::to_string(kScratchRecords) + " available)");
+    const int check_every = o->check_every > 0 ? o->check_every : 32;
+    const int maxit = o->maxit > 0 ? o->maxit : 10000;
+    hipStream_t s = ctx->stream;
+    const long long n = ctx->n_dof;
+    const int B = 256;
+    PcgState *st = reinterpret_cast<PcgState *>(ctx->scratch + kScratchStateOff);
+    int *flags = reinterpret_cast<int *>(ctx->scratch + kScratchFlagsOff);
+    double *part = reinterpret_cast<double *>(ctx->scratch + kScratchRecOff);
+    bhip_stats S;
+    memset(&S, 0, sizeof(S));
+
+    // right-hand side and initial guess into the context's vectors (no allocation)
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->rhs.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (x0) BHIP_TRY(ctx, hipMemcpyAsync(ctx->x.p, x0, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    else k_fill_double<<<grid_for(n, B), B, 0, s>>>(n, ctx->x.p, 0.0);
+    // Jacobi diagonal and its checks (CheckAndFixZeroDiagonal, largeMatrix.cpp:43-79)
+    k_fill_int<<<1, 64, 0, s>>>(4, flags, 0);
+    k_dinv<<<grid_for(n, B), B, 0, s>>>(n, ctx->diag_pos.p, ctx->val.p, ctx->dinv.p, nullptr, flags);
+    BHIP_TRY(ctx, hipGetLastError());
+    // the pinned state mirror receives the flags word
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    BHIP_TRY(ctx, hipStreamSynchronize(s));
+    const int hf = *reinterpret_cast<const int *>(ctx->h_state);
+    if (hf & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "negative diagonal term in the global matrix");
+    if (hf & 1) return bhip_fail(ctx, BHIP_ERR_ZERO_DIAG, "zero diagonal term in the global matrix");
+
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    // r = b - K x0, z = p = r/diag (sparse.cpp:152-163), then the loop in
+    // chunks launched kernel by kernel (no graph: instantiating one allocates)
+    const int gv = vec_grid(ctx, n);
+    k_state_init<<<1, 1, 0, s>>>(st, o->tol, maxit, o->stop_rule);
+    launch_csr<double, double, 2>(ctx, ctx->val.p, ctx->x.p, ctx->r.p, ctx->rhs.p, st, part);
+    k_fin_resid<<<1, kFinThreads, 0, s>>>(csr_grid(ctx), part, st);
+    k_pcg_init<double><<<gv, kThreads, 0, s>>>(n, ctx->x.p, ctx->r.p, ctx->p.p, ctx->dinv.p, part);
+    k_fin_init<<<1, kFinThreads, 0, s>>>(gv, part, st);
+    BHIP_TRY(ctx, hipGetLastError());
+    const long long max_chunks = (long long)maxit / check_every + 2;
+    for (long long chunk = 0;; chunk++) {
+        BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        BHIP_TRY(ctx, hipStreamSynchronize(s));
+        if (ctx->h_state->done) break;
+        if (chunk > max_chunks)
+            return bhip_fail(ctx, BHIP_ERR_STATE, "PCG state machine did not finish (internal error)");
+        for (int k = 0; k < check_every; k++) enqueue_csr64_iteration(ctx, part, st);
+        BHIP_TRY(ctx, hipGetLastError());
+        S.spmv_launches += check_every;
+    }
+    S.iters = ctx->h_state->iter;
+    S.outer_iters = 1;
+    S.ref_norm = ctx->h_state->stop_rule == BHIP_STOP_REF_R_OVER_X ? ctx->h_state->norm
+                                                                    : sqrt(ctx->h_state->rr / ctx->h_state->xx);
+    const bool converged = ctx->h_state->converged != 0;
+    S.spmv_bytes = spmv_bytes(ctx, false, false);
+    BHIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    BHIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    BHIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    S.ms_total = ms;
+    // true residual |b - K x| / |b|
+    launch_csr<double, double, 2>(ctx, ctx->val.p, ctx->x.p, ctx->tmp.p, ctx->rhs.p, st, part);
+    k_fin_resid<<<1, kFinThreads, 0, s>>>(csr_grid(ctx), part, st);
+    BHIP_TRY(ctx, hipGetLastError());
+    BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    BHIP_TRY(ctx, hipStreamSynchronize(s));
+    S.rel_resid = ctx->h_state->bb > 0 ? sqrt(ctx->h_state->rr / ctx->h_state->bb) : sqrt(ctx->h_state->rr);
+    if (stats) *stats = S;
+    BHIP_TRY(ctx, hipMemcpyAsync(b, ctx->x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    BHIP_TRY(ctx, hipStreamSynchronize(s));
+    if (!converged) return bhip_fail(ctx, BHIP_ERR_NOT_CONVERGED, "Solution not converged within the iteration cap");
     return BHIP_OK;
 }
 

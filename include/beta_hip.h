@@ -165,6 +165,18 @@ int bhip_get_solution(bhip_ctx *ctx, double *x);
  * solution on return; x0 may be NULL (zero initial guess). */
 int bhip_pcg_solve(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_opts *opts,
                    bhip_stats *stats);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): bhip_pcg_solve for
+ * opts->precision == BHIP_PREC_FP64 only (other precisions: UNSUPPORTED), with
+ * the same kernels, iterates and bits, launched without graphs.  The solver
+ * state, diagonal flags and reduction records live in a fixed 4 MiB device
+ * array per GPU (held until the process ends) instead of context buffers, so
+ * the call allocates nothing once the context is assembled; a system needing
+ * more than 524,224 reduction records (about 8.3M DOF at 81 entries per row)
+ * is refused with BHIP_ERR_CAPACITY.  Calls that overlap on one GPU (other
+ * threads or contexts) share that array and race. */
+int bhip_pcg_solve_noalloc(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_opts *opts,
+                           bhip_stats *stats);
 /* LargeMatrix::vectorProduct (sparse.cpp:452-478): y = K x. */
 int bhip_spmv(bhip_ctx *ctx, const double *x, double *y);
 /* Several load cases on the assembled stiffness: the multi-column solve of
