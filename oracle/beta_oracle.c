@@ -1100,6 +1100,19 @@ static double j2_hard(const j2_state *g)
     return 4.0 / 9.0 * mm * mm * j2_dsdep(g);
 }
 
+/* dYield is linear in the stress: nf = P s.  j2_flow applies it to any
+ * 6-vector (the derivative of the loop below applies it to columns). */
+static void j2_flow(const double *s, double *nf)
+{
+    const double F = 1.0 / 3.0, G = 1.0 / 3.0, H = 1.0 / 3.0, L = 1, M = 1, N = 1;
+    nf[0] = -G * (s[2] - s[0]) + H * (s[0] - s[1]);
+    nf[1] = F * (s[1] - s[2]) - H * (s[0] - s[1]);
+    nf[2] = -F * (s[1] - s[2]) + G * (s[2] - s[0]);
+    nf[3] = 2 * N * s[3];
+    nf[4] = 2 * L * s[4];
+    nf[5] = 2 * M * s[5];
+}
+
 static void j2_dyield(const j2_state *g, double *nf)
 {
     const double F = 1.0 / 3.0, G = 1.0 / 3.0, H = 1.0 / 3.0, L = 1, M = 1, N = 1;
@@ -1128,9 +1141,39 @@ static double dot6(const double *a, const double *b)
     return s;
 }
 
+/* The return map with, when Dalg is given, its exact derivative
+ * Dalg = d(dstress)/d(dstrain) (row-major 6x6, engineering-shear strain in,
+ * stress out): forward-mode differentiation through the iterations the loop
+ * actually executes (the same branch, the same iteration count).  Per pass,
+ * with s = s0 + ds and D = d(ds)/d(dstrain):
+ *   nf = P s, sp = C nf, hard = (4/3) H hill(s)  (d hill/ds = nf, m_dsdep = H)
+ *   dl = nf.dse / (hard + nf.sp)
+ *   d nf = P D, d num = dse^T P D + nf^T C, d den = (4/3) H nf^T D + 2 sp^T P D
+ *   ds' = dse - dl sp,  D' = C - sp (d dl) - dl C P D.
+ * This is not the reference's tangent (getTangentStiffness, the continuum
+ * C_ep); it is what a Newton iteration on the reference's stress update needs
+ * for quadratic convergence.  It is not symmetric in general. */
+static int j2_increment_impl(const orc_j2_params *p, const double *stress, double eps_bar,
+                             const double *dstrain, double *dstress, double *deps_bar,
+                             double *Cep, double *Dalg, int *plastic);
+
 int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double eps_bar,
                             const double *dstrain, double *dstress, double *deps_bar,
                             double *Cep, int *plastic)
+{
+    return j2_increment_impl(p, stress, eps_bar, dstrain, dstress, deps_bar, Cep, NULL, plastic);
+}
+
+int orc_j2_stress_increment_alg(const orc_j2_params *p, const double *stress, double eps_bar,
+                                const double *dstrain, double *dstress, double *deps_bar,
+                                double *Cep, double *Dalg, int *plastic)
+{
+    return j2_increment_impl(p, stress, eps_bar, dstrain, dstress, deps_bar, Cep, Dalg, plastic);
+}
+
+static int j2_increment_impl(const orc_j2_params *p, const double *stress, double eps_bar,
+                             const double *dstrain, double *dstress, double *deps_bar,
+                             double *Cep, double *Dalg, int *plastic)
 {
     j2_state g;
     g.p = p;
@@ -1142,6 +1185,8 @@ int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double
     j2_cmul(g.C, dstrain, dse);
     for (int i = 0; i < 6; i++) g.ds[i] = dse[i];
     memcpy(Cep, g.C, sizeof(double) * 36);
+    double D[36];  /* d(ds)/d(dstrain), columns j */
+    memcpy(D, g.C, sizeof(D));
     int iter = 0;
     *plastic = 0;
     if (j2_yield_fn(&g) > 0.0) {
@@ -1151,7 +1196,34 @@ int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double
             double hard = j2_hard(&g);
             j2_dyield(&g, nf);
             j2_cmul(g.C, nf, sp);
-            double dl = dot6(nf, dse) / (hard + dot6(nf, sp));
+            double num = dot6(nf, dse), den = hard + dot6(nf, sp);
+            double dl = num / den;
+            if (Dalg) {
+                const double Hh = p->H; /* m_dsdep of the POWER curve with m_n = 1 */
+                double PD[36], CPD[36], Cnf[6];
+                for (int j = 0; j < 6; j++) {
+                    double col[6], pc[6], cpc[6];
+                    for (int i = 0; i < 6; i++) col[i] = D[i * 6 + j];
+                    j2_flow(col, pc);
+                    j2_cmul(g.C, pc, cpc);
+                    for (int i = 0; i < 6; i++) {
+                        PD[i * 6 + j] = pc[i];
+                        CPD[i * 6 + j] = cpc[i];
+                    }
+                }
+                j2_cmul(g.C, nf, Cnf);
+                double ddl[6];
+                for (int j = 0; j < 6; j++) {
+                    double dnum = 0, dden = 0;
+                    for (int i = 0; i < 6; i++) {
+                        dnum += dse[i] * PD[i * 6 + j] + nf[i] * g.C[i * 6 + j];
+                        dden += 4.0 / 3.0 * Hh * nf[i] * D[i * 6 + j] + 2.0 * sp[i] * PD[i * 6 + j];
+                    }
+                    ddl[j] = (dnum - dl * dden) / den;
+                }
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) D[i * 6 + j] = g.C[i * 6 + j] - sp[i] * ddl[j] - dl * CPD[i * 6 + j];
+            }
             for (int i = 0; i < 6; i++) g.ds[i] = dse[i] - dl * sp[i];
             g.dep = 2.0 * dl * j2_eff_stress(&g) / 3.0;
             denom = j2_yield_fn(&g);
@@ -1166,6 +1238,7 @@ int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double
     }
     for (int i = 0; i < 6; i++) dstress[i] = g.ds[i];
     *deps_bar = g.dep;
+    if (Dalg) memcpy(Dalg, D, sizeof(D));
     return iter;
 }
 
@@ -1194,6 +1267,17 @@ static double hex8_ip_grads(const double *xyz, int order, int ip, double *dx, do
  * PlasticMaterial::updateISV -> getStressIncrement at every Gauss point. */
 int orc_plastic_update(const double *xyz, const orc_j2_params *p, int order, const double *ue, double *isv)
 {
+    return orc_plastic_update_tangent(xyz, p, order, ue, isv, ORC_TANGENT_CONTINUUM);
+}
+
+/* As orc_plastic_update; tangent = ORC_TANGENT_ALGORITHMIC stores the
+ * symmetric part (D + D^T)/2 of the return map's exact derivative in the
+ * record's tangent slot instead of the reference's continuum C_ep (the
+ * assembled stiffness must stay symmetric for PCG; the two agree for
+ * proportional loading, where D is symmetric). */
+int orc_plastic_update_tangent(const double *xyz, const orc_j2_params *p, int order, const double *ue,
+                               double *isv, int tangent)
+{
     int nip = order * order * order;
     for (int ip = 0; ip < nip; ip++) {
         double dx[8], dy[8], dz[8];
@@ -1211,7 +1295,11 @@ int orc_plastic_update(const double *xyz, const orc_j2_params *p, int order, con
         double *r = isv + (size_t)ip * ORC_PISV;
         double dep;
         int pl;
-        int it = orc_j2_stress_increment(p, r, r[6], de, r + 7, &dep, r + 16, &pl);
+        double D[36];
+        int it = orc_j2_stress_increment_alg(p, r, r[6], de, r + 7, &dep, r + 16, D, &pl);
+        if (tangent == ORC_TANGENT_ALGORITHMIC)
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) r[16 + i * 6 + j] = 0.5 * (D[i * 6 + j] + D[j * 6 + i]);
         r[13] = dep;
         r[14] = pl;
         r[15] = it;

@@ -110,6 +110,12 @@ typedef struct {
 int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double eps_bar,
                             const double *dstrain, double *dstress, double *deps_bar,
                             double *Cep36, int *plastic);
+/* Same update; Dalg36 = exact derivative d(dstress)/d(dstrain) of the loop as executed. */
+int orc_j2_stress_increment_alg(const orc_j2_params *p, const double *stress, double eps_bar,
+                                const double *dstrain, double *dstress, double *deps_bar,
+                                double *Cep36, double *Dalg36, int *plastic);
+#define ORC_TANGENT_CONTINUUM   0   /* getTangentStiffness (plastic.cpp:583-610) */
+#define ORC_TANGENT_ALGORITHMIC 1   /* symmetric part of the return map's derivative */
 
 /* Plastic hex8 element (plasticElement.cpp:39-250).  Per Gauss point record
  * of ORC_PISV doubles: stress[6], eps_bar, dstress[6], deps_bar, plastic,
@@ -117,6 +123,8 @@ int orc_j2_stress_increment(const orc_j2_params *p, const double *stress, double
 #define ORC_PISV 52
 int  orc_plastic_update(const double *xyz24, const orc_j2_params *p, int order, const double *ue24,
                         double *isv);                                            /* "V" */
+int  orc_plastic_update_tangent(const double *xyz24, const orc_j2_params *p, int order, const double *ue24,
+                                double *isv, int tangent);
 int  orc_plastic_ke_fe(const double *xyz24, const orc_j2_params *p, int order, const double *isv,
                        double *Ke, double *Fe);                                  /* "KI" + "F" */
 void orc_plastic_commit(double *isv, int nip);                                   /* "U" */

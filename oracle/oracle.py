@@ -154,6 +154,20 @@ def ke_heat(xyz8, k3, cap, order=2):
     return st, np.triu(K) + np.triu(K, 1).T, np.triu(M) + np.triu(M, 1).T
 
 
+def j2_increment_alg(E, nu, sigma0, H, stress, eps_bar, dstrain):
+    """As j2_increment, plus the exact derivative d(dstress)/d(dstrain) of
+    the return map as executed (orc_j2_stress_increment_alg)."""
+    L = lib()
+    L.orc_j2_stress_increment_alg.restype = C.c_int
+    L.orc_j2_stress_increment_alg.argtypes = [_P_D, _P_D, C.c_double, _P_D, _P_D, _P_D, _P_D, _P_D,
+                                              C.POINTER(C.c_int)]
+    p = np.array([E, nu, sigma0, H], dtype=np.float64)
+    ds, dep, Cep, D, pl = np.zeros(6), np.zeros(1), np.zeros(36), np.zeros(36), C.c_int()
+    it = L.orc_j2_stress_increment_alg(d(p), d(f64(stress).copy()), eps_bar, d(f64(dstrain).copy()), d(ds),
+                                       d(dep), d(Cep), d(D), C.byref(pl))
+    return it, ds, dep[0], Cep.reshape(6, 6), D.reshape(6, 6), pl.value
+
+
 def j2_increment(E, nu, sigma0, H, stress, eps_bar, dstrain):
     p = np.array([E, nu, sigma0, H], dtype=np.float64)
     ds, dep, Cep, pl = np.zeros(6), np.zeros(1), np.zeros(36), C.c_int()
@@ -385,6 +399,8 @@ def _plastic_sigs():
         return L
     L.orc_plastic_update.restype = C.c_int
     L.orc_plastic_update.argtypes = [_P_D, _P_D, C.c_int, _P_D, _P_D]
+    L.orc_plastic_update_tangent.restype = C.c_int
+    L.orc_plastic_update_tangent.argtypes = [_P_D, _P_D, C.c_int, _P_D, _P_D, C.c_int]
     L.orc_plastic_ke_fe.restype = C.c_int
     L.orc_plastic_ke_fe.argtypes = [_P_D, _P_D, C.c_int, _P_D, _P_D, _P_D]
     L.orc_plastic_commit.restype = None
@@ -393,8 +409,12 @@ def _plastic_sigs():
     return L
 
 
+TANGENT_CONTINUUM = 0
+TANGENT_ALGORITHMIC = 1
+
+
 def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, solve_eps=1e-12,
-                     presc_list=None):
+                     presc_list=None, tangent=TANGENT_CONTINUUM):
     """PlasticityModel::DoPlasticAnalysis (PlasticityModel.cpp:142-398) over
     the oracle's sparse system: per increment, prescribed increment values on
     restrained DOFs, then "V", assembleKandF ("KI"), getGlobalForces ("F"),
@@ -404,7 +424,9 @@ def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, s
     accumulated displacement, the Gauss-point records and per-increment
     (newton solves, max residual, plastic points, largest return-map
     iteration count of the committed records).  presc_list optionally gives the
-    prescribed increment vector of every increment (default: prob.prescribed)."""
+    prescribed increment vector of every increment (default: prob.prescribed).
+    tangent selects the reference's continuum C_ep (default) or the symmetric
+    part of the return map's derivative."""
     L = _plastic_sigs()
     n = prob.n_dof
     order = prob.order
@@ -432,7 +454,7 @@ def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, s
         while True:
             for e in range(prob.n_el):
                 rec = isv[e]
-                L.orc_plastic_update(d(xyz_e[e]), d(p), order, d(f64(du[maps[e]])), d(rec))
+                L.orc_plastic_update_tangent(d(xyz_e[e]), d(p), order, d(f64(du[maps[e]])), d(rec), tangent)
             L.orc_eq_zero(eq.h)
             res = np.zeros(n)
             for e in range(prob.n_el):

@@ -565,3 +565,61 @@ def test_heat_consistent_capacity_closed_form():
         ndiff = (HEX_NODES[:, None, :] != HEX_NODES[None, :, :]).sum(axis=2)
         want = cap * V / 216.0 * np.array([8.0, 4.0, 2.0, 1.0])[ndiff]
         assert np.abs(M - want).max() <= 1e-14 * cap * V
+
+
+def test_j2_algorithmic_derivative_matches_finite_differences():
+    """orc_j2_stress_increment_alg's Dalg is the derivative of the return map
+    it executes: central differences of dstress over dstrain (same branch and
+    iteration count) agree to 1e-7 of |D|.  In a proportional (uniaxial)
+    state it is symmetric and its symmetric part is what the assembly uses."""
+    E, nu, s0, H = 200.0, 0.3, 0.25, 2.0
+    rng = np.random.default_rng(35)
+    h = 1e-8
+    for _ in range(6):
+        stress = rng.standard_normal(6) * 0.05
+        de = rng.standard_normal(6) * 2e-3
+        it, ds, dep, Cep, D, pl = O.j2_increment_alg(E, nu, s0, H, stress, 0.001, de)
+        assert pl == 1
+        it2, ds2, dep2, Cep2, pl2 = O.j2_increment(E, nu, s0, H, stress, 0.001, de)
+        assert np.array_equal(ds, ds2) and np.array_equal(Cep, Cep2)  # same update, same reference tangent
+        Dfd = np.zeros((6, 6))
+        for j in range(6):
+            e1, e2 = de.copy(), de.copy()
+            e1[j] += h
+            e2[j] -= h
+            r1 = O.j2_increment_alg(E, nu, s0, H, stress, 0.001, e1)
+            r2 = O.j2_increment_alg(E, nu, s0, H, stress, 0.001, e2)
+            assert r1[0] == r2[0] == it
+            Dfd[:, j] = (r1[1] - r2[1]) / (2 * h)
+        assert np.abs(D - Dfd).max() <= 1e-7 * np.abs(D).max()
+    # elastic step: D = C
+    it, ds, dep, Cep, D, pl = O.j2_increment_alg(E, nu, s0, H, np.zeros(6), 0.0, np.array([1e-4, 0, 0, 0, 0, 0]))
+    assert pl == 0 and np.array_equal(D, O.iso_cmat(E, nu))
+    # uniaxial trial state: symmetric derivative
+    sig = np.array([0, 0, 0.2, 0, 0, 0.0])
+    it, ds, dep, Cep, D, pl = O.j2_increment_alg(E, nu, s0, H, sig, 0.0, np.array([-3e-4, -3e-4, 1e-3, 0, 0, 0]))
+    assert pl == 1 and np.abs(D - D.T).max() <= 1e-12 * np.abs(D).max()
+
+
+def test_plastic_driver_all_plastic_path_closed_form():
+    """PlasticityModel driver (oracle plastic_analysis) on the rollers bar,
+    2^3 elements, in four increments of 0.3% axial strain (each 2.4x the
+    yield strain), so every Gauss point takes the plastic branch in every
+    increment.  For a uniaxial trial state the flow vector's direction does
+    not depend on its size and the reference's one-pass update is linear in
+    the strain increment, with the J2 continuum modulus: each increment adds
+    E H / (E + H) times its axial strain to sigma_zz and nothing laterally.
+    So sigma_zz = E H/(E+H) * 1.2% at every point (the reference's update
+    does not carry sigma0 into a first plastic increment from zero stress).
+    With the algorithmic tangent Newton reaches it in a few iterations; the
+    reference tangent needs hundreds (it converges linearly)."""
+    prob = meshgen.plastic_bar(2, strain=0.012, increments=4)
+    u, isv, hist = O.plastic_analysis(prob, 4, 1e-9, 300, tangent=O.TANGENT_ALGORITHMIC)
+    Et = 200.0 * 2.0 / 202.0
+    assert all(h[0] <= 8 and h[1] < 2e-9 for h in hist), hist
+    assert np.abs(isv[:, :, 2] - Et * 0.012).max() <= 1e-8
+    assert np.abs(isv[:, :, [0, 1, 3, 4, 5]]).max() <= 1e-7
+    assert np.all(isv[:, :, 14] == 1)
+    # the top face moved by the prescribed 1.2 %
+    top = np.isclose(prob.xyz[:, 2], 1.0)
+    assert np.allclose(u[3 * np.where(top)[0] + 2], 0.012, rtol=0, atol=1e-15)
