@@ -14,10 +14,11 @@ Also reported in the same JSON line:
   packing -- 8 small RVE requests solved one after another and in one pack
   served  -- RVE request stream (prefix of the seeded 4,096-request stream),
              solves/s and p50/p99 latency at 1e-8
-  plastic -- J2 plasticity bar, 20 increments (only with --plastic-n 64; ~70 s)
+  plastic -- J2 plasticity bar 64^3, 20 increments, algorithmic tangent, and the
+             reference's continuum tangent beside it (~60 s)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 96]
-                       [--heat-n 128] [--pack-demo 8] [--served 128] [--plastic-n 0]
+                       [--heat-n 128] [--pack-demo 8] [--served 128] [--plastic-n 64]
 """
 import argparse
 import json
@@ -232,16 +233,20 @@ def heat_leg(n, steps, fp64_steps, inner_tol=1e-4):
     return out
 
 
-def plastic_leg(n, newton_tol, max_newton):
+def plastic_leg(n, newton_tol, max_newton, tangent):
     """Workload 4: J2 plasticity bar n^3, 20 prescribed-displacement
     increments to 0.5% strain, Newton with the tangent reassembled every
-    iteration until max|residual| < 2*newton_tol, linear solves to 1e-8."""
+    iteration until max|residual| < 2*newton_tol, linear solves to 1e-8.
+    tangent: the reference's continuum C_ep or the symmetric part of the
+    return map's exact derivative (the chosen form)."""
     from betanoox_amd import capi, meshgen, setup_problem
     prob = meshgen.plastic_bar(n)
-    out = {"n": n, "n_dof": prob.n_dof, "increments": prob.extra["increments"], "newton_tol": newton_tol}
+    out = {"n": n, "n_dof": prob.n_dof, "increments": prob.extra["increments"], "newton_tol": newton_tol,
+           "tangent": "algorithmic" if tangent == capi.TANGENT_ALGORITHMIC else "continuum (reference)"}
     with capi.Context(0) as ctx:
         setup_problem(ctx, prob)
         ctx.plastic_init()
+        ctx.plastic_set_tangent(tangent)
         ctx.set_load(prob.load)
         o = capi.Context.opts(tol=1e-8, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
                               inner_tol=1e-3, maxit=40000)
@@ -252,9 +257,13 @@ def plastic_leg(n, newton_tol, max_newton):
             hist.append(st)
             log("plastic increment %d: %s" % (inc, json.dumps(st)))
         wall = time.perf_counter() - t0
+        stress, epsb, _ = ctx.plastic_state(prob.n_el, prob.order ** 3)
     out.update({"wall_s": wall, "newton_iters": sum(h["newton_iters"] for h in hist),
                 "pcg_iters": sum(h["pcg_iters"] for h in hist),
+                "newton_per_increment": [h["newton_iters"] for h in hist],
                 "all_converged": all(h["status"] == 0 for h in hist),
+                "plastic_points_end": hist[-1]["plastic_points"],
+                "sigma_zz_min_max": [float(stress[:, :, 2].min()), float(stress[:, :, 2].max())],
                 "dof_iter_per_s": prob.n_dof * sum(h["pcg_iters"] for h in hist) / wall})
     return out
 
@@ -280,7 +289,9 @@ def main():
     ap.add_argument("--heat-steps", type=int, default=100)
     ap.add_argument("--heat-fp64-steps", type=int, default=10)
     ap.add_argument("--heat-inner-tol", type=float, default=1e-4)
-    ap.add_argument("--plastic-n", type=int, default=0, help="J2 bar size (0: skip; the full workload is 64)")
+    ap.add_argument("--plastic-n", type=int, default=64, help="J2 bar size (0: skip; the full workload is 64)")
+    ap.add_argument("--plastic-reference-tangent", type=int, default=1,
+                    help="also run the reference's continuum tangent (1) or not (0)")
     ap.add_argument("--plastic-tol", type=float, default=1e-9)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -412,8 +423,13 @@ def main():
         line["heat"] = heat_leg(args.heat_n, args.heat_steps, args.heat_fp64_steps, args.heat_inner_tol)
         log("heat: %s" % json.dumps(line["heat"]))
     if args.plastic_n > 0:
-        line["plastic"] = plastic_leg(args.plastic_n, args.plastic_tol, 300)
+        line["plastic"] = plastic_leg(args.plastic_n, args.plastic_tol, 300, capi.TANGENT_ALGORITHMIC)
         log("plastic: %s" % json.dumps(line["plastic"]))
+        if args.plastic_reference_tangent:
+            ref = plastic_leg(args.plastic_n, args.plastic_tol, 300, capi.TANGENT_CONTINUUM)
+            line["plastic"]["reference_tangent"] = ref
+            line["plastic"]["speedup_vs_reference_tangent"] = ref["wall_s"] / line["plastic"]["wall_s"]
+            log("plastic (reference tangent): %s" % json.dumps(ref))
     if args.pack_demo > 0:
         line["packing"] = pack_leg(args.pack_demo)
         log("packing: %s" % json.dumps(line["packing"]))

@@ -24,6 +24,8 @@ ERR_NEG_DETJ = -7
 ERR_STATE = -8
 ERR_UNSUPPORTED = -9
 ERR_CAPACITY = -10
+TANGENT_CONTINUUM = 0
+TANGENT_ALGORITHMIC = 1
 
 MAT_ISO_ELASTIC = 1
 MAT_CMAT3D = 2
@@ -66,7 +68,7 @@ EXPORTS = [
     "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_pcg_solve_noalloc", "bhip_spmv",
     "bhip_solve_cases",
     "bhip_bench_spmv", "bhip_bench_stream", "bhip_bench_read", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
-    "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state",
+    "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state", "bhip_plastic_set_tangent",
     "bhip_server_create", "bhip_server_destroy", "bhip_server_submit", "bhip_server_poll", "bhip_server_wait",
     "bhip_server_budget", "bhip_request_footprint", "bhip_heat_run", "bhip_write_matrix", "bhip_write_csr_file",
     "bhip_set_csr", "bhip_read_matrix", "bhip_write_vector", "bhip_read_vector",
@@ -205,6 +207,7 @@ def load_library(path=LIB_PATH):
         "bhip_pack_solve_cases": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(_P_D), C.POINTER(_P_D),
                                             C.POINTER(_P_D), C.POINTER(SolveOpts), C.POINTER(Stats), _P_I]),
         "bhip_plastic_init": (C.c_int, [vp]),
+        "bhip_plastic_set_tangent": (C.c_int, [vp, C.c_int]),
         "bhip_plastic_update": (C.c_int, [vp, _P_D]),
         "bhip_plastic_residual": (C.c_int, [vp, _P_D, _P_D]),
         "bhip_plastic_commit": (C.c_int, [vp]),
@@ -457,6 +460,10 @@ class Context:
     # ---- plasticity
     def plastic_init(self):
         self._chk(self.lib.bhip_plastic_init(self.h))
+
+    def plastic_set_tangent(self, tangent):
+        """TANGENT_CONTINUUM (the reference's C_ep) or TANGENT_ALGORITHMIC."""
+        self._chk(self.lib.bhip_plastic_set_tangent(self.h, tangent))
 
     def plastic_update(self, du):
         self._chk(self.lib.bhip_plastic_update(self.h, _d(_f64(du))))

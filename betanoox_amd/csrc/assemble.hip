@@ -9,8 +9,8 @@
 //
 // MI355X design, two stages, no colouring and no atomics:
 //  1. element stage (k_elem_blocks): a workgroup takes 16 elements, stages
-//     their coordinates, materials and Gauss-point gradients (and J2 tangent
-//     descriptors) in LDS and integrates each element once into its 36
+//     their coordinates, materials and Gauss-point gradients (and J2
+//     tangents) in LDS and integrates each element once into its 36
 //     canonical 3x3 blocks (one per unordered pair of local nodes, rows of the
 //     lower global node), written to HBM;
 //  2. gather stage (k_asm_pairs): one thread per (node n, neighbour m) entry
@@ -35,7 +35,7 @@ struct AsmArgs {
     const int *elem_group;
     int n_el, n_groups, mat_stride, nnpe, nip;
     double factor;
-    // plasticity: per (element, ip) tangent descriptor: sp[6], 1/denom, flag
+    // plasticity: per (element, ip) symmetric 6x6 tangent, upper triangle row by row
     const double *tan;
     int *flags;
     // Gauss tables (ctx->gauss): weights, derivatives [ip][xi|eta|zeta][8]
@@ -65,7 +65,7 @@ __device__ __forceinline__ int group_of(const AsmArgs &A, int e) {
 // The assembly runs in two stages.  The element stage integrates every
 // element once: a workgroup stages its elements' nodal coordinates and
 // material in LDS, then the global shape-function gradients and weights of
-// every Gauss point (and, for J2, the tangent descriptor of the point), then
+// every Gauss point (and, for J2, the 6x6 tangent of the point), then
 // computes each element's canonical blocks -- one per unordered pair of local
 // nodes, oriented with the rows of the lower global node index -- as the sum
 // over Gauss points of B_a^T C B_b detJ w (the reference's per-element Ke,
@@ -92,15 +92,40 @@ __device__ __forceinline__ void pair_of_index(int nn, int pr, int &i, int &j) {
     j = i + rem;
 }
 
-// 3D elasticity, isotropic (E, nu) or full 6x6 C; J2 adds the plastic
-// correction of the consistent tangent.  Block of B_a^T C B_b with Voigt order
+// Index of entry (s, u) of a symmetric 6x6 stored as its upper triangle row
+// by row (21 values).
+__host__ __device__ constexpr int sym_index(int s, int u) {
+    return s <= u ? s * 6 - s * (s - 1) / 2 + (u - s) : u * 6 - u * (u - 1) / 2 + (s - u);
+}
+
+// Block B_a^T C B_b of a general 6x6 C (Voigt xx yy zz xy yz xz, engineering
+// shear): CB = C B_b (6x3), then B_a^T CB.
+template <class CF>
+__device__ __forceinline__ void ctb_block(CF C, const double *ga, const double *gb, double *blk) {
+    double CB[6][3];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        CB[s][0] = gb[0] * C(s, 0) + gb[1] * C(s, 3) + gb[2] * C(s, 5);
+        CB[s][1] = gb[1] * C(s, 1) + gb[0] * C(s, 3) + gb[2] * C(s, 4);
+        CB[s][2] = gb[2] * C(s, 2) + gb[1] * C(s, 4) + gb[0] * C(s, 5);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        blk[0 * 3 + d] = ga[0] * CB[0][d] + ga[1] * CB[3][d] + ga[2] * CB[5][d];
+        blk[1 * 3 + d] = ga[1] * CB[1][d] + ga[0] * CB[3][d] + ga[2] * CB[4][d];
+        blk[2 * 3 + d] = ga[2] * CB[2][d] + ga[1] * CB[4][d] + ga[0] * CB[5][d];
+    }
+}
+
+// 3D elasticity, isotropic (E, nu) or full 6x6 C; J2 takes each Gauss point's
+// own symmetric tangent from the last return map.  Block of B_a^T C B_b with Voigt order
 // xx yy zz xy yz xz and engineering shear (ElasticityElement3D::
 // calculateBmatrix, ElasticityElement3D.cpp:57-87).
 template <int KIND>
 struct ElasHex8 {
     static constexpr int ND = 3, NN = 8, DIM = 3, NB = 9, NBS = 9, NP = 36;
     static constexpr bool J2 = KIND == BHIP_MAT_J2;
-    static constexpr int G = NN * DIM + 1 + (J2 ? 8 : 0);
+    static constexpr int G = NN * DIM + 1 + (J2 ? kTanVals : 0);
     static constexpr int MS = KIND == BHIP_MAT_CMAT3D ? 36 : 2;
     static __device__ __forceinline__ void stage_mat(const AsmArgs &A, int e, double *m) {
         const double *mp = A.mat + (long long)group_of(A, e) * A.mat_stride;
@@ -147,9 +172,9 @@ struct ElasHex8 {
         }
         g[24] = detJ * A.gw[ip] * A.factor;
         if (J2) {
-            const double *t = A.tan + ((long long)e * A.nip + ip) * 8;
+            const double *t = A.tan + ((long long)e * A.nip + ip) * kTanVals;
 #pragma unroll
-            for (int i = 0; i < 8; i++) g[25 + i] = t[i];
+            for (int i = 0; i < kTanVals; i++) g[25 + i] = t[i];
         }
         return detJ < 0;
     }
@@ -159,42 +184,20 @@ struct ElasHex8 {
         const double *ga = g + 3 * a, *gb = g + 3 * b;
         const double f = g[24];
         double blk[9];
-        if (KIND == BHIP_MAT_ISO_ELASTIC || J2) {
+        if (KIND == BHIP_MAT_ISO_ELASTIC) {
             const double lam = m[0], mu = m[1];
             double gg = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
 #pragma unroll
             for (int c = 0; c < 3; c++)
 #pragma unroll
                 for (int d = 0; d < 3; d++) blk[c * 3 + d] = lam * ga[c] * gb[d] + mu * gb[c] * ga[d] + (c == d ? gg : 0.0);
-            if (J2) {
-                const double *t = g + 25;
-                if (t[7] != 0.0) {
-                    double sa[3] = {ga[0] * t[0] + ga[1] * t[3] + ga[2] * t[5], ga[1] * t[1] + ga[0] * t[3] + ga[2] * t[4],
-                                    ga[2] * t[2] + ga[1] * t[4] + ga[0] * t[5]};
-                    double sb[3] = {gb[0] * t[0] + gb[1] * t[3] + gb[2] * t[5], gb[1] * t[1] + gb[0] * t[3] + gb[2] * t[4],
-                                    gb[2] * t[2] + gb[1] * t[4] + gb[0] * t[5]};
-#pragma unroll
-                    for (int c = 0; c < 3; c++)
-#pragma unroll
-                        for (int d = 0; d < 3; d++) blk[c * 3 + d] -= sa[c] * sb[d] * t[6];
-                }
-            }
+        } else if (J2) {
+            // the Gauss point's own symmetric tangent (upper triangle, row by row)
+            const double *t = g + 25;
+            ctb_block([t](int s, int u) { return t[sym_index(s, u)]; }, ga, gb, blk);
         } else {
-            // CB = C * B_b (6x3), then B_a^T * CB
             const double *C = m;
-            double CB[6][3];
-#pragma unroll
-            for (int s = 0; s < 6; s++) {
-                CB[s][0] = gb[0] * C[s * 6 + 0] + gb[1] * C[s * 6 + 3] + gb[2] * C[s * 6 + 5];
-                CB[s][1] = gb[1] * C[s * 6 + 1] + gb[0] * C[s * 6 + 3] + gb[2] * C[s * 6 + 4];
-                CB[s][2] = gb[2] * C[s * 6 + 2] + gb[1] * C[s * 6 + 4] + gb[0] * C[s * 6 + 5];
-            }
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                blk[0 * 3 + d] = ga[0] * CB[0][d] + ga[1] * CB[3][d] + ga[2] * CB[5][d];
-                blk[1 * 3 + d] = ga[1] * CB[1][d] + ga[0] * CB[3][d] + ga[2] * CB[4][d];
-                blk[2 * 3 + d] = ga[2] * CB[2][d] + ga[1] * CB[4][d] + ga[0] * CB[5][d];
-            }
+            ctb_block([C](int s, int u) { return C[s * 6 + u]; }, ga, gb, blk);
         }
 #pragma unroll
         for (int i = 0; i < 9; i++) K[i] += blk[i] * f;
@@ -536,7 +539,10 @@ int upload_gauss(bhip_ctx *ctx, int nnpe, int order, int *nip_out) {
 // Elements per workgroup of the element stage and its LDS bytes.
 template <class P>
 int elem_stage_epw(int nip) {
-    return std::max(1, std::min(16, 256 / std::max(nip, 1)));
+    // at most 16 elements, one thread per Gauss point, and 60 KiB of LDS
+    const size_t per_el = sizeof(double) * ((size_t)P::DIM * P::NN + P::MS + (size_t)nip * P::G);
+    const int by_lds = (int)((60 * 1024) / per_el);
+    return std::max(1, std::min(std::min(16, 256 / std::max(nip, 1)), by_lds));
 }
 template <class P>
 size_t elem_stage_lds(int epw, int nip) {

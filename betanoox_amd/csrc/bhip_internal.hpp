@@ -96,6 +96,7 @@ struct PcgState {
 constexpr int kMaxNnpe = 8;
 constexpr int kMaxIp = 64;    // order <= 4 in 3D
 constexpr int kRedBlocks = 2048;  // upper bound of grid-stride reduction grids
+constexpr int kTanVals = 21;  // J2 tangent per Gauss point: upper triangle of a symmetric 6x6
 constexpr int kGraphKeyWords = 40;  // words of a captured iteration graph's key
 
 struct bhip_ctx {

@@ -15,9 +15,15 @@
 // map, yield function and tangent run in FP64 at every Gauss point.
 //
 // Per Gauss point record (16 doubles): stress[6], eps_bar, dstress[6],
-// deps_bar, plastic flag, return-map iterations.  Tangent descriptor
-// (8 doubles, read by the assembly kernel): sp = C nf [6], 1/denom, flag, so
-// that C_ep = C - sp sp^T / denom.
+// deps_bar, plastic flag, return-map iterations.  Tangent (21 doubles, the
+// upper triangle of a symmetric 6x6 row by row, read by the assembly
+// kernel): the elastic C at an elastic point; at a plastic point either the
+// reference's continuum C_ep = C - sp sp^T / denom, sp = C nf
+// (getTangentStiffness, plastic.cpp:583-610), or, with
+// BHIP_TANGENT_ALGORITHMIC, the symmetric part of the exact derivative of the
+// stress update as executed (forward-mode differentiation through its
+// iterations), which gives Newton quadratic convergence where the update is
+// smooth.
 #include <algorithm>
 #include <cmath>
 
@@ -27,8 +33,9 @@
 
 struct PlasticData {
     int nip = 0;
+    int tangent = BHIP_TANGENT_CONTINUUM;
     dvec<double> isv;     // n_el * nip * 16
-    dvec<double> tan;     // n_el * nip * 8
+    dvec<double> tan;     // n_el * nip * kTanVals
     dvec<double> du;      // increment displacement (currentSolution), n_dof
     dvec<double> utot;    // accumulated displacement, n_dof
     dvec<double> corr;    // Newton correction, n_dof
@@ -160,10 +167,17 @@ __device__ __forceinline__ int group_of(const int *elem_group, int n_groups, int
     return n_groups == n_el ? e : 0;
 }
 
+// dYield applied to any 6-vector (it is linear in the stress).
+__device__ __forceinline__ void flow_lin(const double *s, double *nf) {
+    const double zero[6] = {0, 0, 0, 0, 0, 0};
+    dyield(zero, s, nf);
+}
+
 // "V": strain increment + return map (getStressIncrement, plastic.cpp:368-465).
+// alg: write the symmetric part of the return map's derivative as the tangent.
 __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ gtab, const int *conn, const double *xyz,
                                  const double *mat, const int *elem_group, int n_groups, const double *du,
-                                 double *isv, double *tan) {
+                                 double *isv, double *tan, int alg) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n_el * nip) return;
     int e = (int)(t / nip), ip = (int)(t - (long long)e * nip);
@@ -198,6 +212,9 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
     double dep = 0.0;
     int iter = 0, plastic = 0;
     double inv_denom = 0.0;
+    // D = d(ds)/d(de), column-major by strain component (alg only)
+    double D[36];
+    for (int i = 0; i < 36; i++) D[i] = m.C[(i % 6) * 6 + i / 6];
     if (yield_fn(m, s0, ds, ep + dep) > 0.0) {
         plastic = 1;
         double f;
@@ -205,7 +222,26 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
             double hard = hard_mod(m, s0, ds);
             dyield(s0, ds, nf);
             cmul(m.C, nf, sp);
-            double dl = dot6(nf, dse) / (hard + dot6(nf, sp));
+            const double den = hard + dot6(nf, sp);
+            double dl = dot6(nf, dse) / den;
+            if (alg) {
+                // per strain component j, with s = s0 + ds:
+                //   d nf = P D_j, d num = dse.(P D_j) + nf.C_j,
+                //   d den = (4/3) H nf.D_j + 2 sp.(P D_j),
+                //   D_j' = C_j - sp d dl - dl C P D_j
+                for (int j = 0; j < 6; j++) {
+                    double pc[6], cpc[6];
+                    flow_lin(D + 6 * j, pc);
+                    cmul(m.C, pc, cpc);
+                    double dnum = 0, dden = 0;
+                    for (int i = 0; i < 6; i++) {
+                        dnum += dse[i] * pc[i] + nf[i] * m.C[i * 6 + j];
+                        dden += 4.0 / 3.0 * m.H * nf[i] * D[6 * j + i] + 2.0 * sp[i] * pc[i];
+                    }
+                    const double ddl = (dnum - dl * dden) / den;
+                    for (int i = 0; i < 6; i++) D[6 * j + i] = m.C[i * 6 + j] - sp[i] * ddl - dl * cpc[i];
+                }
+            }
             for (int i = 0; i < 6; i++) ds[i] = dse[i] - dl * sp[i];
             dep = 2.0 * dl * sqrt(3.0 * hill(s0, ds)) / 3.0;
             f = yield_fn(m, s0, ds, ep + dep);
@@ -219,10 +255,14 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
     rec[13] = dep;
     rec[14] = plastic;
     rec[15] = iter;
-    double *tr = tan + t * 8;
-    for (int i = 0; i < 6; i++) tr[i] = plastic ? sp[i] : 0.0;
-    tr[6] = inv_denom;
-    tr[7] = plastic;
+    double *tr = tan + t * kTanVals;
+    int k = 0;
+    for (int i = 0; i < 6; i++)
+        for (int j = i; j < 6; j++, k++) {
+            double v = m.C[i * 6 + j];
+            if (plastic) v = alg ? 0.5 * (D[6 * j + i] + D[6 * i + j]) : v - sp[i] * sp[j] * inv_denom;
+            tr[k] = v;
+        }
 }
 
 // "F", element stage: the internal force vector of element e,
@@ -268,12 +308,12 @@ __global__ void k_plastic_residual(int n_nodes, int n_el, const int *n2e_ptr, co
     double acc[3] = {0, 0, 0};
     for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; k++) {
         const long long e = n2e[k];
-        int a = 0;
+        // every local node of e that is n (a collapsed element lists a node twice)
 #pragma unroll
-        for (int i = 0; i < 8; i++)
-            if (conn[e * 8 + i] == n) a = i;
+        for (int a = 0; a < 8; a++)
+            if (conn[e * 8 + a] == n)
 #pragma unroll
-        for (int c = 0; c < 3; c++) acc[c] += fe[(long long)(3 * a + c) * n_el + e];
+                for (int c = 0; c < 3; c++) acc[c] += fe[(long long)(3 * a + c) * n_el + e];
     }
     unsigned rm = rmask[n];
     for (int c = 0; c < 3; c++) {
@@ -372,12 +412,16 @@ int plastic_check(bhip_ctx *ctx) {
     return BHIP_OK;
 }
 
-int plastic_update_dev(bhip_ctx *ctx, const double *du) {
+// alg: the algorithmic tangent for this pass (the first pass of an
+// increment, with only the prescribed increment applied, takes the
+// continuum one: from that trial state the derivative softens every
+// deviatoric direction of the boundary layer and sends Newton far off).
+int plastic_update_dev(bhip_ctx *ctx, const double *du, bool alg = false) {
     PlasticData *P = ctx->plastic;
     long long npts = (long long)ctx->n_el * P->nip;
     k_plastic_update<<<grid_for(npts, 128), 128, 0, ctx->stream>>>(
         ctx->n_el, P->nip, ctx->gauss.p, ctx->conn.p, ctx->xyz.p, ctx->mat.p, ctx->elem_group.n ? ctx->elem_group.p : nullptr,
-        ctx->n_groups, du, P->isv.p, P->tan.p);
+        ctx->n_groups, du, P->isv.p, P->tan.p, alg ? 1 : 0);
     BHIP_TRY(ctx, hipGetLastError());
     return BHIP_OK;
 }
@@ -431,7 +475,7 @@ int bhip_plastic_init(bhip_ctx *ctx) {
     P->nip = ctx->order * ctx->order * ctx->order;
     long long npts = (long long)ctx->n_el * P->nip;
     BHIP_TRY(ctx, P->isv.resize(npts * kIsv));
-    BHIP_TRY(ctx, P->tan.resize(npts * 8));
+    BHIP_TRY(ctx, P->tan.resize(npts * kTanVals));
     BHIP_TRY(ctx, P->du.resize(ctx->n_dof));
     BHIP_TRY(ctx, P->utot.resize(ctx->n_dof));
     BHIP_TRY(ctx, P->corr.resize(ctx->n_dof));
@@ -448,13 +492,22 @@ int bhip_plastic_init(bhip_ctx *ctx) {
     return BHIP_OK;
 }
 
+int bhip_plastic_set_tangent(bhip_ctx *ctx, int tangent) {
+    if (!ctx) return BHIP_ERR_ARG;
+    BHIP_PROPAGATE(plastic_check(ctx));
+    if (tangent != BHIP_TANGENT_CONTINUUM && tangent != BHIP_TANGENT_ALGORITHMIC)
+        return bhip_fail(ctx, BHIP_ERR_ARG, "unknown tangent");
+    ctx->plastic->tangent = tangent;
+    return BHIP_OK;
+}
+
 int bhip_plastic_update(bhip_ctx *ctx, const double *du) {
     if (!ctx || !du) return BHIP_ERR_ARG;
     BHIP_PROPAGATE(plastic_check(ctx));
     (void)hipSetDevice(ctx->device);
     PlasticData *P = ctx->plastic;
     BHIP_TRY(ctx, hipMemcpyAsync(P->du.p, du, sizeof(double) * ctx->n_dof, hipMemcpyHostToDevice, ctx->stream));
-    BHIP_PROPAGATE(plastic_update_dev(ctx, P->du.p));
+    BHIP_PROPAGATE(plastic_update_dev(ctx, P->du.p, P->tangent == BHIP_TANGENT_ALGORITHMIC));
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return BHIP_OK;
 }
@@ -502,7 +555,7 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
     int it = 0;
     for (;;) {
         // "V" then assembleKandF ("KI") then getGlobalForces ("F")
-        BHIP_PROPAGATE(plastic_update_dev(ctx, P->du.p));
+        BHIP_PROPAGATE(plastic_update_dev(ctx, P->du.p, P->tangent == BHIP_TANGENT_ALGORITHMIC && it > 0));
         BHIP_PROPAGATE(bhip_assemble_impl(ctx, BHIP_ASM_PLASTIC_TANGENT, nullptr, nullptr));
         BHIP_PROPAGATE(plastic_residual_dev(ctx, &maxres));
         if (it > 0 && maxres < 2.0 * max_allowable_residual) break;

@@ -275,6 +275,16 @@ int bhip_plastic_init(bhip_ctx *ctx);
 /* update("V") on every element (plasticElement.cpp:177-180): strain
  * increment from du (n_dof, increment displacement) and the FP64 return map
  * (PlasticMaterial::getStressIncrement, plastic.cpp:368-465). */
+/* Tangent of BHIP_ASM_PLASTIC_TANGENT: the reference's continuum C_ep
+ * (getTangentStiffness, plastic.cpp:583-610; the default) or the symmetric
+ * part of the exact derivative of the return map as executed (no reference
+ * counterpart: it leaves the stress update and so the converged state alone
+ * and changes only how fast Newton gets there).  bhip_plastic_increment takes
+ * the continuum tangent on the first pass of every increment either way;
+ * bhip_plastic_update applies the selected one. */
+#define BHIP_TANGENT_CONTINUUM   0
+#define BHIP_TANGENT_ALGORITHMIC 1
+int bhip_plastic_set_tangent(bhip_ctx *ctx, int tangent);
 int bhip_plastic_update(bhip_ctx *ctx, const double *du);
 /* getGlobalForces + calculateResidualNorm (BasicModel.cpp:1063-1110,
  * equation.cpp:1516-1529): residual = load - resultant of the stress

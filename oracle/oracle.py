@@ -426,7 +426,9 @@ def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, s
     iteration count of the committed records).  presc_list optionally gives the
     prescribed increment vector of every increment (default: prob.prescribed).
     tangent selects the reference's continuum C_ep (default) or the symmetric
-    part of the return map's derivative."""
+    part of the return map's derivative on every pass after the first of an
+    increment (the first pass, with only the prescribed increment applied,
+    takes the continuum tangent)."""
     L = _plastic_sigs()
     n = prob.n_dof
     order = prob.order
@@ -454,7 +456,8 @@ def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, s
         while True:
             for e in range(prob.n_el):
                 rec = isv[e]
-                L.orc_plastic_update_tangent(d(xyz_e[e]), d(p), order, d(f64(du[maps[e]])), d(rec), tangent)
+                L.orc_plastic_update_tangent(d(xyz_e[e]), d(p), order, d(f64(du[maps[e]])), d(rec),
+                                             tangent if it > 0 else TANGENT_CONTINUUM)
             L.orc_eq_zero(eq.h)
             res = np.zeros(n)
             for e in range(prob.n_el):
@@ -479,3 +482,30 @@ def plastic_analysis(prob, increments, max_allowable_residual, max_iterations, s
         utot += du
         hist.append((it, maxres, int((isv[:, :, 14] != 0).sum()), int(isv[:, :, 15].max())))
     return utot, isv, hist
+
+
+def plastic_tangent_system(prob, du, tangent=TANGENT_CONTINUUM, isv=None):
+    """One "V" (return map of the increment displacement du from the records
+    isv, zero when None) and the "KI" assembly of PlasticElement::update
+    (plasticElement.cpp:39-250) over the oracle's sparse system; returns the
+    Equations object and the updated records."""
+    L = _plastic_sigs()
+    order = prob.order
+    nip = order ** 3
+    p = f64(prob.mat[0][:4])
+    first, nd = dof_numbering(prob.n_nodes, 3)
+    eq = Equations(prob.n_dof)
+    eq.set_restraints(prob.restrained, prob.prescribed)
+    eq.specify_mesh(prob.conn, first, nd)
+    conn = ai32(prob.conn)
+    maps = (3 * conn[:, :, None] + np.arange(3)[None, None, :]).reshape(conn.shape[0], 24).astype(np.int32)
+    if isv is None:
+        isv = np.zeros((prob.n_el, nip, PISV))
+    Ke, Fe = np.zeros(576), np.zeros(24)
+    for e in range(prob.n_el):
+        xe = f64(prob.xyz[conn[e]]).ravel().copy()
+        L.orc_plastic_update_tangent(d(xe), d(p), order, d(f64(du[maps[e]])), d(isv[e]), tangent)
+        L.orc_plastic_ke_fe(d(xe), d(p), order, d(isv[e]), d(Ke), d(Fe))
+        if L.orc_eq_add_matrix(eq.h, d(Ke), i32(maps[e]), 24, None) != OK:
+            raise RuntimeError("plastic scatter failed")
+    return eq, isv
