@@ -1,6 +1,12 @@
-"""Autotuner of the SpMV tile shapes.
+"""Autotuner of the SpMV tile shapes and of the server's batch shape.
 
-Two levels:
+Batch shape (tune_server): the served solver's slot count (requests on the
+device at once) and pack size (small requests solved together in one packed
+batched PCG) are picked by timing closed-loop runs of a probe set of
+requests, every combination of the candidate values, and keeping the one with
+the highest capacity (requests per second with the whole probe queued).
+
+SpMV tile shapes, two levels:
   * at run time, bhip_tune_spmv (Context.tune_spmv) times every SELL tile
     shape inside inner PCG iterations on a context's own matrix and keeps the
     shape with the fastest iteration;
@@ -51,6 +57,40 @@ TIE_US = 0.2
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def closed_loop(reqs, slots, max_pack, precision=None, tol=1e-8):
+    """Serve every request of `reqs` queued at once on a fresh server;
+    returns (seconds, per-request stats)."""
+    from . import capi
+    prec = capi.PREC_FP32_IR if precision is None else precision
+    with capi.Server(0, max_slots=slots, tol=tol, max_pack=max_pack, precision=prec) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(r) for r in reqs]
+        sts = [srv.wait(i)[1] for i in ids]
+        return time.perf_counter() - t0, sts
+
+
+def tune_server(probe, slots_list=(2, 4, 8), pack_list=(1, 4, 8, 16), reps=1):
+    """Batch shape of the served solver: closed-loop capacity of the probe
+    requests for every (slots, max_pack); returns (best, table), best =
+    {"slots", "max_pack", "req_per_s"}.  A warm-up run on the first
+    combination comes first (kernel loading, host allocations).  Capacities
+    within 2% of the best tie; the tie goes to the fewest slots."""
+    closed_loop(probe[: max(2, len(probe) // 4)], slots_list[0], pack_list[0])
+    table = []
+    for slots in slots_list:
+        for mp in pack_list:
+            best_t = min(closed_loop(probe, slots, mp)[0] for _ in range(reps))
+            table.append({"slots": slots, "max_pack": mp, "req_per_s": len(probe) / best_t})
+            log("server tuning: slots %d, max_pack %d: %.2f req/s" % (slots, mp, table[-1]["req_per_s"]))
+    # capacities within 2% count as equal (run-to-run spread of a 32-request
+    # probe); among them the fewest slots, then the smallest pack, win: fewer
+    # requests share the GPU, so each finishes sooner (lower tail latency)
+    top = max(r["req_per_s"] for r in table)
+    near = [r for r in table if r["req_per_s"] >= 0.98 * top]
+    best = min(near, key=lambda r: (r["slots"], r["max_pack"]))
+    return best, table
 
 
 def _assembled(ctx, gen, n):

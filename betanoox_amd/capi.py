@@ -66,6 +66,7 @@ EXPORTS = [
     "bhip_get_csr", "bhip_set_materials", "bhip_set_integration", "bhip_assemble",
     "bhip_get_pseudo_load", "bhip_set_load", "bhip_build_rhs", "bhip_get_rhs", "bhip_set_rhs",
     "bhip_set_x0", "bhip_solve", "bhip_get_solution", "bhip_pcg_solve", "bhip_pcg_solve_noalloc", "bhip_spmv",
+    "bhip_internal_force", "bhip_set_zero_diag_replacement",
     "bhip_solve_cases",
     "bhip_bench_spmv", "bhip_bench_stream", "bhip_bench_read", "bhip_plastic_init", "bhip_plastic_update", "bhip_plastic_residual",
     "bhip_plastic_commit", "bhip_plastic_increment", "bhip_plastic_get_state", "bhip_plastic_set_tangent",
@@ -86,7 +87,7 @@ class SolveOpts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("iters", C.c_int), ("outer_iters", C.c_int), ("ref_norm", C.c_double),
                 ("rel_resid", C.c_double), ("ms_total", C.c_double), ("ms_spmv_avg", C.c_double),
-                ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double)]
+                ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double), ("case_iters", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -110,7 +111,7 @@ class RequestStats(C.Structure):
                 ("outer_iters", C.c_int), ("rel_resid", C.c_double), ("queue_ms", C.c_double),
                 ("service_ms", C.c_double), ("latency_ms", C.c_double), ("start_ms", C.c_double),
                 ("end_ms", C.c_double), ("pack_size", C.c_int), ("setup_ms", C.c_double),
-                ("solve_ms", C.c_double), ("message", C.c_char * 256)]
+                ("solve_ms", C.c_double), ("case_iters", C.c_longlong), ("message", C.c_char * 256)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "message"}
@@ -194,6 +195,8 @@ def load_library(path=LIB_PATH):
         "bhip_pcg_solve": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_pcg_solve_noalloc": (C.c_int, [vp, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_spmv": (C.c_int, [vp, _P_D, _P_D]),
+        "bhip_internal_force": (C.c_int, [vp, C.c_int, _P_D, _P_D, _P_D, _P_D]),
+        "bhip_set_zero_diag_replacement": (C.c_int, [vp, C.c_double]),
         "bhip_solve_cases": (C.c_int, [vp, C.c_int, _P_D, _P_D, _P_D, C.POINTER(SolveOpts), C.POINTER(Stats)]),
         "bhip_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, _P_D, _P_D]),
         "bhip_bench_stream": (C.c_int, [vp, C.c_longlong, C.c_int, _P_D, _P_D]),
@@ -411,6 +414,22 @@ class Context:
         d = s.as_dict()
         d["status"] = st
         return bb, d
+
+    def internal_force(self, u):
+        """BasicModel::getGlobalForces: (force, residual, max |residual|) for
+        one displacement vector or each row of an (nvec, n_dof) array."""
+        U = np.ascontiguousarray(np.atleast_2d(u), dtype=np.float64)
+        if U.shape[1] != self.n_dof:
+            raise ValueError("u must be (nvec, n_dof)")
+        F, R, M = np.zeros_like(U), np.zeros_like(U), np.zeros(U.shape[0])
+        self._chk(self.lib.bhip_internal_force(self.h, U.shape[0], _d(U), _d(F), _d(R), _d(M)))
+        if np.ndim(u) == 1:
+            return F[0], R[0], M[0]
+        return F, R, M
+
+    def set_zero_diag_replacement(self, value):
+        """LargeMatrix::setReplaceZeroDiagonal_Option (0 = report zero diagonals)."""
+        self._chk(self.lib.bhip_set_zero_diag_replacement(self.h, value))
 
     def solve_cases(self, presc, load=None, opts=None, allow_not_converged=False, **kw):
         """Solve every load case of `presc` ((nrhs, n_dof) prescribed values)

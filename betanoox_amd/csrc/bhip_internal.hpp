@@ -185,6 +185,7 @@ struct bhip_ctx {
     dvec<double> partials;  // kRedBlocks * 4
     dvec<PcgState> state;
     dvec<int> flags;        // assembly / diagonal check flags
+    double zero_diag_value = 0.0;  // ReplaceZeroDiagonal value (0: zero diagonals are an error)
     PcgState *h_state = nullptr;  // pinned mirror
     unsigned char *scratch = nullptr;  // device address of this GPU's allocation-free scratch (solver.hip)
     // ---- graphs: one captured chunk of iterations per iteration kind, with
@@ -345,3 +346,49 @@ void bhip_runs_from_csr64(bhip_ctx *ctx, const double *csr_vals, double *out);
 void bhip_runs_spmv64(bhip_ctx *ctx, const double *vals, const double *x, double *y);
 int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb);
 const double *bhip_plastic_tangent(bhip_ctx *ctx);
+
+// ------------------------------------------------ hex8 element helpers
+// (plastic.hip, forces.hip)
+// All 8 global gradients at ip (isoElement.cpp:324-372).
+__device__ __forceinline__ double hex8_all_grads(const double *dN, const double *x, const double *y,
+                                                 const double *z, int ip, double (*g)[3]) {
+    const double *dxi = dN + ip * 24, *deta = dxi + 8, *dzeta = dxi + 16;
+    double px1 = 0, px2 = 0, px3 = 0, py1 = 0, py2 = 0, py3 = 0, pz1 = 0, pz2 = 0, pz3 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        px1 += dxi[i] * x[i];
+        px2 += deta[i] * x[i];
+        px3 += dzeta[i] * x[i];
+        py1 += dxi[i] * y[i];
+        py2 += deta[i] * y[i];
+        py3 += dzeta[i] * y[i];
+        pz1 += dxi[i] * z[i];
+        pz2 += deta[i] * z[i];
+        pz3 += dzeta[i] * z[i];
+    }
+    double d1 = py2 * pz3 - py3 * pz2, d2 = px2 * pz3 - px3 * pz2, d3 = px2 * py3 - px3 * py2;
+    double d4 = py1 * pz3 - py3 * pz1, d5 = px1 * pz3 - px3 * pz1, d6 = px1 * py3 - px3 * py1;
+    double d7 = py1 * pz2 - py2 * pz1, d8 = px1 * pz2 - px2 * pz1, d9 = px1 * py2 - px2 * py1;
+    double detJ = px1 * d1 - py1 * d2 + pz1 * d3;
+    double inv = 1. / detJ;
+    double a11 = d1 * inv, a12 = -d4 * inv, a13 = d7 * inv;
+    double a21 = -d2 * inv, a22 = d5 * inv, a23 = -d8 * inv;
+    double a31 = d3 * inv, a32 = -d6 * inv, a33 = d9 * inv;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        g[i][0] = a11 * dxi[i] + a12 * deta[i] + a13 * dzeta[i];
+        g[i][1] = a21 * dxi[i] + a22 * deta[i] + a23 * dzeta[i];
+        g[i][2] = a31 * dxi[i] + a32 * deta[i] + a33 * dzeta[i];
+    }
+    return detJ;
+}
+
+__device__ __forceinline__ void load_coords(const double *xyz, const int *ec, double *x, double *y, double *z) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const double *q = xyz + 3 * (long long)ec[i];
+        x[i] = q[0];
+        y[i] = q[1];
+        z[i] = q[2];
+    }
+}

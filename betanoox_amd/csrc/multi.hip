@@ -321,6 +321,7 @@ int solve_in_turn(bhip_ctx *ctx, int nrhs, const bhip_solve_opts *opts, bhip_sta
         if (e != BHIP_OK && e != BHIP_ERR_NOT_CONVERGED) return e;
         if (e == BHIP_ERR_NOT_CONVERGED) st = e;
         S.iters += s1.iters;
+        S.case_iters += s1.iters;
         S.outer_iters = std::max(S.outer_iters, s1.outer_iters);
         S.spmv_launches += s1.spmv_launches;
         BHIP_TRY(ctx, hipMemcpyAsync(M.x64.p + r * n, ctx->x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));

@@ -591,6 +591,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
     const char *fu = getenv("BHIP_PACK_UNROLL");
     if (fu && (fu[0] == '1' || fu[0] == '2')) unr = fu[0] - '0';
     std::vector<int> total(n_all, 0), outer(n_all, 0), active(n_all, 1), ok(n_all, 0);
+    std::vector<long long> case_total(n_all, 0);
     std::vector<double> worst(n_all, 1.0);  // worst unconverged relative residual per system
     for (int i = 0; i < n_all; i++) {
         memset(&stats[i], 0, sizeof(bhip_stats));
@@ -762,7 +763,10 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
         for (int k = 0; k < ns; k++) {
             const int i = run[k];
             int it = 0;
-            for (int r = 0; r < R; r++) it = std::max(it, pe->h_st[k].iter[r]);
+            for (int r = 0; r < R; r++) {
+                it = std::max(it, pe->h_st[k].iter[r]);
+                case_total[i] += pe->h_st[k].iter[r];
+            }
             total[i] += it;
             outer[i] += 1;
             bhip_multi_enqueue_axpy(ctxs[i], R, s);
@@ -772,6 +776,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
     PK_TRY(hipStreamSynchronize(s));
     for (int i = 0; i < n_all; i++) {
         stats[i].iters = total[i];
+        stats[i].case_iters = case_total[i];
         stats[i].outer_iters = outer[i];
         status[i] = ok[i] ? BHIP_OK : BHIP_ERR_NOT_CONVERGED;
     }

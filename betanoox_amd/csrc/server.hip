@@ -222,6 +222,7 @@ void run_pack(bhip_server *srv, Slot &slot, std::vector<Request *> &pk) {
                 st = bhip_solve_cases(ctx, nc, r.presc.data() + (size_t)c0 * n_dof,
                                       r.load.empty() ? nullptr : r.load.data(), r.x.data() + (size_t)c0 * n_dof, &o, &s);
                 r.st.iters += s.iters;
+                r.st.case_iters += s.case_iters;
                 r.st.outer_iters = std::max(r.st.outer_iters, s.outer_iters);
                 r.st.rel_resid = std::max(r.st.rel_resid, s.rel_resid);
             }
@@ -254,6 +255,7 @@ void run_pack(bhip_server *srv, Slot &slot, std::vector<Request *> &pk) {
         int st = e != BHIP_OK ? e : BHIP_OK;
         r.st.solve_ms = solve_ms;
         r.st.iters = S[j].iters;
+        r.st.case_iters = S[j].case_iters;
         r.st.outer_iters = S[j].outer_iters;
         if (st == BHIP_OK) {
             bhip_stats fin{};

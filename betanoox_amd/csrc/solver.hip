@@ -29,7 +29,9 @@
 // per node row) when every node carries the same number B <= 3 of DOFs, and
 // FP32 CSR otherwise.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include <mutex>
@@ -903,17 +905,52 @@ int ensure_partials(bhip_ctx *ctx) {
     return BHIP_OK;
 }
 
+// LargeMatrix::CheckAndFixZeroDiagonal (largeMatrix.cpp:43-79): rows are
+// scanned in order up to the first zero diagonal; a negative one before it is
+// an error; zeros are an error unless a replacement value is set, which then
+// replaces every zero diagonal of the matrix.  flags[1] / flags[2] receive
+// the first zero / negative row (unsigned minimum, cleared to ~0 before).
+__global__ void k_diag_scan(int n, const int *diag_pos, const double *val, unsigned *first) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = val[diag_pos[i]];
+    if (d == 0.0) atomicMin(first, (unsigned)i);
+    else if (d < 0.0) atomicMin(first + 1, (unsigned)i);
+}
+
+__global__ void k_replace_zero_diag(int n, const int *diag_pos, double *val, double value) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && val[diag_pos[i]] == 0.0) val[diag_pos[i]] = value;
+}
+
 int prepare_diag(bhip_ctx *ctx) {
     const int B = 256;
-    BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p, 0, sizeof(int) * 4, ctx->stream));
-    k_dinv<<<grid_for(ctx->n_dof, B), B, 0, ctx->stream>>>(ctx->n_dof, ctx->diag_pos.p, ctx->val.p, ctx->dinv.p,
-                                                           ctx->dinv32.p, ctx->flags.p);
+    const int n = ctx->n_dof;
+    unsigned *first = reinterpret_cast<unsigned *>(ctx->flags.p + 1);
+    BHIP_TRY(ctx, hipMemsetAsync(first, 0xff, sizeof(unsigned) * 2, ctx->stream));
+    k_diag_scan<<<grid_for(n, B), B, 0, ctx->stream>>>(n, ctx->diag_pos.p, ctx->val.p, first);
     BHIP_TRY(ctx, hipGetLastError());
-    int hf = 0;
-    BHIP_TRY(ctx, hipMemcpyAsync(&hf, ctx->flags.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned hf[2] = {0, 0};
+    BHIP_TRY(ctx, hipMemcpyAsync(hf, first, sizeof(hf), hipMemcpyDeviceToHost, ctx->stream));
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (hf & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "negative diagonal term in the global matrix");
-    if (hf & 1) return bhip_fail(ctx, BHIP_ERR_ZERO_DIAG, "zero diagonal term in the global matrix");
+    const unsigned none = ~0u;
+    if (hf[1] != none && (hf[0] == none || hf[1] < hf[0]))
+        return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "negative diagonal term in the global matrix (row " +
+                                                     std::to_string(hf[1]) + ")");
+    if (hf[0] != none) {
+        if (ctx->zero_diag_value == 0.0)
+            return bhip_fail(ctx, BHIP_ERR_ZERO_DIAG, "zero diagonal term in the global matrix (row " +
+                                                          std::to_string(hf[0]) +
+                                                          "); set a replacement with bhip_set_zero_diag_replacement");
+        k_replace_zero_diag<<<grid_for(n, B), B, 0, ctx->stream>>>(n, ctx->diag_pos.p, ctx->val.p,
+                                                                   ctx->zero_diag_value);
+        BHIP_TRY(ctx, hipGetLastError());
+        ctx->f32_ready = ctx->f64b_ready = false;  // the values changed
+    }
+    BHIP_TRY(ctx, hipMemsetAsync(ctx->flags.p, 0, sizeof(int), ctx->stream));
+    k_dinv<<<grid_for(n, B), B, 0, ctx->stream>>>(n, ctx->diag_pos.p, ctx->val.p, ctx->dinv.p, ctx->dinv32.p,
+                                                  ctx->flags.p);
+    BHIP_TRY(ctx, hipGetLastError());
     return BHIP_OK;
 }
 
@@ -1473,6 +1510,7 @@ int bhip_solve_impl(bhip_ctx *ctx, const bhip_solve_opts *o, bhip_stats *stats) 
         BHIP_TRY(ctx, hipStreamSynchronize(s));
         S.rel_resid = ctx->h_state->bb > 0 ? sqrt(ctx->h_state->rr / ctx->h_state->bb) : sqrt(ctx->h_state->rr);
     }
+    S.case_iters = S.iters;
     if (stats) *stats = S;
     if (st_code == BHIP_ERR_NOT_CONVERGED)
         return bhip_fail(ctx, st_code, "Solution not converged within the iteration cap");
@@ -1535,6 +1573,13 @@ int bhip_scratch_setup(bhip_ctx *ctx) {
         g_scratch_dev[ctx->device] = static_cast<unsigned char *>(p);
     }
     ctx->scratch = g_scratch_dev[ctx->device];
+    return BHIP_OK;
+}
+
+extern "C" int bhip_set_zero_diag_replacement(bhip_ctx *ctx, double value) {
+    if (!ctx) return BHIP_ERR_ARG;
+    if (!std::isfinite(value)) return bhip_fail(ctx, BHIP_ERR_ARG, "replacement value must be finite");
+    ctx->zero_diag_value = value;
     return BHIP_OK;
 }
 
@@ -1618,6 +1663,7 @@ extern "C" int bhip_pcg_solve_noalloc(bhip_ctx *ctx, double *b, const double *x0
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipStreamSynchronize(s));
     S.rel_resid = ctx->h_state->bb > 0 ? sqrt(ctx->h_state->rr / ctx->h_state->bb) : sqrt(ctx->h_state->rr);
+    S.case_iters = S.iters;
     if (stats) *stats = S;
     BHIP_TRY(ctx, hipMemcpyAsync(b, ctx->x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipStreamSynchronize(s));

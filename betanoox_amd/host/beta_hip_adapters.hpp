@@ -78,6 +78,10 @@ class HipEquations {
     void setTolerance(double eps) { eps_ = eps; }         // setSparseSolverTolerance
     void setStopRule(int r) { stop_rule_ = r; }
     void setPrecision(int p) { precision_ = p; }
+    // ReplaceZeroDiagonal (equation.cpp:1819-1821 -> setReplaceZeroDiagonal_Option)
+    void setReplaceZeroDiagonal_Option(double v) {
+        dev_.check(bhip_set_zero_diag_replacement(dev_.ctx(), v), "setReplaceZeroDiagonal_Option");
+    }
     const bhip_stats &lastStats() const { return stats_; }
     Device &device() { return dev_; }
 
@@ -104,6 +108,9 @@ class HipLargeMatrix {
     void setSOLUTION_EPSILON(double e) { eps_ = e; }
     void setStopRule(int r) { stop_rule_ = r; }
     void setPrecision(int p) { precision_ = p; }
+    void setReplaceZeroDiagonal_Option(double v) {          // largeMatrix.cpp:43-79
+        dev_.check(bhip_set_zero_diag_replacement(dev_.ctx(), v), "setReplaceZeroDiagonal_Option");
+    }
     void solve(double *b, double *initialGuess);           // sparse.cpp:114-242
     void vectorProduct(double *result, double *vector);    // sparse.cpp:452-478
     double getaij(int i, int j);                            // sparse.cpp:80-99, i >= j

@@ -156,9 +156,9 @@ static int run_mesh_file(int argc, char **argv) {
         const bool hip_assembly = parseParallelAssembly(pa_token);
         MeshData mesh = mf.toMeshData();
         std::printf("n_nodes=%d\nn_el=%d\nnum_dims=%d\nnode_offset=%d\nstorage=%s\nprecision=%d\nmax_iterations=%d\n"
-                    "tolerance=%.6e\nparallel_assembly_hip=%d\n",
+                    "tolerance=%.6e\nparallel_assembly_hip=%d\nzero_diag_replacement=%.6e\n",
                     mf.n_nodes(), mf.n_el(), mf.num_dims, mf.node_offset, ss.storage.c_str(), ss.precision,
-                    ss.max_iterations, ss.tolerance, hip_assembly ? 1 : 0);
+                    ss.max_iterations, ss.tolerance, hip_assembly ? 1 : 0, ss.zero_diag_replacement);
         if (parse_only) return 0;
         Device dev(0);
         HipEquations eq(dev);

@@ -229,8 +229,9 @@ SolverSettings readSolverSettings(std::istream &in) {
             s.tolerance = std::atof(arg.c_str());
         } else if (cmd == "setSolverVerboseFlag") {
             s.verbose = std::atoi(arg.c_str());
-        } else if (cmd == "ReplaceZeroDiagonal" || cmd == "setReorderingScheme" ||
-                   cmd == "setUsePreviousFillInOrdering" || cmd == "UseMultiCore") {
+        } else if (cmd == "ReplaceZeroDiagonal") {
+            s.zero_diag_replacement = std::atof(arg.c_str());
+        } else if (cmd == "setReorderingScheme" || cmd == "setUsePreviousFillInOrdering" || cmd == "UseMultiCore") {
             // direct-solver and threading options: nothing to do for the device PCG
         } else {
             bad("Did not recognize command: " + cmd);
@@ -244,6 +245,7 @@ void applySolverSettings(const SolverSettings &s, HipEquations &eq) {
     eq.setTolerance(s.tolerance);
     eq.setStopRule(s.stop_rule);
     eq.setPrecision(s.precision);
+    eq.setReplaceZeroDiagonal_Option(s.zero_diag_replacement);
 }
 
 bool parseParallelAssembly(const std::string &token) {

@@ -61,6 +61,7 @@ struct SolverSettings {
     int max_iterations = 10000;      // equation.cpp:1566-1567 defaults
     double tolerance = 1e-4;
     int verbose = 0;
+    double zero_diag_replacement = 0.0;  // ReplaceZeroDiagonal (equation.cpp:1819-1821)
 };
 
 // Reads commands until "end" (Equations::readCommands).  Unknown commands

@@ -234,7 +234,9 @@ def _rve_base(n):
         cases.append(pv)
     for a in (xyz, conn, rest):
         a.flags.writeable = False
-    if len(_RVE_BASE) >= 16:
+    # every size of the stream (16..96) stays cached: rebuilding an evicted
+    # base cost ~90 ms per request; all 81 bases take ~5 GB of host memory
+    if len(_RVE_BASE) >= 128:
         _RVE_BASE.clear()
     b = {"xyz": xyz, "conn": conn, "cent": cent.reshape(n, n, n, 3), "rest": rest, "cases": cases}
     _RVE_BASE[n] = b

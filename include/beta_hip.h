@@ -85,6 +85,9 @@ typedef struct bhip_stats {
     double ms_spmv_avg;      /* mean duration of one SpMV kernel inside the solve */
     int    spmv_launches;    /* SpMV kernels launched in the timed region */
     double spmv_bytes;       /* algorithmic bytes of one SpMV */
+    long long case_iters;    /* iterations summed over load cases (bhip_solve_cases: a packed
+                                solve advances all its cases per iteration, so iters counts
+                                the longest case and case_iters every case); = iters otherwise */
 } bhip_stats;
 
 /* --------------------------------------------------------------- context */
@@ -179,6 +182,27 @@ int bhip_pcg_solve_noalloc(bhip_ctx *ctx, double *b, const double *x0, const bhi
                            bhip_stats *stats);
 /* LargeMatrix::vectorProduct (sparse.cpp:452-478): y = K x. */
 int bhip_spmv(bhip_ctx *ctx, const double *x, double *y);
+/* BasicModel::getGlobalForces (BasicModel.cpp:1063-1110) for hex8
+ * elasticity (BHIP_MAT_ISO_ELASTIC or BHIP_MAT_CMAT3D): for each of nvec
+ * displacement vectors u (nvec x n_dof, full displacements with the
+ * prescribed values on restrained DOFs), the internal force
+ * force = sum_e f_e(u) over every DOF (ElasticityElement3D update("F"); on a
+ * restrained DOF it is the reaction), optionally residual = load - force on
+ * free DOFs and 0 on restrained ones (calculateResidualNorm,
+ * equation.cpp:1516-1529) and max_abs = max |residual| per vector.  Applied
+ * to the displacements of bhip_solve_cases it gives the reactions that
+ * MultiFieldMacroElement::solveForStiffnessMatrixColumn takes as a column of
+ * the macro-element stiffness (MultiFieldMacroElement.cpp:217-219).
+ * residual and max_abs may be NULL. */
+int bhip_internal_force(bhip_ctx *ctx, int nvec, const double *u, double *force, double *residual,
+                        double *max_abs);
+/* LargeMatrix::setReplaceZeroDiagonal_Option (largeMatrix.cpp:43-79, set by
+ * the ReplaceZeroDiagonal command, equation.cpp:1819-1821): before a solve,
+ * zero diagonal entries of the matrix are replaced by value (0, the default:
+ * a zero diagonal is reported as BHIP_ERR_ZERO_DIAG).  As the reference, the
+ * diagonal is scanned in row order: a negative entry before the first zero is
+ * BHIP_ERR_NEG_DIAG. */
+int bhip_set_zero_diag_replacement(bhip_ctx *ctx, double value);
 /* Several load cases on the assembled stiffness: the multi-column solve of
  * MultiFieldMacroElement::CalculateK / solveForStiffnessMatrixColumn
  * (MultiFieldMacroElement.cpp:135-407), where each case re-lifts its prescribed
@@ -428,6 +452,7 @@ typedef struct bhip_request_stats {
     int    pack_size;        /* requests solved in the same packed solve (1: alone) */
     double setup_ms;         /* mesh, pattern, assembly and right-hand sides of this request */
     double solve_ms;         /* the (packed) solve it took part in */
+    long long case_iters;    /* inner iterations summed over the request's load cases */
     char   message[256];     /* error text when status != BHIP_OK */
 } bhip_request_stats;
 

@@ -11,6 +11,7 @@
 #include "beta_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -883,6 +884,129 @@ int orc_eq_pcg(orc_eq *eq, double *b, const double *x0, double eps, int maxit,
     free(q); free(p); free(temp); free(r); free(diag);
     if (iters) *iters = it;
     if (it == maxit) return ORC_NOT_CONVERGED;
+    return ORC_OK;
+}
+
+/* ParCGMatrix::vectorProduct (ParCG.cpp:504-616): the symmetric product from
+ * lower rows split at L1 = L/2 over two threads.  Section 1: thread 0 takes
+ * rows < L1 (all their columns), thread 1 rows >= L1 with columns >= L1;
+ * section 2: thread 0 scatters the transposed products of rows >= L1 into
+ * columns < L1, thread 1 adds the rows' own products with those columns.
+ * No two threads write one entry within a section. */
+static void parcg_spmv(const orc_eq *eq, const double *v, double *res)
+{
+    const int L = eq->n, L1 = L / 2;
+    for (int i = 0; i < L; i++) res[i] = 0;
+#pragma omp parallel num_threads(2)
+    {
+        const int t = omp_get_thread_num();
+        const int i0 = t == 0 ? 0 : L1, i1 = t == 0 ? L1 : L;
+        for (int i = i0; i < i1; i++) {
+            const int nt = eq->len[i] - 1;
+            if (nt < 0) continue;
+            const double vi = v[i];
+            const double *m = eq->val[i];
+            const int *c = eq->col[i];
+            double acc = 0;
+            for (int j = 0; j < nt; j++) {
+                if (t == 1 && c[j] < L1) continue;
+                acc += m[j] * v[c[j]];
+                res[c[j]] += m[j] * vi;
+            }
+            res[i] += m[nt] * v[c[nt]] + acc;
+        }
+    }
+#pragma omp parallel num_threads(2)
+    {
+        const int t = omp_get_thread_num();
+        for (int i = L1; i < L; i++) {
+            const int nt = eq->len[i] - 1;
+            if (nt < 0) continue;
+            const double vi = v[i];
+            const double *m = eq->val[i];
+            const int *c = eq->col[i];
+            double acc = 0;
+            for (int j = 0; j < nt; j++) {
+                if (c[j] >= L1) continue;
+                if (t == 0) res[c[j]] += m[j] * vi;
+                else acc += m[j] * v[c[j]];
+            }
+            if (t == 1) res[i] += acc;
+        }
+    }
+}
+
+/* ParCGMatrix::solve (ParCG.cpp:93-347): the Jacobi-PCG of sparse.cpp with
+ * its vector loops fused into three passes, each split at L1 = L/2 over two
+ * OpenMP threads with (+) reductions, and the ||r||/||x|| stop rule.  The
+ * CPU baseline's two-thread row; same iterates as orc_eq_pcg up to the
+ * order of the partial sums. */
+int orc_eq_pcg_parcg(orc_eq *eq, double *b, const double *x0, double eps, int maxit, int *iters,
+                     double *final_norm)
+{
+    const int L = eq->n, L1 = L / 2;
+    int st = orc_eq_check_fix_diag(eq);
+    if (st != ORC_OK) return st;
+    double *x = (double *)malloc(sizeof(double) * L);
+    double *r = (double *)malloc(sizeof(double) * L);
+    double *diag = (double *)malloc(sizeof(double) * L);
+    for (int i = 0; i < L; i++) diag[i] = eq->val[i][eq->len[i] - 1];
+    if (x0 == 0) for (int i = 0; i < L; i++) x[i] = 0;
+    else for (int i = 0; i < L; i++) x[i] = x0[i];
+    parcg_spmv(eq, x, r);
+    free(x);
+    double *p = (double *)malloc(sizeof(double) * L);
+    double *q = (double *)malloc(sizeof(double) * L);
+    double *temp = (double *)malloc(sizeof(double) * L);
+    for (int i = 0; i < L; i++) {
+        r[i] = b[i] - r[i];
+        p[i] = q[i] = r[i] / diag[i];
+    }
+    if (x0 == 0) for (int i = 0; i < L; i++) b[i] = 0;
+    else for (int i = 0; i < L; i++) b[i] = x0[i];
+    int it = 0, conv = 0;
+    while (!conv && it < maxit) {
+        parcg_spmv(eq, p, temp);
+        double pAp = 0, rq = 0;
+#pragma omp parallel num_threads(2) reduction(+ : pAp, rq)
+        {
+            const int t = omp_get_thread_num();
+            for (int i = t == 0 ? 0 : L1; i < (t == 0 ? L1 : L); i++) {
+                pAp = pAp + p[i] * temp[i];
+                rq = rq + r[i] * q[i];
+            }
+        }
+        const double a = rq / pAp;
+        double newrq = 0;
+#pragma omp parallel num_threads(2) reduction(+ : newrq)
+        {
+            const int t = omp_get_thread_num();
+            for (int i = t == 0 ? 0 : L1; i < (t == 0 ? L1 : L); i++) {
+                b[i] += a * p[i];
+                r[i] -= a * temp[i];
+                q[i] = r[i] / diag[i];
+                newrq = newrq + r[i] * q[i];
+            }
+        }
+        const double bk = newrq / rq;
+        double rnorm = 0, xnorm = 0;
+#pragma omp parallel num_threads(2) reduction(+ : rnorm, xnorm)
+        {
+            const int t = omp_get_thread_num();
+            for (int i = t == 0 ? 0 : L1; i < (t == 0 ? L1 : L); i++) {
+                p[i] = q[i] + bk * p[i];
+                rnorm = rnorm + r[i] * r[i];
+                xnorm = xnorm + b[i] * b[i];
+            }
+        }
+        it++;
+        const double nrm = sqrt(rnorm / xnorm);
+        if (nrm < eps) conv = 1;
+        if (final_norm) *final_norm = nrm;
+    }
+    free(q); free(p); free(temp); free(r); free(diag);
+    if (iters) *iters = it;
+    if (it == maxit && !conv) return ORC_NOT_CONVERGED;
     return ORC_OK;
 }
 

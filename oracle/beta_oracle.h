@@ -84,6 +84,9 @@ int       orc_eq_check_fix_diag(orc_eq *eq);
 int       orc_eq_pcg(orc_eq *eq, double *b, const double *x0, double eps, int maxit,
                      int stop_rule, int *iters, double *final_norm);
 int       orc_eq_skyline_solve(const orc_eq *eq, double *b);
+/* ParCGMatrix::solve (ParCG.cpp:93-347): two OpenMP threads, ||r||/||x|| stop rule */
+int       orc_eq_pcg_parcg(orc_eq *eq, double *b, const double *x0, double eps, int maxit,
+                           int *iters, double *final_norm);
 
 /* ---- element DOF map (BasicElement::calculateDofList) ---- */
 int orc_dof_list(int nnpe, const int *conn, const int *dof_first, const int *ndof, int *map);

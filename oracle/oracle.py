@@ -11,6 +11,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libbeta_oracle.so")
+# the CPU baseline's -O3 -march=native build (make -C oracle native), used
+# when ORACLE_NATIVE=1 is set before the first call
+NATIVE_PATH = os.path.join(HERE, "_build", "native", "libbeta_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libbeta_ref.so")
 
 _P_D = C.POINTER(C.c_double)
@@ -26,9 +29,10 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise OSError("oracle not built: make -C oracle")
-    L = C.CDLL(LIB_PATH)
+    path = NATIVE_PATH if os.environ.get("ORACLE_NATIVE") == "1" else LIB_PATH
+    if not os.path.exists(path):
+        raise OSError("oracle not built: make -C oracle (native: make -C oracle native)")
+    L = C.CDLL(path)
     vp = C.c_void_p
     sig = {
         "orc_gauss": (None, [C.c_int, _P_D, _P_D]),
@@ -75,6 +79,7 @@ def lib():
         "orc_assemble_heat_transient": (C.c_int, [vp, C.c_int, _P_I, _P_D, _P_I, _P_I, _P_D, _P_I, C.c_int,
                                                   C.c_double, C.c_double, _P_D]),
         "orc_j2_stress_increment": (C.c_int, [_P_D, _P_D, C.c_double, _P_D, _P_D, _P_D, _P_D, _P_I]),
+        "orc_eq_pcg_parcg": (C.c_int, [vp, _P_D, _P_D, C.c_double, C.c_int, _P_I, _P_D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -244,6 +249,14 @@ class Equations:
         it, nrm = C.c_int(), C.c_double()
         st = self.L.orc_eq_pcg(self.h, d(bb), d(None if x0 is None else f64(x0)), eps, maxit, stop_rule,
                                C.byref(it), C.byref(nrm))
+        return st, bb, it.value, nrm.value
+
+    def pcg_parcg(self, b, x0=None, eps=1e-8, maxit=10000):
+        """ParCGMatrix::solve: two threads, ||r||/||x|| stop rule."""
+        bb = f64(b).copy()
+        it, nrm = C.c_int(), C.c_double()
+        st = self.L.orc_eq_pcg_parcg(self.h, d(bb), d(None if x0 is None else f64(x0)), eps, maxit,
+                                     C.byref(it), C.byref(nrm))
         return st, bb, it.value, nrm.value
 
     def direct(self, b):

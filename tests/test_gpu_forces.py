@@ -1,0 +1,216 @@
+"""Internal forces and reactions (bhip_internal_force, BasicModel::
+getGlobalForces, BasicModel.cpp:1063-1110), the zero-diagonal replacement
+(LargeMatrix::CheckAndFixZeroDiagonal, largeMatrix.cpp:43-79), the general
+6x6 material and integration order 3, against the oracle.
+
+Tolerances:
+  - internal force: |f_gpu - f_orc|_inf <= 1e-12 |f_orc|_inf, f_orc the sum of
+    the oracle's element forces (orc_fe_hex8_elastic, pinned by K_e u_e)
+  - reactions of the six affine cases of a homogeneous RVE: the homogenized
+    stress (1/V) sum_a f_a x_a^T of case k equals C eps_k to 1e-8 of |C|
+    (closed form: the hex8 field reproduces affine displacements exactly)
+  - equilibrium of a solved model: free-DOF residual <= 1e-9 |load|_inf, the
+    reactions balance the applied load to 1e-9
+  - zero-diagonal replacement: the solution equals the oracle's, whose
+    Equations carries the same replacement value, to 1e-8
+  - BHIP_MAT_CMAT3D and order 3: K entries <= 1e-12 of the row's largest,
+    solutions <= 1e-8 against the oracle's FP64 PCG
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from betanoox_amd import capi, meshgen, setup_problem
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_force(prob, u, cmats=None):
+    L = O.lib()
+    L.orc_fe_hex8_elastic.restype = ctypes.c_int
+    L.orc_fe_hex8_elastic.argtypes = [O._P_D, O._P_D, ctypes.c_int, O._P_D, O._P_D]
+    conn = np.asarray(prob.conn)
+    f = np.zeros(prob.n_dof)
+    Fe = np.zeros(24)
+    for e in range(prob.n_el):
+        if cmats is not None:
+            Cm = cmats[e if len(cmats) == prob.n_el else 0]
+        else:
+            m = prob.mat[e if len(prob.mat) == prob.n_el else 0]
+            Cm = O.iso_cmat(m[0], m[1])
+        idx = (3 * conn[e][:, None] + np.arange(3)).ravel()
+        st = L.orc_fe_hex8_elastic(O.d(O.f64(prob.xyz[conn[e]]).ravel().copy()), O.d(O.f64(Cm).ravel().copy()),
+                                   prob.order, O.d(O.f64(u[idx]).copy()), O.d(Fe))
+        assert st == 0
+        np.add.at(f, idx, Fe)
+    return f
+
+
+def _row_err(rp, a, b):
+    err = 0.0
+    for i in range(len(rp) - 1):
+        s = slice(rp[i], rp[i + 1])
+        if rp[i + 1] > rp[i]:
+            err = max(err, np.abs(a[s] - b[s]).max() / max(np.abs(b[s]).max(), 1e-300))
+    return err
+
+
+def _random_cmats(rng, k):
+    out = []
+    for _ in range(k):
+        A = rng.standard_normal((6, 6))
+        out.append(A @ A.T + 6 * np.eye(6))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("material", ["iso", "cmat3d"])
+def test_internal_force_matches_oracle(hip_ctx_factory, material):
+    print("case internal force", material, flush=True)
+    prob = meshgen.elastic_cube(4, jitter=0.15)
+    cm = None
+    if material == "cmat3d":
+        cm = _random_cmats(np.random.default_rng(3), prob.n_el)
+        prob.mat_kind = capi.MAT_CMAT3D
+        prob.mat = cm.reshape(prob.n_el, 36)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.set_load(prob.load)
+    u = np.random.default_rng(4).standard_normal((2, prob.n_dof))
+    F, R, M = ctx.internal_force(u)
+    rest = prob.restrained.astype(bool)
+    for k in range(2):
+        f_o = _oracle_force(prob, u[k], cm)
+        assert np.abs(F[k] - f_o).max() <= 1e-12 * np.abs(f_o).max()
+        r_o = np.where(rest, 0.0, prob.load - f_o)
+        assert np.abs(R[k] - r_o).max() <= 1e-12 * np.abs(f_o).max()
+        assert abs(M[k] - np.abs(R[k]).max()) == 0.0
+
+
+def test_reactions_give_homogenized_stiffness(hip_ctx_factory):
+    """MultiFieldMacroElement's columns: the six affine load cases of a
+    homogeneous RVE, solved together (bhip_solve_cases, FP32 inner refined
+    to 1e-11), then their reactions; (1/V) sum_a f_a x_a^T is the stress of
+    each case, C eps_k with the isotropic C of the matrix material."""
+    prob = meshgen.rve_request(6, 7, n_incl=0)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble(capi.ASM_STIFFNESS)
+    P = np.stack(prob.extra["cases"])
+    o = capi.Context.opts(tol=1e-11, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR, maxit=20000)
+    X, st = ctx.solve_cases(P, opts=o)
+    F, R, M = ctx.internal_force(X)
+    assert M.max() <= 1e-9 * np.abs(F).max()
+    Cm = O.iso_cmat(3.0, 0.35)
+    Ch = np.zeros((6, 6))
+    x = prob.xyz
+    for k in range(6):
+        f = F[k].reshape(-1, 3)
+        S = f.T @ x  # S[i, j] = sum_a f_a,i x_a,j; V = 1
+        Ch[:, k] = [S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[1, 2], S[0, 2]]
+    assert np.abs(Ch - Cm).max() <= 1e-8 * np.abs(Cm).max()
+    assert np.abs(Ch - Ch.T).max() <= 1e-8 * np.abs(Cm).max()
+
+
+def test_solved_model_equilibrium(hip_ctx_factory):
+    prob = meshgen.elastic_cube(6)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    ctx.solve(tol=1e-12, precision=capi.PREC_FP64, maxit=20000)
+    u = ctx.solution()
+    f, r, m = ctx.internal_force(u)
+    scale = np.abs(prob.load).max()
+    assert m <= 1e-9 * scale
+    rest = prob.restrained.astype(bool)
+    for c in range(3):
+        comp = np.arange(prob.n_dof) % 3 == c
+        assert abs(f[rest & comp].sum() + prob.load[~rest & comp].sum()) <= 1e-9 * np.abs(prob.load).sum()
+
+
+def _isolated_node_problem():
+    prob = meshgen.elastic_cube(4)
+    prob.xyz = np.vstack([prob.xyz, [[2.0, 2.0, 2.0]]])
+    prob.restrained = np.concatenate([prob.restrained, [0, 0, 0]]).astype(np.uint8)
+    prob.prescribed = np.concatenate([prob.prescribed, [0.0, 0.0, 0.0]])
+    prob.load = np.concatenate([prob.load, [0.0, 0.0, 0.0]])
+    return prob
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_FP32_IR])
+def test_zero_diagonal_replacement(hip_ctx_factory, precision):
+    """A node without elements leaves three zero diagonals: refused without a
+    replacement value, solved with one (ReplaceZeroDiagonal 2.5), equal to
+    the oracle's Equations with the same option."""
+    prob = _isolated_node_problem()
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    with pytest.raises(capi.BhipError) as ei:
+        ctx.solve(tol=1e-10, precision=precision)
+    assert ei.value.status == capi.ERR_ZERO_DIAG
+    ctx.set_zero_diag_replacement(2.5)
+    s = ctx.solve(tol=1e-11, precision=precision, maxit=20000)
+    x = ctx.solution()
+    eq = O.assemble_problem(prob)
+    eq.L.orc_eq_set_replace_zero_diag(eq.h, 2.5)
+    st, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-13, stop_rule=1, maxit=20000)
+    assert st == 0
+    assert np.linalg.norm(x - x_o) <= 1e-8 * np.linalg.norm(x_o)
+    assert np.all(x[-3:] == 0.0)
+    rp, ci = ctx.pattern_lower()
+    v = ctx.values_lower()
+    assert np.all(v[rp[-3:] - 1] == 2.5)  # the isolated node's diagonals, last in their rows
+
+
+def test_cmat3d_assembly_and_solve(hip_ctx_factory):
+    prob = meshgen.elastic_cube(4)
+    cm = _random_cmats(np.random.default_rng(8), 3)
+    group = np.random.default_rng(9).integers(0, 3, prob.n_el).astype(np.int32)
+    prob.mat_kind = capi.MAT_CMAT3D
+    prob.mat = cm.reshape(3, 36)
+    ctx = setup_problem(hip_ctx_factory(), prob, elem_group=group)
+    ctx.assemble()
+    eq = O.assemble_problem(prob, elem_group=group)
+    rp, ci = eq.pattern()
+    assert np.array_equal(ctx.pattern_lower()[1], ci)
+    assert _row_err(rp, ctx.values_lower(), eq.values()) <= 1e-12
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    ctx.solve(tol=1e-11, precision=capi.PREC_FP32_IR, maxit=20000)
+    st, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-13, stop_rule=1, maxit=20000)
+    assert np.linalg.norm(ctx.solution() - x_o) <= 1e-8 * np.linalg.norm(x_o)
+
+
+@pytest.mark.parametrize("kind", ["elastic", "heat"])
+def test_integration_order_3(hip_ctx_factory, kind):
+    """Order 3 (27 Gauss points), the reference's default for 3D elasticity
+    and heat elements (ElasticityElement3D.cpp:25,
+    HeatTransferElement3DTransient.cpp:22)."""
+    print("case order 3", kind, flush=True)
+    if kind == "elastic":
+        prob = meshgen.elastic_cube(4)
+        prob.order = 3
+        ctx = setup_problem(hip_ctx_factory(), prob)
+        ctx.assemble()
+        eq = O.assemble_problem(prob)
+        v_o = eq.values()
+    else:
+        prob = meshgen.diffusion_cube(5)
+        prob.order = 3
+        ctx = setup_problem(hip_ctx_factory(), prob)
+        u0 = prob.extra["u0"]
+        ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[1.0, prob.extra["dt"]], u_state=u0)
+        eq = O.assemble_heat(prob, 1.0, prob.extra["dt"], u0)
+        v_o = eq.values()
+    rp, ci = eq.pattern()
+    assert np.array_equal(ctx.pattern_lower()[1], ci)
+    assert _row_err(rp, ctx.values_lower(), v_o) <= 1e-12
+    if kind == "elastic":
+        ctx.set_load(prob.load)
+        ctx.build_rhs()
+        ctx.solve(tol=1e-11, precision=capi.PREC_FP32_IR, maxit=20000)
+        st, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-13, stop_rule=1, maxit=20000)
+        assert np.linalg.norm(ctx.solution() - x_o) <= 1e-8 * np.linalg.norm(x_o)
+

@@ -17,11 +17,14 @@ Checks:
     the server keeps serving
   - a server run in BHIP_PREC_FP64 or BHIP_PREC_FP64_BLOCKED solves each
     request's cases in turn with the same bits as the private solve
+  - a served request (FP32-inner server refined to 1e-11) against the
+    oracle's FP64 solve of each load case to 1e-13: relative error <= 1e-8
 """
 import numpy as np
 import pytest
 
 from betanoox_amd import capi, meshgen, setup_problem
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -180,3 +183,22 @@ def test_fp64_precisions_served(hip_ctx_factory, prec):
     for (X, st), Xr in zip(out, ref):
         assert st["status"] == capi.OK and st["rel_resid"] <= TOL and st["pack_size"] == 1
         assert np.array_equal(X, Xr)
+
+
+def test_served_request_against_oracle():
+    """Served displacements of an RVE request (served alone and beside three
+    other requests, FP32-inner refined to 1e-11) against the oracle: the
+    reference's assembly and FP64 PCG to 1e-13 for each load case."""
+    probs = [meshgen.rve_request(n, 300 + n) for n in (7, 5, 9, 6)]
+    with capi.Server(0, max_slots=4, tol=1e-11) as srv:
+        ids = [srv.submit(p) for p in probs]
+        out = [srv.wait(i) for i in ids]
+    for prob, (X, st) in zip(probs[:2], out[:2]):
+        assert st["status"] == capi.OK and st["rel_resid"] <= 1e-11
+        P = np.stack(prob.extra["cases"])
+        for k in range(P.shape[0]):
+            prob.prescribed = P[k]
+            eq = O.assemble_problem(prob, elem_group=prob.extra["group"])
+            status, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-13, maxit=20000, stop_rule=1)
+            assert status == 0
+            assert np.linalg.norm(X[k] - x_o) <= 1e-8 * np.linalg.norm(x_o), k

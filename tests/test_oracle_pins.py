@@ -623,3 +623,24 @@ def test_plastic_driver_all_plastic_path_closed_form():
     # the top face moved by the prescribed 1.2 %
     top = np.isclose(prob.xyz[:, 2], 1.0)
     assert np.allclose(u[3 * np.where(top)[0] + 2], 0.012, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [3, 4])
+def test_parcg_two_threads_matches_serial(n):
+    """ParCGMatrix::solve restated with two OpenMP threads: the same solution
+    as the serial sparse.cpp PCG and a dense solve, iterations within one
+    (only the grouping of partial sums differs); at n = 4 (375 DOF) the
+    L/2 split cuts a node."""
+    prob = meshgen.elastic_cube(n)
+    eq = O.assemble_problem(prob)
+    b = eq.rhs()
+    st1, x1, it1, _ = eq.pcg(b, eps=1e-12, stop_rule=0, maxit=10000)
+    st2, x2, it2, _ = eq.pcg_parcg(b, eps=1e-12, maxit=10000)
+    assert st1 == 0 and st2 == 0 and abs(it1 - it2) <= 1
+    A = eq.to_dense()
+    xd = np.linalg.solve(A, b)
+    assert np.linalg.norm(x2 - xd) <= 1e-9 * np.linalg.norm(xd)
+    assert np.linalg.norm(x2 - x1) <= 1e-9 * np.linalg.norm(x1)
+    # the two-thread product equals the serial one
+    y = eq.spmv(b)
+    assert np.allclose(A @ b, y, rtol=0, atol=1e-12 * np.abs(y).max())
