@@ -347,6 +347,18 @@ void bhip_runs_spmv64(bhip_ctx *ctx, const double *vals, const double *x, double
 int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb);
 const double *bhip_plastic_tangent(bhip_ctx *ctx);
 
+// ------------------------------------------------ global-memory pointers
+// A pointer a kernel reads from device memory (a descriptor) is generic to
+// the compiler, so its loads become flat loads, which count against both
+// the vector-memory and the LDS counters: the compiler then waits for every
+// outstanding load at once instead of only the ones a use needs.  Casting
+// to the global address space turns them into global loads.
+#define BHIP_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ BHIP_GLOBAL T *to_global(T *p) {
+    return (BHIP_GLOBAL T *)p;
+}
+
 // ------------------------------------------------ hex8 element helpers
 // (plastic.hip, forces.hip)
 // All 8 global gradients at ip (isoElement.cpp:324-372).

@@ -88,7 +88,7 @@ struct PackHdr {
 };
 
 template <int NT, class T>
-__device__ __forceinline__ T ld_stream(const T *p) {
+__device__ __forceinline__ T ld_stream(const BHIP_GLOBAL T *p) {
     if constexpr (NT) return __builtin_nontemporal_load(p);
     else return *p;
 }
@@ -96,7 +96,7 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // Fixed-order sum of nb records of NV doubles by one block, result in every
 // thread (thread t adds records t, t + 256, ...; eight loads in flight).
 template <int NV>
-__device__ __forceinline__ void sum_records_all(const double *rec, int nb, double (&tot)[NV]) {
+__device__ __forceinline__ void sum_records_all(const BHIP_GLOBAL double *rec, int nb, double (&tot)[NV]) {
     __shared__ double sm[16 * NV];
     __shared__ double res[NV];
     constexpr int U = 8;
@@ -135,9 +135,10 @@ __device__ __forceinline__ void item_range(long long n, int lb, int nb, long lon
 // Row sums of one lane over block columns k = k0, k0 + kstep, ... < ke of its
 // node's row for all R cases (CT: wide node indices or narrow offsets).
 template <int R, int NT, int UNR, class CT>
-__device__ __forceinline__ void pk_rows(const CT *__restrict__ scol, const float *__restrict__ sval, long long nsell,
-                                        const float *__restrict__ x, long long ld, long long node, int k0, int ke,
-                                        int kstep, float (&a0)[R], float (&a1)[R], float (&a2)[R]) {
+__device__ __forceinline__ void pk_rows(const BHIP_GLOBAL CT *__restrict__ scol, const BHIP_GLOBAL float *__restrict__ sval,
+                                        long long nsell, const BHIP_GLOBAL float *__restrict__ x, long long ld,
+                                        long long node, int k0, int ke, int kstep, float (&a0)[R], float (&a1)[R],
+                                        float (&a2)[R]) {
 #pragma unroll UNR
     for (int k = k0; k < ke; k += kstep) {
         long long m;
@@ -148,7 +149,7 @@ __device__ __forceinline__ void pk_rows(const CT *__restrict__ scol, const float
         const float v4 = ld_stream<NT>(sval + 4 * nsell + k), v5 = ld_stream<NT>(sval + 5 * nsell + k);
         const float v6 = ld_stream<NT>(sval + 6 * nsell + k), v7 = ld_stream<NT>(sval + 7 * nsell + k);
         const float v8 = ld_stream<NT>(sval + 8 * nsell + k);
-        const float *xm = x + 3 * m;
+        const BHIP_GLOBAL float *xm = x + 3 * m;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const float x0 = xm[r * ld], x1 = xm[r * ld + 1], x2 = xm[r * ld + 2];
@@ -172,8 +173,8 @@ __global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restri
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n_items = hdr->n_sitems;
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const PackSysDesc &D = hdr->sys[hdr->sitem_sys[item]];
-        const PackState *st = D.st;
+        const BHIP_GLOBAL PackSysDesc &D = to_global(hdr->sys)[to_global(hdr->sitem_sys)[item]];
+        const BHIP_GLOBAL PackState *st = to_global(D.st);
         bool live = false;
 #pragma unroll
         for (int r = 0; r < R; r++) live |= st->done[r] == 0;
@@ -191,12 +192,14 @@ __global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restri
             a0[r] = a1[r] = a2[r] = 0.f;
         }
         if (slice < D.n_slices && node < D.n_nodes) {
-            const int kb = D.sptr[slice], ke = D.sptr[slice + 1];
+            const int kb = to_global(D.sptr)[slice], ke = to_global(D.sptr)[slice + 1];
             const int k0 = kb + lane + (split ? 64 * wave : 0), kstep = split ? 256 : 64;
             if (D.narrow)
-                pk_rows<R, NT, UNR, short>(D.scol16, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
+                pk_rows<R, NT, UNR, short>(to_global(D.scol16), to_global(D.sval), D.nsell, to_global(D.p), ld, node,
+                                           k0, ke, kstep, a0, a1, a2);
             else
-                pk_rows<R, NT, UNR, int>(D.scol, D.sval, D.nsell, D.p, ld, node, k0, ke, kstep, a0, a1, a2);
+                pk_rows<R, NT, UNR, int>(to_global(D.scol), to_global(D.sval), D.nsell, to_global(D.p), ld, node, k0,
+                                         ke, kstep, a0, a1, a2);
         }
         if (split) {
             if (wave > 0)
@@ -217,8 +220,8 @@ __global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restri
                     }
         }
         if (slice < D.n_slices && node < D.n_nodes && (!split || wave == 0)) {
-            const float *x = D.p;
-            float *y = D.q;
+            const BHIP_GLOBAL float *x = to_global(D.p);
+            BHIP_GLOBAL float *y = to_global(D.q);
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const long long j = r * ld + 3 * node;
@@ -231,7 +234,7 @@ __global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restri
         block_sum<R>(dots, sm);
         if (threadIdx.x == 0)
 #pragma unroll
-            for (int r = 0; r < R; r++) hdr->rec[D.rec_s + (long long)r * D.n_sitems + local] = dots[r];
+            for (int r = 0; r < R; r++) to_global(hdr->rec)[D.rec_s + (long long)r * D.n_sitems + local] = dots[r];
         __syncthreads();  // LDS is reused by the next item
     }
 }
@@ -244,22 +247,22 @@ __global__ void __launch_bounds__(kThreads) k_pk_update(const PackHdr *__restric
     const int nv = hdr->n_vitems;
     for (int item = blockIdx.x; item < nv * R; item += gridDim.x) {
         const int r = item / nv, vi = item - r * nv;
-        const PackSysDesc &D = hdr->sys[hdr->vitem_sys[vi]];
-        const PackState *st = D.st;
+        const BHIP_GLOBAL PackSysDesc &D = to_global(hdr->sys)[to_global(hdr->vitem_sys)[vi]];
+        const BHIP_GLOBAL PackState *st = to_global(D.st);
         if (st->done[r]) continue;
         const double rz = st->rzb[par][r];
         double pap[1];
-        sum_records_all<1>(hdr->rec + D.rec_s + (long long)r * D.n_sitems, D.n_sitems, pap);
+        sum_records_all<1>(to_global(hdr->rec) + D.rec_s + (long long)r * D.n_sitems, D.n_sitems, pap);
         const float alpha = (float)(rz / pap[0]);
         const int lv = vi - D.vitem0;
         long long i0, i1;
         item_range(D.n, lv, D.n_vitems, i0, i1);
         const long long o = (long long)r * D.n;
-        float *__restrict__ x = D.d;
-        float *__restrict__ rv = D.r;
-        const float *__restrict__ p = D.p;
-        const float *__restrict__ q = D.q;
-        const float *__restrict__ dinv = D.dinv;
+        BHIP_GLOBAL float *__restrict__ x = to_global(D.d);
+        BHIP_GLOBAL float *__restrict__ rv = to_global(D.r);
+        const BHIP_GLOBAL float *__restrict__ p = to_global(D.p);
+        const BHIP_GLOBAL float *__restrict__ q = to_global(D.q);
+        const BHIP_GLOBAL float *__restrict__ dinv = to_global(D.dinv);
         double acc[2] = {0, 0};
         // kBatch elements per trip, operands loaded before any update (same
         // elements and order of sums as a plain loop)
@@ -291,7 +294,7 @@ __global__ void __launch_bounds__(kThreads) k_pk_update(const PackHdr *__restric
         }
         block_sum<2>(acc, sm);
         if (threadIdx.x == 0) {
-            double *rec = hdr->rec + D.rec_v + ((long long)r * D.n_vitems + lv) * 2;
+            BHIP_GLOBAL double *rec = to_global(hdr->rec) + D.rec_v + ((long long)r * D.n_vitems + lv) * 2;
             rec[0] = acc[0];
             rec[1] = acc[1];
         }
@@ -308,12 +311,12 @@ __global__ void __launch_bounds__(kThreads) k_pk_pupdate(const PackHdr *__restri
     const int nv = hdr->n_vitems;
     for (int item = blockIdx.x; item < nv * R; item += gridDim.x) {
         const int r = item / nv, vi = item - r * nv;
-        const PackSysDesc &D = hdr->sys[hdr->vitem_sys[vi]];
-        PackState *st = D.st;
+        const BHIP_GLOBAL PackSysDesc &D = to_global(hdr->sys)[to_global(hdr->vitem_sys)[vi]];
+        BHIP_GLOBAL PackState *st = to_global(D.st);
         if (st->done[r]) continue;
         const double rz = st->rzb[par][r];
         double t[2];
-        sum_records_all<2>(hdr->rec + D.rec_v + (long long)r * D.n_vitems * 2, D.n_vitems, t);
+        sum_records_all<2>(to_global(hdr->rec) + D.rec_v + (long long)r * D.n_vitems * 2, D.n_vitems, t);
         const int lv = vi - D.vitem0;
         if (lv == 0 && threadIdx.x == 0) {
             const int it = st->iter[r] + 1;
@@ -328,9 +331,9 @@ __global__ void __launch_bounds__(kThreads) k_pk_pupdate(const PackHdr *__restri
         long long i0, i1;
         item_range(D.n, lv, D.n_vitems, i0, i1);
         const long long o = (long long)r * D.n;
-        float *__restrict__ p = D.p;
-        const float *__restrict__ rv = D.r;
-        const float *__restrict__ dinv = D.dinv;
+        BHIP_GLOBAL float *__restrict__ p = to_global(D.p);
+        const BHIP_GLOBAL float *__restrict__ rv = to_global(D.r);
+        const BHIP_GLOBAL float *__restrict__ dinv = to_global(D.dinv);
         const long long stride = blockDim.x;
         for (long long i = i0 + threadIdx.x; i < i1; i += kBatch * stride) {
             float pv[kBatch], rr[kBatch], dv[kBatch];
@@ -359,7 +362,7 @@ __global__ void __launch_bounds__(kThreads) k_pk_init(const PackHdr *__restrict_
     const int nv = hdr->n_vitems;
     for (int item = blockIdx.x; item < nv * R; item += gridDim.x) {
         const int r = item / nv, vi = item - r * nv;
-        const PackSysDesc &D = hdr->sys[hdr->vitem_sys[vi]];
+        const BHIP_GLOBAL PackSysDesc &D = to_global(hdr->sys)[to_global(hdr->vitem_sys)[vi]];
         const int lv = vi - D.vitem0;
         long long i0, i1;
         item_range(D.n, lv, D.n_vitems, i0, i1);
@@ -367,15 +370,15 @@ __global__ void __launch_bounds__(kThreads) k_pk_init(const PackHdr *__restrict_
         double acc[2] = {0, 0};
 #pragma unroll 4
         for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            const float ri = D.r[o + i];
-            const float zi = ri * D.dinv[i];
-            D.p[o + i] = zi;
+            const float ri = to_global(D.r)[o + i];
+            const float zi = ri * to_global(D.dinv)[i];
+            to_global(D.p)[o + i] = zi;
             acc[0] += (double)ri * zi;
             acc[1] += (double)ri * ri;
         }
         block_sum<2>(acc, sm);
         if (threadIdx.x == 0) {
-            double *rec = hdr->rec + D.rec_v + ((long long)r * D.n_vitems + lv) * 2;
+            BHIP_GLOBAL double *rec = to_global(hdr->rec) + D.rec_v + ((long long)r * D.n_vitems + lv) * 2;
             rec[0] = acc[0];
             rec[1] = acc[1];
         }
@@ -389,11 +392,11 @@ template <int R>
 __global__ void __launch_bounds__(kThreads) k_pk_init_state(const PackHdr *__restrict__ hdr) {
     for (int item = blockIdx.x; item < hdr->n_sys * R; item += gridDim.x) {
         const int s = item / R, r = item - s * R;
-        const PackSysDesc &D = hdr->sys[s];
+        const BHIP_GLOBAL PackSysDesc &D = to_global(hdr->sys)[s];
         double t[2];
-        sum_records_all<2>(hdr->rec + D.rec_v + (long long)r * D.n_vitems * 2, D.n_vitems, t);
+        sum_records_all<2>(to_global(hdr->rec) + D.rec_v + (long long)r * D.n_vitems * 2, D.n_vitems, t);
         if (threadIdx.x == 0) {
-            PackState *st = D.st;
+            BHIP_GLOBAL PackState *st = to_global(D.st);
             st->rzb[0][r] = t[0];
             st->rzb[1][r] = 0;
             st->rr[r] = t[1];
