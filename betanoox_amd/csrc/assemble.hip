@@ -439,6 +439,14 @@ struct AsmOut {
     long long n_pairs;
     int mode;  // 0: val = K ; 1 (heat): val = (a1 K + M)/dt, val_aux = K
     double a1, dt;
+    // mode 0 with a current blocked SELL layout: the pair's block is also
+    // written into the solver's SELL value arrays (FP32, and FP64 when
+    // sval64 is set), restrained entries zero and restrained diagonals one,
+    // the values the CSR-to-SELL conversion would write
+    float *sval32;
+    double *sval64;
+    const int *sptr, *nbr_ptr;
+    long long nsell;
 };
 
 template <class P>
@@ -453,6 +461,21 @@ __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const in
     pair_block<P>(A, n2e_ptr, n2e, n, lo, hi, K, K2);
     const bool nlo = (n == lo);
     unsigned rn = O.rmask[n], rm = O.rmask[m];
+    if (O.mode == 0 && O.sval32) {
+        const long long dst = (long long)O.sptr[n / 64] + 64LL * (p - O.nbr_ptr[n]) + (n % 64);
+#pragma unroll
+        for (int c = 0; c < P::ND; c++)
+#pragma unroll
+            for (int d = 0; d < P::ND; d++) {
+                const bool rc = (rn >> c) & 1u, rd = (rm >> d) & 1u;
+                double v = 0;
+                if (!rc && !rd) v = oriented<P::ND>(K, nlo, c, d);
+                else if (rc && n == m && c == d) v = 1;
+                const long long o = (long long)(c * P::ND + d) * O.nsell + dst;
+                O.sval32[o] = (float)v;
+                if (O.sval64) O.sval64[o] = v;
+            }
+    }
 #pragma unroll
     for (int c = 0; c < P::ND; c++) {
         if ((rn >> c) & 1u) continue;
@@ -627,6 +650,17 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     O.mode = 0;
     O.a1 = 0;
     O.dt = 1;
+    // assembly straight into the solver's blocked layout when it is current
+    O.sval32 = nullptr;
+    O.sval64 = nullptr;
+    O.sptr = ctx->sell_ptr.p;
+    O.nbr_ptr = ctx->nbr_ptr.p;
+    O.nsell = ctx->n_sell;
+    const bool to_sell = bhip_sell_values_direct(ctx) && (what == BHIP_ASM_STIFFNESS || what == BHIP_ASM_PLASTIC_TANGENT);
+    if (to_sell) {
+        O.sval32 = ctx->bval32.p;
+        if (ctx->bval64_layout_ok) O.sval64 = ctx->bval64.p;
+    }
 
     const int kind = ctx->mat_kind;
     int st = BHIP_OK;
@@ -678,7 +712,8 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     BHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->assembled = true;
     ctx->asm_kind = what;
-    ctx->f32_ready = ctx->f64b_ready = false;
+    ctx->f32_ready = O.sval32 != nullptr;
+    ctx->f64b_ready = O.sval64 != nullptr;
     ctx->lift_ready = false;
     if (hflags[0] & 1) return bhip_fail(ctx, BHIP_ERR_NEG_DETJ, "negative Jacobian determinant (inverted element)");
     if (hflags[0] & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "bad diagonal in an element stiffness");

@@ -151,6 +151,7 @@ struct bhip_ctx {
     bool bval64_layout_ok = false; // bval64 sized and zero-padded for the current layout
     dvec<double> bval64;           // SELL block values, SoA: sell_vals(ctx) x n_sell
     bool sell_layout_ok = false;  // SELL pointers/columns built for the current pattern
+    bool sell_cols_ok = false;    // SELL block columns written (by the first CSR-to-SELL pass)
     int sell_shape = 0;      // tile shape id of the FP32 SELL SpMV
     bool sell_shape_set = false;  // chosen by bhip_tune_spmv / bhip_set_spmv_shape (else the
                                   // tuned default table picks it when the SELL layout is built)
@@ -177,6 +178,9 @@ struct bhip_ctx {
     dvec<double> lift_val;      // ndof x ndof per entry, row comp of the owner node first
                                 // (heat: K and (a1 K + M)/dt per entry)
     double heat_a1 = 0, heat_dt = 1;  // coefficients of the last heat-transient assembly
+    // bhip_heat_run buffers (grow-only, kept between runs): u_s, the FP64
+    // column-run copy of K and K u_s
+    dvec<double> heat_us, heat_kruns, heat_ku;
     // ---- vectors (FP64)
     dvec<double> load, pseudo, rhs, x, x0, r, p, q, dinv, tmp;
     // ---- vectors (FP32 inner)
@@ -295,7 +299,11 @@ int bhip_tune_spmv_impl(bhip_ctx *ctx, int reps, bhip_spmv_shape *table, int max
 int bhip_spmv_shape_info(int id, bhip_spmv_shape *info);
 int bhip_set_spmv_shape_impl(bhip_ctx *ctx, int id);
 void bhip_invalidate_graphs(bhip_ctx *ctx);
-int bhip_scratch_setup(bhip_ctx *ctx);  // once per device: address of the allocation-free scratch
+int bhip_scratch_setup(bhip_ctx *ctx);
+// The assembly may write the solver's SELL block values itself: the layout is
+// the blocked form (not column runs) of the current pattern and its columns
+// have been written.
+bool bhip_sell_values_direct(const bhip_ctx *ctx);  // once per device: address of the allocation-free scratch
 void bhip_plastic_release(bhip_ctx *ctx);
 void bhip_multi_invalidate(bhip_ctx *ctx);
 void bhip_multi_release(bhip_ctx *ctx);

@@ -132,29 +132,39 @@ extern "C" int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode,
     BHIP_PROPAGATE(bhip_lift_build(ctx));
     bhip_heat_stats H;
     memset(&H, 0, sizeof(H));
-    scoped_dvec<double> us;
+    // the run's device time includes its preparation (layout, run copy of K)
+    BHIP_TRY(ctx, hipEventRecord(ctx->drv0, s));
+    dvec<double> &us = ctx->heat_us;
     BHIP_TRY(ctx, us.resize((size_t)n));
     BHIP_TRY(ctx, hipMemcpyAsync(us.p, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
     // The refined solver's layout decides the form of -K u_s: on a column-run
-    // layout K is copied once per call into FP64 runs, which stream at about
-    // twice the rate of the CSR form with 0.8 of its bytes; otherwise the CSR
-    // of K is read every step.
-    scoped_dvec<double> kruns, ku;
+    // layout K is copied once per call into FP64 runs (grow-only context
+    // buffers), which stream at about twice the rate of the CSR form with 0.8
+    // of its bytes; otherwise, or when that copy does not fit in device
+    // memory, the CSR of K is read every step.
+    const double *kruns = nullptr;
+    double *ku = nullptr;
     if (opts->precision == BHIP_PREC_FP32_IR) {
         BHIP_PROPAGATE(bhip_prepare_refined(ctx));
         if (ctx->use_bsr && ctx->sell_runs) {
-            BHIP_TRY(ctx, kruns.resize((size_t)3 * (ctx->n_sell > 0 ? ctx->n_sell : 1)));
-            BHIP_TRY(ctx, ku.resize((size_t)n));
-            bhip_runs_from_csr64(ctx, ctx->val_aux.p, kruns.p);
-            BHIP_TRY(ctx, hipGetLastError());
+            const size_t nr = (size_t)3 * (ctx->n_sell > 0 ? ctx->n_sell : 1);
+            if (ctx->heat_kruns.resize(nr) == hipSuccess && ctx->heat_ku.resize((size_t)n) == hipSuccess) {
+                bhip_runs_from_csr64(ctx, ctx->val_aux.p, ctx->heat_kruns.p);
+                BHIP_TRY(ctx, hipGetLastError());
+                kruns = ctx->heat_kruns.p;
+                ku = ctx->heat_ku.p;
+            } else {
+                (void)hipGetLastError();  // clear the failed allocation; the CSR form needs no buffer
+                ctx->heat_kruns.release();
+                ctx->heat_ku.release();
+            }
         }
     }
     // Fbar = -(a1 + a2) K u_s / dt with a1 + a2 = dt for every alpha
     const double fcoef = -1.0;
-    BHIP_TRY(ctx, hipEventRecord(ctx->drv0, s));
     int status = BHIP_OK;
     for (int step = 0; step < n_steps; step++) {
-        launch_heat_rhs(ctx, us.p, fcoef, kruns.p, ku.p);
+        launch_heat_rhs(ctx, us.p, fcoef, kruns, ku);
         BHIP_TRY(ctx, hipGetLastError());
         bhip_solve_opts o = *opts;
         o.use_x0 = 0;

@@ -1045,6 +1045,7 @@ int build_sell(bhip_ctx *ctx) {
     // (the B = 1 rows of the table are measured on column runs, the form hex8 diffusion takes)
     if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
     ctx->sell_layout_ok = true;
+    ctx->sell_cols_ok = false;
     ctx->bval64_layout_ok = false;
     ctx->f32_ready = ctx->f64b_ready = false;
     return BHIP_OK;
@@ -1077,6 +1078,7 @@ void enqueue_to_sell(bhip_ctx *ctx, VT *sval, double *sval64 = nullptr) {
         return;
     }
     const int grid = grid_for(ctx->n_pairs, B);
+    ctx->sell_cols_ok = true;
 #define BHIP_TO_SELL(BB, CT, COL)                                                                                   \
     k_csr_to_sell<BB, CT, VT><<<grid, B, 0, ctx->stream>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, \
                                                            ctx->nbr_off.p, ctx->rowptr.p, ctx->node_rmask.p,          \
@@ -1344,6 +1346,11 @@ void enqueue_init(bhip_ctx *ctx, const T *x, const T *r, T *p, const T *dinv) {
 }
 
 }  // namespace
+
+bool bhip_sell_values_direct(const bhip_ctx *ctx) {
+    return ctx->sell_layout_ok && ctx->sell_cols_ok && !ctx->sell_runs && ctx->use_bsr && ctx->sell_b == ctx->max_ndof &&
+           ctx->sell_ptr.n == (size_t)((ctx->n_nodes + 63) / 64 + 1);
+}
 
 void bhip_invalidate_graphs(bhip_ctx *ctx) {
     if (ctx->graph64) (void)hipGraphExecDestroy(ctx->graph64);

@@ -20,11 +20,15 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   (k_plastic_update), the tangent assembly (k_elem_blocks / k_asm_pairs of
   ElasHex8<J2>), the device residual (k_plastic_fe, k_plastic_residual) and an
   FP32-inner refined solve to 1e-8
+--workload plastic-kernels (J2 bar, --n 64): the per-Newton-iteration kernels
+  without the solve: reps x (return map with the algorithmic tangent from a
+  plastic trial increment, tangent assembly, device residual)
 
 The SELL SpMV runs in the shape the library's tuned table gives the matrix,
 or in --shape ID (see bhip_tune_spmv); the shape is printed.
 
-usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat|plastic] [--n N] [--reps 10]
+usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat|plastic|plastic-kernels] [--n N]
+                                              [--reps 10]
                                               [--no-solve] [--shape ID] [--increments K]
 """
 import argparse
@@ -38,7 +42,8 @@ from . import capi, meshgen, setup_problem
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("elastic", "cases", "heat", "plastic"), default="elastic")
+    ap.add_argument("--workload", choices=("elastic", "cases", "heat", "plastic", "plastic-kernels"),
+                    default="elastic")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--no-solve", action="store_true")
@@ -92,6 +97,24 @@ def main(argv=None):
                 hist.append(st)
                 print("increment %d: %s" % (inc, json.dumps(st)), file=sys.stderr, flush=True)
             out["plastic_increments"] = hist
+        elif a.workload == "plastic-kernels":
+            n = a.n or 64
+            prob = meshgen.plastic_bar(n)
+            setup_problem(ctx, prob)
+            ctx.plastic_init()
+            ctx.plastic_set_tangent(capi.TANGENT_ALGORITHMIC)
+            ctx.set_load(prob.load)
+            # a uniform axial strain of 0.3 % with lateral contraction: every
+            # Gauss point takes the plastic branch
+            du = np.zeros(prob.n_dof)
+            du[2::3] = 0.003 * prob.xyz[:, 2]
+            du[0::3] = -0.0015 * prob.xyz[:, 0]
+            du[1::3] = -0.0015 * prob.xyz[:, 1]
+            for _ in range(a.reps):
+                ctx.plastic_update(du)
+                ctx.assemble(capi.ASM_PLASTIC_TANGENT)
+                _, mx = ctx.plastic_residual()
+            out["max_residual"] = mx
         else:
             n = a.n or 128
             prob = meshgen.diffusion_cube(n)

@@ -32,13 +32,26 @@ def _fresh(prob, prec):
         return _solve(c, prec)
 
 
+def _pair(kind):
+    if kind == "hex":
+        a, b = meshgen.elastic_cube(6), meshgen.elastic_cube(6)
+        b.mat = b.mat.copy()
+        b.mat[:, 0] *= 3.0  # other stiffness, same mesh and buffers
+    else:
+        a, b = meshgen.plane_stress_quad(16), meshgen.plane_stress_quad(16)
+        b.mat = b.mat.copy()
+        b.mat[:, [0, 1, 3]] *= 3.0
+    return a, b
+
+
+@pytest.mark.parametrize("kind", ["hex", "quad"])
 @pytest.mark.parametrize("prec", PRECS)
-def test_reassembly_reuses_graphs_bitwise(prec):
-    a = meshgen.elastic_cube(6)
-    b = meshgen.elastic_cube(6)
-    b.mat = b.mat.copy()
-    b.mat[:, 0] *= 3.0  # other stiffness, same mesh and buffers
-    print("reassembly", prec, flush=True)
+def test_reassembly_reuses_graphs_bitwise(prec, kind):
+    """The second and third assemblies write the SELL block values of the
+    current layout themselves (no CSR-to-SELL pass): the same bits as a
+    fresh context, which converts."""
+    a, b = _pair(kind)
+    print("reassembly", prec, kind, flush=True)
     with capi.Context(0) as ctx:
         setup_problem(ctx, a)
         ctx.assemble()
