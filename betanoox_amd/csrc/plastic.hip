@@ -50,19 +50,19 @@ constexpr int kIsv = 16;
 
 struct J2 {
     double E, nu, s0, H;
-    double C[36];
+    double lam, mu;  // isotropic C: lam + 2 mu on the normal diagonal, lam off it, mu for shear
     __device__ void init(const double *p) {
         E = p[0];
         nu = p[1];
         s0 = p[2];
         H = p[3];
-        double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
-        for (int i = 0; i < 36; i++) C[i] = 0;
-        for (int i = 0; i < 3; i++) {
-            for (int j = 0; j < 3; j++) C[i * 6 + j] = lam;
-            C[i * 6 + i] = lam + 2 * mu;
-            C[(i + 3) * 6 + (i + 3)] = mu;
-        }
+        lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+        mu = E / (2 * (1 + nu));
+    }
+    // entry (i, j) of the 6x6 C (Voigt xx yy zz xy yz xz, engineering shear)
+    __device__ __forceinline__ double c(int i, int j) const {
+        if (i < 3 && j < 3) return i == j ? lam + 2 * mu : lam;
+        return i == j ? mu : 0.0;
     }
 };
 
@@ -104,16 +104,27 @@ __device__ __forceinline__ double hard_mod(const J2 &m, const double *s0, const 
     return 4.0 / 9.0 * mm2 * m.H;
 }
 
-__device__ __forceinline__ void cmul(const double *C, const double *v, double *o) {
-    for (int i = 0; i < 6; i++) {
-        double s = 0;
-        for (int j = 0; j < 6; j++) s += C[i * 6 + j] * v[j];
-        o[i] = s;
-    }
+// o = C v, each row summed over its nonzero entries in column order (the
+// zero entries of the isotropic C add nothing)
+__device__ __forceinline__ void cmul(const J2 &m, const double *v, double *o) {
+    const double d = m.lam + 2 * m.mu;
+    o[0] = d * v[0];
+    o[0] += m.lam * v[1];
+    o[0] += m.lam * v[2];
+    o[1] = m.lam * v[0];
+    o[1] += d * v[1];
+    o[1] += m.lam * v[2];
+    o[2] = m.lam * v[0];
+    o[2] += m.lam * v[1];
+    o[2] += d * v[2];
+    o[3] = m.mu * v[3];
+    o[4] = m.mu * v[4];
+    o[5] = m.mu * v[5];
 }
 
 __device__ __forceinline__ double dot6(const double *a, const double *b) {
     double s = 0;
+#pragma unroll
     for (int i = 0; i < 6; i++) s += a[i] * b[i];
     return s;
 }
@@ -131,7 +142,7 @@ __device__ __forceinline__ void flow_lin(const double *s, double *nf) {
 
 // "V": strain increment + return map (getStressIncrement, plastic.cpp:368-465).
 // alg: write the symmetric part of the return map's derivative as the tangent.
-__global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ gtab, const int *conn, const double *xyz,
+__global__ void __launch_bounds__(128) k_plastic_update(int n_el, int nip, const double *__restrict__ gtab, const int *conn, const double *xyz,
                                  const double *mat, const int *elem_group, int n_groups, const double *du,
                                  double *isv, double *tan, int alg) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,21 +174,22 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
     double s0[6], ds[6], dse[6], nf[6], sp[6];
     for (int i = 0; i < 6; i++) s0[i] = rec[i];
     const double ep = rec[6];
-    cmul(m.C, de, dse);  // elastic predictor
+    cmul(m, de, dse);  // elastic predictor
     for (int i = 0; i < 6; i++) ds[i] = dse[i];
     double dep = 0.0;
     int iter = 0, plastic = 0;
     double inv_denom = 0.0;
     // D = d(ds)/d(de), column-major by strain component (alg only)
     double D[36];
-    for (int i = 0; i < 36; i++) D[i] = m.C[(i % 6) * 6 + i / 6];
+#pragma unroll
+    for (int i = 0; i < 36; i++) D[i] = m.c(i % 6, i / 6);
     if (yield_fn(m, s0, ds, ep + dep) > 0.0) {
         plastic = 1;
         double f;
         do {
             double hard = hard_mod(m, s0, ds);
             dyield(s0, ds, nf);
-            cmul(m.C, nf, sp);
+            cmul(m, nf, sp);
             const double den = hard + dot6(nf, sp);
             double dl = dot6(nf, dse) / den;
             if (alg) {
@@ -185,17 +197,20 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
                 //   d nf = P D_j, d num = dse.(P D_j) + nf.C_j,
                 //   d den = (4/3) H nf.D_j + 2 sp.(P D_j),
                 //   D_j' = C_j - sp d dl - dl C P D_j
+#pragma unroll
                 for (int j = 0; j < 6; j++) {
                     double pc[6], cpc[6];
                     flow_lin(D + 6 * j, pc);
-                    cmul(m.C, pc, cpc);
+                    cmul(m, pc, cpc);
                     double dnum = 0, dden = 0;
+#pragma unroll
                     for (int i = 0; i < 6; i++) {
-                        dnum += dse[i] * pc[i] + nf[i] * m.C[i * 6 + j];
+                        dnum += dse[i] * pc[i] + nf[i] * m.c(i, j);
                         dden += 4.0 / 3.0 * m.H * nf[i] * D[6 * j + i] + 2.0 * sp[i] * pc[i];
                     }
                     const double ddl = (dnum - dl * dden) / den;
-                    for (int i = 0; i < 6; i++) D[6 * j + i] = m.C[i * 6 + j] - sp[i] * ddl - dl * cpc[i];
+#pragma unroll
+                    for (int i = 0; i < 6; i++) D[6 * j + i] = m.c(i, j) - sp[i] * ddl - dl * cpc[i];
                 }
             }
             for (int i = 0; i < 6; i++) ds[i] = dse[i] - dl * sp[i];
@@ -204,7 +219,7 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
             iter++;
         } while (f > 1e-6 && iter < 5000);
         dyield(s0, ds, nf);
-        cmul(m.C, nf, sp);
+        cmul(m, nf, sp);
         inv_denom = 1.0 / (hard_mod(m, s0, ds) + dot6(nf, sp));
     }
     for (int i = 0; i < 6; i++) rec[7 + i] = ds[i];
@@ -213,9 +228,11 @@ __global__ void k_plastic_update(int n_el, int nip, const double *__restrict__ g
     rec[15] = iter;
     double *tr = tan + t * kTanVals;
     int k = 0;
+#pragma unroll
     for (int i = 0; i < 6; i++)
+#pragma unroll
         for (int j = i; j < 6; j++, k++) {
-            double v = m.C[i * 6 + j];
+            double v = m.c(i, j);
             if (plastic) v = alg ? 0.5 * (D[6 * j + i] + D[6 * i + j]) : v - sp[i] * sp[j] * inv_denom;
             tr[k] = v;
         }
