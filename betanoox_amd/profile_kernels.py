@@ -9,6 +9,7 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
   reps x FP64 SELL-64 3x3-block SpMV      (k_sell_spmv<3, 1, ..., double>, BHIP_PREC_FP64_BLOCKED)
   one FP32-inner + FP64-refined solve     (unless --no-solve)
+  one reassembly into the solver's layout (unless --no-solve)
 --workload cases (RVE request, --n 64, six load cases):
   one six-case solve in the packed solver (k_pk_spmv<6, ...>, k_pk_update<6>, k_pk_pupdate<6>)
 --workload heat (diffusion, --n 128, 1 DOF per node):
@@ -72,6 +73,9 @@ def main(argv=None):
             if not a.no_solve:
                 s = ctx.solve(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
                 out["solve_fp32_ir"] = {k: s[k] for k in ("iters", "outer_iters", "rel_resid", "ms_total")}
+                # a reassembly on the current blocked layout: the gather stage
+                # writes the FP32 and FP64 SELL values itself (no conversion)
+                ctx.assemble()
         elif a.workload == "cases":
             n = a.n or 64
             prob = meshgen.rve_request(n, 7)
