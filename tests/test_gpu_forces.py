@@ -214,3 +214,26 @@ def test_integration_order_3(hip_ctx_factory, kind):
         st, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-13, stop_rule=1, maxit=20000)
         assert np.linalg.norm(ctx.solution() - x_o) <= 1e-8 * np.linalg.norm(x_o)
 
+
+
+def test_quad4_thickness_factor(hip_ctx_factory):
+    """Plane stress with a thickness factor other than 1
+    (getIntegrationFactor of the 2D element): K and the solution against the
+    oracle's assembly with the same factor."""
+    prob = meshgen.plane_stress_quad(12)
+    prob.factor = 0.37
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.assemble()
+    eq = O.assemble_problem(prob)
+    rp, ci = eq.pattern()
+    assert _row_err(rp, ctx.values_lower(), eq.values()) <= 1e-12
+    ctx.set_load(prob.load)
+    ctx.build_rhs()
+    ctx.solve(tol=1e-12, precision=capi.PREC_FP64, maxit=20000)
+    st, x_o, _, _ = eq.pcg(eq.rhs(), eps=1e-13, stop_rule=1, maxit=20000)
+    assert np.linalg.norm(ctx.solution() - x_o) <= 1e-8 * np.linalg.norm(x_o)
+    # the displacement scales with 1/thickness
+    p1 = meshgen.plane_stress_quad(12)
+    eq1 = O.assemble_problem(p1)
+    st, x1, _, _ = eq1.pcg(eq1.rhs(), eps=1e-13, stop_rule=1, maxit=20000)
+    assert np.linalg.norm(x_o * 0.37 - x1) <= 1e-9 * np.linalg.norm(x1)
