@@ -21,6 +21,9 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   (k_plastic_update), the tangent assembly (k_elem_blocks / k_asm_pairs of
   ElasHex8<J2>), the device residual (k_plastic_fe, k_plastic_residual) and an
   FP32-inner refined solve to 1e-8
+--workload served (RVE requests): --reps requests of the seeded probe draw
+  (n = 16..96, six load cases each) queued at once on a 4-slot server
+  (pack 8), FP32-inner refined to 1e-8
 --workload plastic-kernels (J2 bar, --n 64): the per-Newton-iteration kernels
   without the solve: reps x (return map with the algorithmic tangent from a
   plastic trial increment, tangent assembly, device residual)
@@ -28,7 +31,8 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
 The SELL SpMV runs in the shape the library's tuned table gives the matrix,
 or in --shape ID (see bhip_tune_spmv); the shape is printed.
 
-usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat|plastic|plastic-kernels] [--n N]
+usage: python -m betanoox_amd.profile_kernels [--workload elastic|cases|heat|plastic|plastic-kernels|served]
+                                              [--n N]
                                               [--reps 10]
                                               [--no-solve] [--shape ID] [--increments K]
 """
@@ -43,7 +47,7 @@ from . import capi, meshgen, setup_problem
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("elastic", "cases", "heat", "plastic", "plastic-kernels"),
+    ap.add_argument("--workload", choices=("elastic", "cases", "heat", "plastic", "plastic-kernels", "served"),
                     default="elastic")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--reps", type=int, default=10)
@@ -52,6 +56,18 @@ def main(argv=None):
     ap.add_argument("--increments", type=int, default=6, help="plastic: increments of the 20-increment path run")
     a = ap.parse_args(argv)
     out = {"workload": a.workload}
+    if a.workload == "served":
+        rng = np.random.default_rng(meshgen.SEED_BASE + 5 + 1000)
+        ns = np.rint(np.exp(rng.uniform(np.log(16), np.log(96), a.reps))).astype(int)
+        reqs = [meshgen.rve_request(int(n), int(sd)) for n, sd in zip(ns, rng.integers(0, 2 ** 31 - 1, a.reps))]
+        with capi.Server(0, max_slots=4, tol=1e-8, max_pack=8) as srv:
+            ids = [srv.submit(r) for r in reqs]
+            sts = [srv.wait(i)[1] for i in ids]
+        out.update({"requests": a.reps, "n": [int(n) for n in ns],
+                    "case_iters": int(sum(st["case_iters"] for st in sts)),
+                    "worst_rel_resid": max(st["rel_resid"] for st in sts)})
+        print(json.dumps(out), flush=True)
+        return 0
     with capi.Context(0) as ctx:
         if a.workload == "elastic":
             n = a.n or 96

@@ -296,9 +296,6 @@ typedef struct bhip_plastic_stats {
  * stresses and effective plastic strain at every Gauss point; needs a J2
  * material, hex8 and the pattern. */
 int bhip_plastic_init(bhip_ctx *ctx);
-/* update("V") on every element (plasticElement.cpp:177-180): strain
- * increment from du (n_dof, increment displacement) and the FP64 return map
- * (PlasticMaterial::getStressIncrement, plastic.cpp:368-465). */
 /* Tangent of BHIP_ASM_PLASTIC_TANGENT: the reference's continuum C_ep
  * (getTangentStiffness, plastic.cpp:583-610; the default) or the symmetric
  * part of the exact derivative of the return map as executed (no reference
@@ -309,6 +306,9 @@ int bhip_plastic_init(bhip_ctx *ctx);
 #define BHIP_TANGENT_CONTINUUM   0
 #define BHIP_TANGENT_ALGORITHMIC 1
 int bhip_plastic_set_tangent(bhip_ctx *ctx, int tangent);
+/* update("V") on every element (plasticElement.cpp:177-180): strain
+ * increment from du (n_dof, increment displacement) and the FP64 return map
+ * (PlasticMaterial::getStressIncrement, plastic.cpp:368-465). */
 int bhip_plastic_update(bhip_ctx *ctx, const double *du);
 /* getGlobalForces + calculateResidualNorm (BasicModel.cpp:1063-1110,
  * equation.cpp:1516-1529): residual = load - resultant of the stress
