@@ -523,6 +523,10 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
     k_du_start<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->rest.p, ctx->presc.p, P->du.p);
     BHIP_TRY(ctx, hipGetLastError());
     bhip_solve_opts o = *opts;
+    // use_x0: every Newton correction starts from the previous correction
+    // (left in ctx->x by the last solve, zero before the first); otherwise
+    // from zero
+    const bool warm = opts->use_x0 != 0;
     o.use_x0 = 0;
     double maxres = 0;
     int it = 0;
@@ -536,6 +540,10 @@ int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double 
         if (it > max_iterations) break;
         // K d = residual (rhs already holds it), then du += d on free DOFs
         bhip_stats st;
+        if (warm) {
+            BHIP_TRY(ctx, hipMemcpyAsync(ctx->x0.p, ctx->x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            o.use_x0 = 1;
+        }
         int rc = bhip_solve_impl(ctx, &o, &st);
         if (rc != BHIP_OK) return rc;
         S.pcg_iters += st.iters;

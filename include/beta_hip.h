@@ -322,7 +322,9 @@ int bhip_plastic_commit(bhip_ctx *ctx);
  * (n_dof, increment; NULL keeps the current ones), the applied load increment
  * from bhip_set_load, Newton with the tangent reassembled every iteration
  * until max|residual| < 2*max_allowable_residual or max_iterations extra
- * solves, then update("U"). */
+ * solves, then update("U").  opts->use_x0 = 0 starts every Newton correction
+ * from zero; 1 starts it from the previous correction (the reference passes
+ * the increment's displacement as the guess, equation.cpp:1678-1686). */
 int bhip_plastic_increment(bhip_ctx *ctx, const double *presc_increment, double max_allowable_residual,
                            int max_iterations, const bhip_solve_opts *opts, bhip_plastic_stats *stats);
 /* Converged state: stress (n_el*nip*6), effective plastic strain (n_el*nip),

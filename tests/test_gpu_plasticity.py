@@ -232,3 +232,31 @@ def test_configured_path_20_increments(hip_ctx_factory, tangent):
         assert _yield_f(st, ep).max() <= 1e-6
         assert ep.min() >= 0
     assert hist[-1]["plastic_points"] > 0
+
+
+def test_newton_warm_start(hip_ctx_factory):
+    """use_x0 in the increment's solve options starts every Newton correction
+    from the previous one instead of from zero.  The converged path is the
+    same to the solve tolerance: the four elastic increments of the
+    configured 4^3 bar and the first plastic increments agree with the
+    zero-start run to 1e-7 relative, every increment converges, and the
+    warm-started corrections take no more inner iterations."""
+    print("case newton warm start", flush=True)
+    prob = meshgen.plastic_bar(4)
+    runs = []
+    for warm in (0, 1):
+        ctx = hip_ctx_factory()
+        setup_problem(ctx, prob)
+        ctx.plastic_init()
+        ctx.plastic_set_tangent(capi.TANGENT_ALGORITHMIC)
+        ctx.set_load(prob.load)
+        o = capi.Context.opts(tol=1e-10, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP32_IR,
+                              inner_tol=1e-3, maxit=40000, use_x0=warm)
+        hist = [ctx.plastic_increment(prob.prescribed, 1e-9, 300, opts=o) for _ in range(6)]
+        assert all(h["status"] == 0 for h in hist), hist
+        runs.append((hist, ctx.plastic_state(prob.n_el, 8)))
+    (h0, (s0, e0, u0)), (h1, (s1, e1, u1)) = runs
+    assert np.linalg.norm(u1 - u0) <= 1e-7 * np.linalg.norm(u0)
+    assert np.abs(s1 - s0).max() <= 1e-7 * np.abs(s0).max()
+    assert [h["newton_iters"] for h in h1] == [h["newton_iters"] for h in h0]
+    assert sum(h["pcg_iters"] for h in h1) <= sum(h["pcg_iters"] for h in h0)
