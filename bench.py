@@ -447,6 +447,7 @@ def main():
 
     from betanoox_amd import capi, meshgen, setup_problem
 
+    t_start = time.perf_counter()
     t0 = time.perf_counter()
     prob = meshgen.elastic_cube(args.n)
     log("generated %s: %d nodes, %d elements, %d DOF in %.1f s"
@@ -593,6 +594,7 @@ def main():
     if args.heat_n > 0:
         line["heat"] = heat_leg(args.heat_n, args.heat_steps, args.heat_fp64_steps, args.heat_inner_tol)
         log("heat: %s" % json.dumps(line["heat"]))
+        log("elapsed %.1f s" % (time.perf_counter() - t_start))
     if args.plastic_n > 0:
         line["plastic"] = plastic_leg(args.plastic_n, args.plastic_tol, 300, capi.TANGENT_ALGORITHMIC, True)
         log("plastic: %s" % json.dumps(line["plastic"]))
@@ -601,6 +603,7 @@ def main():
             line["plastic"]["reference_tangent"] = ref
             line["plastic"]["speedup_vs_reference_tangent"] = ref["wall_s"] / line["plastic"]["wall_s"]
             log("plastic (reference tangent): %s" % json.dumps(ref))
+        log("elapsed %.1f s" % (time.perf_counter() - t_start))
     if args.pack_demo > 0:
         line["packing"] = pack_leg(args.pack_demo)
         log("packing: %s" % json.dumps(line["packing"]))
@@ -608,8 +611,10 @@ def main():
         line["served"] = served_stream(args.served, args.served_load, args.served_probe,
                                        slots=args.served_slots, max_pack=args.served_pack)
         log("served: %s" % json.dumps(line["served"]))
+        log("elapsed %.1f s" % (time.perf_counter() - t_start))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_budget)
+    line["bench_wall_s"] = time.perf_counter() - t_start
     print(json.dumps(line), flush=True)
 
 

@@ -69,13 +69,15 @@ __device__ __forceinline__ void fin_sum(const double *partials, int nb, double (
 
 // Fixed-order sum of nb partial records by one block, result in every thread.
 // Thread t adds records t, t + blockDim.x, ... in that order; the loads of
-// eight records are issued before their sums so that they overlap (the
-// records come from L2 and a thread may own dozens of them).
+// U records (about 32 doubles) are issued before their sums so that they
+// overlap: the records come from L2 and a thread may own dozens of them
+// (33 for the SpMV records of a 2.1M-DOF heat matrix), so each round of U
+// is one L2 round trip.  U does not change the order of the sums.
 template <int NV>
 __device__ __forceinline__ void sum_records_all(const double *partials, int nb, double (&tot)[NV]) {
     __shared__ double sm[16 * 4];
     __shared__ double res[NV];
-    constexpr int U = 8;
+    constexpr int U = NV == 1 ? 32 : 32 / NV;
 #pragma unroll
     for (int k = 0; k < NV; k++) tot[k] = 0;
     for (int b0 = threadIdx.x; b0 < nb; b0 += U * blockDim.x) {
