@@ -665,7 +665,7 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     const int kind = ctx->mat_kind;
     int st = BHIP_OK;
     if (what == BHIP_ASM_STIFFNESS) {
-        if (ctx->nnpe == 8 && ctx->dim == 3 && ctx->max_ndof == 3 && ctx->uniform_ndof) {
+        if (ctx->nnpe == 8 && ctx->dim == 3 && ctx->max_ndof == 3 && ctx->elem_ndof_full) {
             if (kind == BHIP_MAT_ISO_ELASTIC)
                 st = launch_assembly<ElasHex8<BHIP_MAT_ISO_ELASTIC>>(ctx, A, O, nullptr, 0.0);
             else if (kind == BHIP_MAT_CMAT3D)
@@ -674,7 +674,7 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
                 st = launch_assembly<ElasHex8<BHIP_MAT_ISO_ELASTIC>>(ctx, A, O, nullptr, 0.0);
             else
                 return bhip_fail(ctx, BHIP_ERR_UNSUPPORTED, "material kind not valid for hex8 stiffness");
-        } else if (ctx->nnpe == 4 && ctx->dim == 2 && ctx->max_ndof == 2 && ctx->uniform_ndof &&
+        } else if (ctx->nnpe == 4 && ctx->dim == 2 && ctx->max_ndof == 2 && ctx->elem_ndof_full &&
                    kind == BHIP_MAT_PLANE_STRESS) {
             st = launch_assembly<ElasQuad4>(ctx, A, O, nullptr, 0.0);
         } else {

@@ -110,6 +110,7 @@ struct bhip_ctx {
     // ---- mesh
     int dim = 0, n_nodes = 0, n_el = 0, nnpe = 0, n_dof = 0, max_ndof = 0;
     bool uniform_ndof = true;
+    bool elem_ndof_full = true;  // every element node has max_ndof DOFs
     std::vector<int> h_conn, h_ndof, h_dof_first;
     dvec<double> xyz;
     dvec<int> conn, dof_first, ndof;

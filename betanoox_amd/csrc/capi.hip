@@ -154,6 +154,11 @@ int bhip_set_mesh(bhip_ctx *ctx, int dim, int n_nodes, const double *xyz, int n_
         if (k != ctx->h_ndof[0]) uni = false;
     }
     if (dof >= (1LL << 31)) return bhip_fail(ctx, BHIP_ERR_CAPACITY, "too many DOFs");
+    // every node of every element carries the largest DOF count (the element
+    // kernels' block size); nodes outside the elements may carry fewer
+    bool elem_full = true;
+    for (long long t = 0; t < (long long)n_el * nnpe && elem_full; t++)
+        if (ctx->h_ndof[conn[t]] != mx) elem_full = false;
     bhip_invalidate_graphs(ctx);
     ctx->dim = dim;
     ctx->n_nodes = n_nodes;
@@ -162,6 +167,7 @@ int bhip_set_mesh(bhip_ctx *ctx, int dim, int n_nodes, const double *xyz, int n_
     ctx->n_dof = (int)dof;
     ctx->max_ndof = mx;
     ctx->uniform_ndof = uni;
+    ctx->elem_ndof_full = elem_full;
     ctx->pattern_ready = ctx->assembled = ctx->f32_ready = ctx->f64b_ready = false;
     ctx->have_rest = false;
     ctx->external_matrix = false;
@@ -279,7 +285,7 @@ int bhip_get_csr(bhip_ctx *ctx, int *rowptr, int *colind, double *val) {
 
 int bhip_set_materials(bhip_ctx *ctx, int kind, int n_groups, const double *params, const int *elem_group) {
     if (!ctx || !params || n_groups <= 0) return BHIP_ERR_ARG;
-    if (ctx->n_el == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
+    if (ctx->n_nodes == 0) return bhip_fail(ctx, BHIP_ERR_STATE, "mesh not set");
     int stride;
     switch (kind) {
         case BHIP_MAT_ISO_ELASTIC: stride = 2; break;
@@ -307,7 +313,7 @@ int bhip_set_materials(bhip_ctx *ctx, int kind, int n_groups, const double *para
     (void)hipSetDevice(ctx->device);
     BHIP_TRY(ctx, ctx->mat.resize((size_t)n_groups * stride));
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->mat.p, params, sizeof(double) * n_groups * stride, hipMemcpyHostToDevice, ctx->stream));
-    if (elem_group) {
+    if (elem_group && ctx->n_el > 0) {
         BHIP_TRY(ctx, ctx->elem_group.resize(ctx->n_el));
         BHIP_TRY(ctx, hipMemcpyAsync(ctx->elem_group.p, elem_group, sizeof(int) * ctx->n_el, hipMemcpyHostToDevice, ctx->stream));
     } else {

@@ -338,6 +338,7 @@ int bhip_set_csr(bhip_ctx *ctx, int n, long long nnz, const int *ia, const int *
     ctx->n_dof = n;
     ctx->max_ndof = 0;
     ctx->uniform_ndof = false;
+    ctx->elem_ndof_full = false;
     ctx->have_rest = ctx->have_mat = false;
     ctx->f32_ready = ctx->f64b_ready = ctx->use_bsr = ctx->lift_ready = false;
     ctx->sell_b = 0;

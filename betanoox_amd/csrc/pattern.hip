@@ -230,7 +230,9 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipGetLastError());
     int nbits = 1;
     while ((1LL << nbits) < (long long)nn) nbits++;
-    {
+    // a mesh without elements has no keys: no sort, no pairs (every row then
+    // holds only its diagonal)
+    if (nkeys > 0) {
         size_t tb = 0;
         BHIP_TRY(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.p, keys2.p, (int)nkeys, 0,
                                                        32 + nbits, s));
@@ -242,7 +244,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
     // the unique count lands in the per-DOF scratch, which is not in use yet
     BHIP_TRY(ctx, ctx->tmp_l1.resize(std::max<size_t>(ctx->tmp_l1.n, 1)));
     int *nsel = reinterpret_cast<int *>(ctx->tmp_l1.p);
-    {
+    if (nkeys > 0) {
         size_t tb = 0;
         BHIP_TRY(ctx, hipcub::DeviceSelect::Unique(nullptr, tb, keys2.p, keys.p, nsel, (int)nkeys, s));
         BHIP_TRY(ctx, ctx->work.resize(tb > 0 ? tb : 1));
@@ -250,7 +252,7 @@ int bhip_pattern_build(bhip_ctx *ctx) {
         BHIP_TRY(ctx, hipStreamSynchronize(s));
     }
     int h_nsel = 0;
-    BHIP_TRY(ctx, bhip_copy(ctx, &h_nsel, nsel, sizeof(int), hipMemcpyDeviceToHost));
+    if (nkeys > 0) BHIP_TRY(ctx, bhip_copy(ctx, &h_nsel, nsel, sizeof(int), hipMemcpyDeviceToHost));
     ctx->n_pairs = h_nsel;
     long long np = ctx->n_pairs;
     BHIP_TRY(ctx, ctx->pair_row.resize(np > 0 ? np : 1));

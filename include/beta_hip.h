@@ -101,7 +101,10 @@ const char *bhip_version(void);
 /* BasicMesh::ReadMesh (BasicMesh.cpp:79-190) data + Node::setNumDofPerNode
  * (node.cpp:158-227).  xyz: n_nodes*dim doubles; conn: n_el*nnpe ints in the
  * reference node order (hex8: BasicElement.cpp:518-525; quad4 CCW);
- * ndof_per_node NULL means every node has ndof_uniform DOFs. */
+ * ndof_per_node NULL means every node has ndof_uniform DOFs.  Per-node counts
+ * may differ only on nodes outside every element (the stiffness assembly and
+ * the scalar CSR solves take such a mesh; the blocked forms, load cases,
+ * internal forces, heat and plasticity need the same count at every node). */
 int bhip_set_mesh(bhip_ctx *ctx, int dim, int n_nodes, const double *xyz, int n_el, int nnpe,
                   const int *conn, const int *ndof_per_node, int ndof_uniform);
 int bhip_num_dof(const bhip_ctx *ctx);
