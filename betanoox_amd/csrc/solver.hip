@@ -1629,9 +1629,24 @@ extern "C" int bhip_pcg_solve_noalloc(bhip_ctx *ctx, double *b, const double *x0
     // the pinned state mirror receives the flags word
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, flags, sizeof(int), hipMemcpyDeviceToHost, s));
     BHIP_TRY(ctx, hipStreamSynchronize(s));
-    const int hf = *reinterpret_cast<const int *>(ctx->h_state);
+    int hf = *reinterpret_cast<const int *>(ctx->h_state);
+    if (!(hf & 2) && (hf & 1) && ctx->zero_diag_value != 0.0) {
+        // the zero-diagonal replacement of bhip_solve, in place in the FP64
+        // values (the FP32 and FP64 block copies are rebuilt from them)
+        k_replace_zero_diag<<<grid_for(n, B), B, 0, s>>>((int)n, ctx->diag_pos.p, ctx->val.p, ctx->zero_diag_value);
+        ctx->f32_ready = ctx->f64b_ready = false;
+        k_fill_int<<<1, 64, 0, s>>>(4, flags, 0);
+        k_dinv<<<grid_for(n, B), B, 0, s>>>(n, ctx->diag_pos.p, ctx->val.p, ctx->dinv.p, nullptr, flags);
+        BHIP_TRY(ctx, hipGetLastError());
+        BHIP_TRY(ctx, hipMemcpyAsync(ctx->h_state, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+        BHIP_TRY(ctx, hipStreamSynchronize(s));
+        hf = *reinterpret_cast<const int *>(ctx->h_state);
+    }
     if (hf & 2) return bhip_fail(ctx, BHIP_ERR_NEG_DIAG, "negative diagonal term in the global matrix");
-    if (hf & 1) return bhip_fail(ctx, BHIP_ERR_ZERO_DIAG, "zero diagonal term in the global matrix");
+    if (hf & 1)
+        return bhip_fail(ctx, BHIP_ERR_ZERO_DIAG,
+                         "zero diagonal term in the global matrix; set a replacement with "
+                         "bhip_set_zero_diag_replacement");
 
     BHIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
     // r = b - K x0, z = p = r/diag (sparse.cpp:152-163), then the loop in

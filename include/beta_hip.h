@@ -179,8 +179,9 @@ int bhip_pcg_solve(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_
  * array per GPU (held until the process ends) instead of context buffers, so
  * the call allocates nothing once the context is assembled; a system needing
  * more than 524,224 reduction records (about 8.3M DOF at 81 entries per row)
- * is refused with BHIP_ERR_CAPACITY.  Calls that overlap on one GPU (other
- * threads or contexts) share that array and race. */
+ * is refused with BHIP_ERR_CAPACITY.  Zero diagonals are replaced as in
+ * bhip_pcg_solve when a replacement is set.  Calls that overlap on one GPU
+ * (other threads or contexts) share that array and race. */
 int bhip_pcg_solve_noalloc(bhip_ctx *ctx, double *b, const double *x0, const bhip_solve_opts *opts,
                            bhip_stats *stats);
 /* LargeMatrix::vectorProduct (sparse.cpp:452-478): y = K x. */

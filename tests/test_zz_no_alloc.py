@@ -2,7 +2,8 @@
 allocate device memory) against bhip_pcg_solve, called through the C ABI:
 the same iterates bit for bit on two calls in a row, the second on a smaller
 system, with and without an initial guess; no device memory is taken by the
-call; other precisions and oversized systems are refused."""
+call; zero diagonals are refused or replaced as by bhip_pcg_solve; other
+precisions are refused."""
 import ctypes as C
 
 import numpy as np
@@ -79,3 +80,48 @@ def test_noalloc_refusals():
         bb = np.zeros(3)
         st = capi.Stats()
         assert lib.bhip_pcg_solve_noalloc(c2.h, capi._d(bb), None, C.byref(o), C.byref(st)) == capi.ERR_STATE
+
+
+def _with_isolated_node():
+    """The 3^3 cube and one node outside every element: its three rows hold
+    only a zero diagonal."""
+    prob = meshgen.elastic_cube(3)
+    xyz = np.vstack([prob.xyz, [[2.0, 2.0, 2.0]]])
+    ctx = capi.Context(0)
+    ctx.set_mesh(3, xyz, prob.conn)
+    r = np.concatenate([prob.restrained, [0, 0, 0]])
+    ctx.set_restraints(r, np.concatenate([prob.prescribed, [0.0, 0.0, 0.0]]))
+    ctx.build_pattern()
+    ctx.set_materials(prob.mat_kind, prob.mat, prob.extra.get("group"))
+    ctx.set_integration(prob.order, prob.factor)
+    ctx.assemble()
+    ctx.set_load(np.concatenate([prob.load, [0.0, 0.0, 0.0]]))
+    ctx.build_rhs()
+    return ctx, ctx.rhs()
+
+
+def test_noalloc_zero_diagonal():
+    """Refused without a replacement value; with one, the same bits as
+    bhip_pcg_solve on a second context (each replaces in its own values)."""
+    print("noalloc case zero diagonal", flush=True)
+    lib = capi.load_library()
+    opts = capi.Context.opts(tol=1e-10, precision=capi.PREC_FP64, stop_rule=capi.STOP_REL_RESID, maxit=20000)
+    ctx, b = _with_isolated_node()
+    ref, b2 = _with_isolated_node()
+    try:
+        rc, _, _ = _call(lib.bhip_pcg_solve_noalloc, ctx, b, None, opts)
+        assert rc == capi.ERR_ZERO_DIAG
+        ctx.set_zero_diag_replacement(1.0)
+        ref.set_zero_diag_replacement(1.0)
+        free0 = _free_bytes()
+        rc1, x_na, s_na = _call(lib.bhip_pcg_solve_noalloc, ctx, b, None, opts)
+        assert _free_bytes() == free0
+        assert rc1 == 0, ctx.lib.bhip_last_error(ctx.h)
+        rc0, x_ref, s_ref = _call(lib.bhip_pcg_solve, ref, b2, None, opts)
+        assert rc0 == 0
+        assert np.array_equal(x_na, x_ref)
+        assert s_na["iters"] == s_ref["iters"]
+        assert not np.any(x_na[-3:])
+    finally:
+        ctx.close()
+        ref.close()
