@@ -50,7 +50,8 @@ constexpr int kMaxR = 8;
 constexpr int kChunk = 32;             // iterations per captured graph (even: r.z parity)
 constexpr int kSplitSlices = 2048;     // systems with fewer slices run four waves per slice
 constexpr double kStreamBytes = 240e6;
-constexpr int kPackUnroll = 2;         // SpMV block columns per loop trip
+constexpr int kPackUnroll = 2;         // SpMV block columns per loop trip (plain-load packs)
+constexpr int kPackUnrollStream = 3;   // the same for streaming packs (large systems)
 constexpr int kBatch = 4;              // elements per thread and trip in the vector kernels  // packs whose SELL arrays exceed the 256 MB last-level cache stream
 
 struct PackState {
@@ -164,8 +165,9 @@ __device__ __forceinline__ void pk_rows(const BHIP_GLOBAL CT *__restrict__ scol,
 // item.  Small systems: the four waves share the item's slice, each taking
 // every fourth block column, and wave 0 adds the others' row sums in wave
 // order through LDS.  Lanes past the system's last node skip their rows.
-// UNR block columns per loop trip (1: fewer registers, more waves per SIMD).
-// At least 4 waves per SIMD: the six-case form fits 126 VGPRs without spills.
+// UNR block columns per loop trip (1: fewer registers, more waves per SIMD;
+// 3: more loads of each wave in flight, for systems whose slices each get a
+// single wave).  At least 4 waves per SIMD: every form fits 128 VGPRs.
 template <int R, int NT, int UNR>
 __global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restrict__ hdr) {
     __shared__ float red[3 * R][kThreads];
@@ -413,6 +415,9 @@ void enqueue_pk_iteration(int grid, hipStream_t s, const PackHdr *hdr, int par, 
     if (unr == 1) {
         if (nt) k_pk_spmv<R, 1, 1><<<grid, kThreads, 0, s>>>(hdr);
         else k_pk_spmv<R, 0, 1><<<grid, kThreads, 0, s>>>(hdr);
+    } else if (unr == 3) {
+        if (nt) k_pk_spmv<R, 1, 3><<<grid, kThreads, 0, s>>>(hdr);
+        else k_pk_spmv<R, 0, 3><<<grid, kThreads, 0, s>>>(hdr);
     } else {
         if (nt) k_pk_spmv<R, 1, 2><<<grid, kThreads, 0, s>>>(hdr);
         else k_pk_spmv<R, 0, 2><<<grid, kThreads, 0, s>>>(hdr);
@@ -492,7 +497,7 @@ struct PackEngine {
     size_t h_resid_cap = 0;
     double *h_scale = nullptr;      // per system: R scales then R norms
     size_t h_scale_cap = 0;
-    hipGraphExec_t graph[4 * (kMaxR + 1)] = {};  // by (R, streaming, unroll)
+    hipGraphExec_t graph[6 * (kMaxR + 1)] = {};  // by (R, streaming, unroll 1-3)
 };
 
 namespace {
@@ -589,10 +594,12 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
     bool nt = by > kStreamBytes;
     const char *force = getenv("BHIP_PACK_STREAMING");
     if (force && (force[0] == '0' || force[0] == '1')) nt = force[0] == '1';
-    // block columns per SpMV loop trip (BHIP_PACK_UNROLL=1/2 forces it)
-    int unr = kPackUnroll;
+    // block columns per SpMV loop trip: three for streaming packs (a large
+    // system's waves each own a whole slice, so a wave's own loads in flight
+    // set the rate), two otherwise (BHIP_PACK_UNROLL=1/2/3 forces it)
+    int unr = nt ? kPackUnrollStream : kPackUnroll;
     const char *fu = getenv("BHIP_PACK_UNROLL");
-    if (fu && (fu[0] == '1' || fu[0] == '2')) unr = fu[0] - '0';
+    if (fu && (fu[0] == '1' || fu[0] == '2' || fu[0] == '3')) unr = fu[0] - '0';
     std::vector<int> total(n_all, 0), outer(n_all, 0), active(n_all, 1), ok(n_all, 0);
     std::vector<long long> case_total(n_all, 0);
     std::vector<double> worst(n_all, 1.0);  // worst unconverged relative residual per system
@@ -725,7 +732,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
         // chunks of kChunk iterations until every case of every system is done
         hipGraphExec_t g = nullptr;
         if (graphs_enabled()) {
-            hipGraphExec_t &slot = pe->graph[4 * R + (nt ? 2 : 0) + (unr == 1 ? 1 : 0)];
+            hipGraphExec_t &slot = pe->graph[6 * R + (nt ? 3 : 0) + (unr - 1)];
             if (!slot) {
                 hipGraph_t graph;
                 PK_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
