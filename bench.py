@@ -518,8 +518,8 @@ def main():
     value = prob.n_dof * iters / dt
 
     x_fp32 = ctx.solution()
-    # error of each form against an FP64 solve to 1e-12 (north_star states
-    # the error per result); the chosen form also at a tighter tolerance
+    # error of each form against an FP64 solve to 1e-12 (each result states
+    # its error against FP64); the chosen form also at a tighter tolerance
     errors = None
     if not args.no_fp64:
         s_ref = ctx.solve(tol=1e-12, stop_rule=capi.STOP_REL_RESID, precision=capi.PREC_FP64_BLOCKED, maxit=40000)
