@@ -57,6 +57,7 @@ VEC_SOLUTION = 2
 HEAT_X0_ZERO = 0
 HEAT_X0_PREVIOUS = 1
 HEAT_X0_REFERENCE = 2
+HEAT_X0_PROJECTED = 3
 
 # every symbol the header declares
 EXPORTS = [

@@ -181,7 +181,7 @@ struct bhip_ctx {
     double heat_a1 = 0, heat_dt = 1;  // coefficients of the last heat-transient assembly
     // bhip_heat_run buffers (grow-only, kept between runs): u_s, the FP64
     // column-run copy of K and K u_s
-    dvec<double> heat_us, heat_kruns, heat_ku;
+    dvec<double> heat_us, heat_kruns, heat_ku, heat_V, heat_z, heat_part;
     // ---- vectors (FP64)
     dvec<double> load, pseudo, rhs, x, x0, r, p, q, dinv, tmp;
     // ---- vectors (FP32 inner)

@@ -80,7 +80,7 @@ void bhip_destroy(bhip_ctx *ctx) {
     dvec<double> *v64[] = {&ctx->xyz,  &ctx->presc, &ctx->val, &ctx->val_aux, &ctx->mat, &ctx->load,
                            &ctx->pseudo, &ctx->rhs, &ctx->x,   &ctx->x0,      &ctx->r,   &ctx->p,
                            &ctx->q,    &ctx->dinv,  &ctx->tmp, &ctx->partials, &ctx->heat_us,
-                           &ctx->heat_kruns, &ctx->heat_ku};
+                           &ctx->heat_kruns, &ctx->heat_ku, &ctx->heat_V, &ctx->heat_z, &ctx->heat_part};
     for (auto *v : v64) v->release();
     dvec<int> *vi[] = {&ctx->conn,     &ctx->dof_first, &ctx->ndof,   &ctx->n2e_ptr, &ctx->n2e,
                        &ctx->nbr_ptr,  &ctx->nbr,       &ctx->pair_row, &ctx->nbr_off, &ctx->rowptr,

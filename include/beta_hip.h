@@ -380,6 +380,11 @@ int bhip_read_vector(const char *path, int kind, int n, double *v);
 #define BHIP_HEAT_X0_PREVIOUS   1   /* du of the previous step (0 on the first) */
 #define BHIP_HEAT_X0_REFERENCE  2   /* u_s on free DOFs, the prescribed value on restrained ones,
                                        as Equations::solve(u_s1) (equation.cpp:1367-1376) */
+#define BHIP_HEAT_X0_PROJECTED  3   /* no reference counterpart: the Mbar-orthogonal projection of the
+                                       step's solution on an Mbar-orthonormal basis of the increments
+                                       since the last restart (up to 16),
+                                       computed from the right-hand side alone (Fischer's projection
+                                       for successive right-hand sides) */
 
 typedef struct bhip_heat_stats {
     int    steps;            /* steps completed */

@@ -11,7 +11,8 @@ Checks and tolerances:
     eigenvector of the hex8 K and consistent M (2-point Gauss is exact), so
     backward Euler scales it by 1/(1 + 3 mu dt) per step with
     mu = 6 (1 - cos(pi h)) / (h^2 (2 + cos(pi h))); agreement to 1e-9
-  - the three initial-guess modes give the same temperatures (1e-8)
+  - the four initial-guess modes give the same temperatures (1e-8); the
+    projected guess takes fewer iterations than the previous increment
   - a run without a heat-transient assembly is refused
 """
 import numpy as np
@@ -83,17 +84,47 @@ def test_heat_sine_mode_decay(hip_ctx_factory):
 
 
 def test_heat_x0_modes_agree(hip_ctx_factory):
+    """The four initial-guess modes reach the same field (1e-8), and the
+    projected guess (whose basis of up to 16 increments restarts at step 17)
+    needs fewer inner iterations than the previous increment over 20 steps."""
     prob, u0 = _bumped(6)
     ctx = setup_problem(hip_ctx_factory(), prob)
     ctx.set_restraints(prob.restrained, prob.prescribed)
     ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[1.0, 1e-3], u_state=u0)
-    outs = []
-    for mode in (capi.HEAT_X0_ZERO, capi.HEAT_X0_PREVIOUS, capi.HEAT_X0_REFERENCE):
+    outs, iters = [], {}
+    for mode in (capi.HEAT_X0_ZERO, capi.HEAT_X0_PREVIOUS, capi.HEAT_X0_REFERENCE, capi.HEAT_X0_PROJECTED):
         u, st = ctx.heat_run(5, u0, mode, tol=1e-10, stop_rule=capi.STOP_REL_RESID,
                              precision=capi.PREC_FP32_IR, maxit=20000)
         outs.append(u)
     for u in outs[1:]:
         assert np.abs(u - outs[0]).max() <= 1e-8 * np.abs(outs[0]).max()
+    for mode in (capi.HEAT_X0_PREVIOUS, capi.HEAT_X0_PROJECTED):
+        u, st = ctx.heat_run(20, u0, mode, tol=1e-10, stop_rule=capi.STOP_REL_RESID,
+                             precision=capi.PREC_FP32_IR, maxit=20000)
+        iters[mode] = (st["iters"], u)
+    assert np.abs(iters[capi.HEAT_X0_PROJECTED][1] - iters[capi.HEAT_X0_PREVIOUS][1]).max() <= \
+        1e-8 * np.abs(iters[capi.HEAT_X0_PREVIOUS][1]).max()
+    assert iters[capi.HEAT_X0_PROJECTED][0] < iters[capi.HEAT_X0_PREVIOUS][0]
+
+
+def test_heat_projected_guess_fp64_and_csr(hip_ctx_factory):
+    """The projected guess on the plain FP64 solve (Mbar du on the FP64 CSR)
+    against the oracle's per-step reassembly, and repeated bit for bit."""
+    prob, u0 = _bumped(5, seed=2)
+    ctx = setup_problem(hip_ctx_factory(), prob)
+    ctx.set_restraints(prob.restrained, prob.prescribed)
+    ctx.assemble(capi.ASM_HEAT_TRANSIENT, coef=[1.0, 1e-3], u_state=u0)
+    u1, s1 = ctx.heat_run(12, u0, capi.HEAT_X0_PROJECTED, tol=1e-12, stop_rule=capi.STOP_REL_RESID,
+                          precision=capi.PREC_FP64, maxit=20000)
+    u2, _ = ctx.heat_run(12, u0, capi.HEAT_X0_PROJECTED, tol=1e-12, stop_rule=capi.STOP_REL_RESID,
+                         precision=capi.PREC_FP64, maxit=20000)
+    assert np.array_equal(u1, u2)
+    u3, s3 = ctx.heat_run(12, u0, capi.HEAT_X0_PREVIOUS, tol=1e-12, stop_rule=capi.STOP_REL_RESID,
+                          precision=capi.PREC_FP64, maxit=20000)
+    u_ref = _oracle_steps(prob, u0, 1.0, 1e-3, 12)
+    assert np.abs(u1 - u_ref).max() <= 1e-9 * np.abs(u_ref).max()
+    assert np.abs(u1 - u3).max() <= 1e-9 * np.abs(u3).max()
+    assert s1["iters"] < s3["iters"]
 
 
 def test_heat_run_needs_heat_assembly(hip_ctx_factory):
