@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
 
-SOURCES = ["capi.hip", "pattern.hip", "assemble.hip", "solver.hip", "plastic.hip", "multi.hip", "server.hip", "heat.hip", "formats.hip", "pack.hip", "forces.hip"]
+SOURCES = ["capi.hip", "pattern.hip", "assemble.hip", "solver.hip", "plastic.hip", "multi.hip", "server.hip", "heat.hip", "formats.hip", "pack.hip", "forces.hip", "project.hip"]
 
 
 def _run(cmd):

@@ -353,6 +353,19 @@ int bhip_prepare_solver(bhip_ctx *ctx, bool fp32);
 int bhip_prepare_refined(bhip_ctx *ctx);
 void bhip_runs_from_csr64(bhip_ctx *ctx, const double *csr_vals, double *out);
 void bhip_runs_spmv64(bhip_ctx *ctx, const double *vals, const double *x, double *y);
+// projected initial guesses (project.hip): at most kProjMax basis vectors;
+// a shares buffer of kProjShares doubles; coefficient arrays of kProjNV
+constexpr int kProjMax = 16;
+constexpr int kProjNV = kProjMax + 1;
+constexpr int kProjBlocks = 512;
+constexpr size_t kProjShares = (size_t)kProjBlocks * kProjNV;
+// coef[i] = v_i . y (i < m, 0 for slot skip); mode 1 adds coef[kProjMax] =
+// 1 / sqrt(a.y - sum coef[i]^2) (0 when that vanishes)
+void bhip_proj_dots(bhip_ctx *ctx, long long n, int m, const double *V, const double *a, const double *y, int skip,
+                    int mode, double *part, double *coef);
+// out = scale * (base + sign * sum_{i<m} coef[i] v_i)
+void bhip_proj_combo(bhip_ctx *ctx, long long n, int m, const double *V, const double *coef, const double *base,
+                     double sign, int use_scale, double *out);
 int bhip_residual_dev(bhip_ctx *ctx, const double *x, const double *b, double *r, double *rr, double *bb);
 const double *bhip_plastic_tangent(bhip_ctx *ctx);
 

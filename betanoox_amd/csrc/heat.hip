@@ -78,71 +78,8 @@ __global__ void k_add_inplace(long long n, double *u, const double *du) {
 // projection of P. F. Fischer (Comput. Methods Appl. Mech. Engrg. 163,
 // 1998).  (Keeping a sliding window instead, the new vector in the oldest
 // one's slot, took 41% more iterations on the 128^3 bump: the oldest
-// increment carries the slowest-decaying mode.)  Every sum is a fixed-order
-// reduction, so runs repeat bit for bit.
-constexpr int kProjMax = 16;           // increments kept
-constexpr int kProjNV = kProjMax + 1;  // dot products per pass (basis + one)
-constexpr int kProjBlocks = 512;
-
-// part[blk][i] = this block's share of v_i . y (i < m) and, when a is given,
-// of a . y in slot kProjMax.
-__global__ void __launch_bounds__(kThreads) k_proj_dots(long long n, int m, const double *__restrict__ V,
-                                                      const double *__restrict__ a, const double *__restrict__ y,
-                                                      double *__restrict__ part) {
-    __shared__ double sm[16 * kProjNV];
-    double acc[kProjNV];
-#pragma unroll
-    for (int i = 0; i < kProjNV; i++) acc[i] = 0;
-    const long long per = (n + gridDim.x - 1) / gridDim.x;
-    const long long i0 = (long long)blockIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
-    for (long long j = i0 + threadIdx.x; j < i1; j += blockDim.x) {
-        const double yj = y[j];
-#pragma unroll
-        for (int i = 0; i < kProjMax; i++)
-            if (i < m) acc[i] += V[(long long)i * n + j] * yj;
-        if (a) acc[kProjMax] += a[j] * yj;
-    }
-    block_sum<kProjNV>(acc, sm);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int i = 0; i < kProjNV; i++) part[(long long)blockIdx.x * kProjNV + i] = acc[i];
-}
-
-// coef[i] = sum of the nb block shares of slot i (fixed order).  mode 1 (new
-// basis vector): coef[kProjMax] = 1 / |w|, |w|^2 = du' Mbar du - sum coef[i]^2,
-// or 0 when w vanishes to rounding (the vector then adds nothing).
-__global__ void __launch_bounds__(kThreads) k_proj_fin(int nb, int m, const double *__restrict__ part,
-                                                     double *__restrict__ coef, int mode) {
-    __shared__ double sm[16 * kProjNV];
-    double acc[kProjNV];
-#pragma unroll
-    for (int i = 0; i < kProjNV; i++) acc[i] = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-#pragma unroll
-        for (int i = 0; i < kProjNV; i++) acc[i] += part[(long long)b * kProjNV + i];
-    block_sum<kProjNV>(acc, sm);
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < kProjMax; i++) coef[i] = i < m ? acc[i] : 0.0;
-        if (mode == 1) {
-            const double duz = acc[kProjMax];
-            double w2 = duz;
-            for (int i = 0; i < m; i++) w2 -= coef[i] * coef[i];
-            coef[kProjMax] = (duz > 0 && w2 > 1e-12 * duz) ? 1.0 / sqrt(w2) : 0.0;
-        }
-    }
-}
-
-// out = scale * (base + sign * sum_{i<m} coef[i] v_i); scale = coef[kProjMax]
-// when use_scale, else 1; base may be null.
-__global__ void k_proj_combo(long long n, int m, const double *__restrict__ V, const double *__restrict__ coef,
-                             const double *__restrict__ base, double sign, int use_scale, double *__restrict__ out) {
-    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    double t = 0;
-    for (int i = 0; i < m; i++) t += coef[i] * V[(long long)i * n + j];
-    const double v = (base ? base[j] : 0.0) + sign * t;
-    out[j] = use_scale ? coef[kProjMax] * v : v;
-}
+// increment carries the slowest-decaying mode.)  The sums are fixed-order
+// reductions (project.hip), so runs repeat bit for bit.
 
 int heat_lpr(bhip_ctx *ctx) {
     const double avg = ctx->n_dof > 0 ? (double)ctx->nnz / (double)ctx->n_dof : 1.0;
@@ -245,13 +182,12 @@ extern "C" int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode,
     if (proj) {
         if (ctx->heat_V.n < (size_t)kProjMax * n) BHIP_TRY(ctx, ctx->heat_V.resize((size_t)kProjMax * n));
         if (ctx->heat_z.n < (size_t)n) BHIP_TRY(ctx, ctx->heat_z.resize((size_t)n));
-        if (ctx->heat_part.n < (size_t)kProjBlocks * kProjNV + 16)
-            BHIP_TRY(ctx, ctx->heat_part.resize((size_t)kProjBlocks * kProjNV + 16));
+        if (ctx->heat_part.n < kProjShares + kProjNV) BHIP_TRY(ctx, ctx->heat_part.resize(kProjShares + kProjNV));
     }
     const bool z_sell = proj && opts->precision == BHIP_PREC_FP32_IR && ctx->use_bsr && ctx->f64b_ready &&
                         ctx->sell_b == 1;
     double *part = proj ? ctx->heat_part.p : nullptr;
-    double *coef = proj ? ctx->heat_part.p + (size_t)kProjBlocks * kProjNV : nullptr;
+    double *coef = proj ? ctx->heat_part.p + kProjShares : nullptr;
     // Fbar = -(a1 + a2) K u_s / dt with a1 + a2 = dt for every alpha
     const double fcoef = -1.0;
     int status = BHIP_OK;
@@ -267,9 +203,8 @@ extern "C" int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode,
             k_guess_ref<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->rest.p, us.p, ctx->presc.p, ctx->x0.p);
             o.use_x0 = 1;
         } else if (proj && pm > 0) {
-            k_proj_dots<<<kProjBlocks, kThreads, 0, s>>>(n, pm, ctx->heat_V.p, nullptr, ctx->rhs.p, part);
-            k_proj_fin<<<1, kThreads, 0, s>>>(kProjBlocks, pm, part, coef, 0);
-            k_proj_combo<<<grid_for(n, 256), 256, 0, s>>>(n, pm, ctx->heat_V.p, coef, nullptr, 1.0, 0, ctx->x0.p);
+            bhip_proj_dots(ctx, n, pm, ctx->heat_V.p, nullptr, ctx->rhs.p, -1, 0, part, coef);
+            bhip_proj_combo(ctx, n, pm, ctx->heat_V.p, coef, nullptr, 1.0, 0, ctx->x0.p);
             BHIP_TRY(ctx, hipGetLastError());
             o.use_x0 = 1;
         }
@@ -294,10 +229,8 @@ extern "C" int bhip_heat_run(bhip_ctx *ctx, int n_steps, double *u, int x0_mode,
             if (pm == kProjMax) pm = 0;
             if (z_sell) bhip_runs_spmv64(ctx, ctx->bval64.p, ctx->x.p, ctx->heat_z.p);
             else BHIP_PROPAGATE(bhip_spmv_dev(ctx, ctx->x.p, ctx->heat_z.p));
-            k_proj_dots<<<kProjBlocks, kThreads, 0, s>>>(n, pm, ctx->heat_V.p, ctx->x.p, ctx->heat_z.p, part);
-            k_proj_fin<<<1, kThreads, 0, s>>>(kProjBlocks, pm, part, coef, 1);
-            k_proj_combo<<<grid_for(n, 256), 256, 0, s>>>(n, pm, ctx->heat_V.p, coef, ctx->x.p, -1.0, 1,
-                                                          ctx->heat_V.p + (size_t)pm * n);
+            bhip_proj_dots(ctx, n, pm, ctx->heat_V.p, ctx->x.p, ctx->heat_z.p, -1, 1, part, coef);
+            bhip_proj_combo(ctx, n, pm, ctx->heat_V.p, coef, ctx->x.p, -1.0, 1, ctx->heat_V.p + (size_t)pm * n);
             BHIP_TRY(ctx, hipGetLastError());
             pm++;
         }
