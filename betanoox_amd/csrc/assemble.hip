@@ -446,6 +446,7 @@ struct AsmOut {
     float *sval32;
     double *sval64;
     const int *sptr, *nbr_ptr;
+    const unsigned char *pair_slot;  // union slots (nullptr: slot = the pair's rank in its row)
     long long nsell;
 };
 
@@ -462,7 +463,8 @@ __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const in
     const bool nlo = (n == lo);
     unsigned rn = O.rmask[n], rm = O.rmask[m];
     if (O.mode == 0 && O.sval32) {
-        const long long dst = (long long)O.sptr[n / 64] + 64LL * (p - O.nbr_ptr[n]) + (n % 64);
+        const long long slot = O.pair_slot ? (long long)O.pair_slot[p] : p - O.nbr_ptr[n];
+        const long long dst = (long long)O.sptr[n / 64] + 64LL * slot + (n % 64);
 #pragma unroll
         for (int c = 0; c < P::ND; c++)
 #pragma unroll
@@ -655,6 +657,7 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     O.sval64 = nullptr;
     O.sptr = ctx->sell_ptr.p;
     O.nbr_ptr = ctx->nbr_ptr.p;
+    O.pair_slot = ctx->sell_union ? ctx->pair_slot.p : nullptr;
     O.nsell = ctx->n_sell;
     const bool to_sell = bhip_sell_values_direct(ctx) && (what == BHIP_ASM_STIFFNESS || what == BHIP_ASM_PLASTIC_TANGENT);
     if (to_sell) {

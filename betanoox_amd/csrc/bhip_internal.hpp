@@ -146,6 +146,17 @@ struct bhip_ctx {
     // consecutive columns (16-bit offset of the first from the row's node) with
     // three values, so a 27-column hex8 row takes 9 entries.
     bool sell_runs = false;
+    // Union slots (three DOFs per node, 16-bit offsets): slot u of slice s
+    // holds, in every lane, the block at the u-th smallest offset of the
+    // union of the slice's neighbour offsets (zero where the lane's node has
+    // no such neighbour, its column then the node itself), so slot order is
+    // column order in every row.  sell_uoff[sell_ptr[s] / 64 + u] is that
+    // offset; pair_slot[p] is the slot of node pair p.  Runs of consecutive
+    // offsets let the SpMV gather x once per run and take the rest from the
+    // neighbouring lanes.
+    bool sell_union = false;
+    dvec<int> sell_uoff;
+    dvec<unsigned char> pair_slot;
     dvec<float> bval32;      // SELL block values, SoA: sell_vals(ctx) x n_sell
     // ---- FP64 values in the same SELL layout (BHIP_PREC_FP64_BLOCKED)
     bool f64b_ready = false;       // bval64 holds the current matrix

@@ -342,6 +342,7 @@ int bhip_set_csr(bhip_ctx *ctx, int n, long long nnz, const int *ia, const int *
     ctx->have_rest = ctx->have_mat = false;
     ctx->f32_ready = ctx->f64b_ready = ctx->use_bsr = ctx->lift_ready = false;
     ctx->sell_b = 0;
+    ctx->sell_union = false;
     ctx->sell_ptr.release();
     ctx->sell_col.release();
     ctx->bval32.release();

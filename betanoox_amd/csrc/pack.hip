@@ -64,6 +64,7 @@ struct PackState {
 struct PackSysDesc {
     const int *sptr, *scol;  // scol: node indices (wide form)
     const short *scol16;     // offsets from the row's node (narrow form)
+    const int *uoff;         // union slots: the slices' offset lists (nullptr: rank slots)
     int narrow, pad0;
     const float *sval;
     long long nsell;
@@ -161,6 +162,142 @@ __device__ __forceinline__ void pk_rows(const BHIP_GLOBAL CT *__restrict__ scol,
     }
 }
 
+// Union slots: the same row sums over runs of up to three consecutive offsets
+// of the slice's offset list (as in the one-case union SpMV): x is gathered
+// at node + o and node + o + len - 1, and the middle of a run of three comes
+// from the next lane (lane 63: lane 62's second gather) by DPP.  The wave
+// takes runs sel, sel + nsel, ... in order; every lane of the slice runs the
+// loop (zero blocks and clamped gathers where a lane has no such neighbour).
+typedef float f3u __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ float pk_lane_next(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float pk_lane_prev(float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(src), 0x138, 0xf, 0xf, false));
+}
+
+// Loads from a uniform base plus a 32-bit byte offset per lane (the
+// scalar-base form of the global loads: one offset register serves every
+// value array and every case).  The bases pass through readfirstlane so that
+// they stay scalar and are not folded into per-lane 64-bit addresses.  Union
+// slots are only built where every such offset fits 32 bits.
+template <class T>
+__device__ __forceinline__ const BHIP_GLOBAL T *pk_uniform(const BHIP_GLOBAL T *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (const BHIP_GLOBAL T *)(((unsigned long long)hi << 32) | lo);
+}
+
+template <int NT, class T>
+__device__ __forceinline__ T pk_ld(const BHIP_GLOBAL T *base, unsigned byte_off) {
+    return ld_stream<NT>((const BHIP_GLOBAL T *)((const BHIP_GLOBAL char *)base + byte_off));
+}
+
+template <int NT>
+__device__ __forceinline__ void pk_load_block(const BHIP_GLOBAL float *const (&vb)[9], unsigned koff, float (&v)[9]) {
+#pragma unroll
+    for (int e = 0; e < 9; e++) v[e] = pk_ld<NT>(vb[e], koff);
+}
+
+// Cases r0 <= r < r1 of a block (xs indexed from r0).
+template <int R, int R0, int R1>
+__device__ __forceinline__ void pk_block_rows(const float (&v)[9], const float (&xs)[R1 - R0][3], float (&a0)[R],
+                                              float (&a1)[R], float (&a2)[R]) {
+#pragma unroll
+    for (int r = R0; r < R1; r++) {
+        const float x0 = xs[r - R0][0], x1 = xs[r - R0][1], x2 = xs[r - R0][2];
+        a0[r] += v[0] * x0 + v[1] * x1 + v[2] * x2;
+        a1[r] += v[3] * x0 + v[4] * x1 + v[5] * x2;
+        a2[r] += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+}
+
+template <int R, int R0, int R1>
+__device__ __forceinline__ void pk_gather(const BHIP_GLOBAL f3u *const (&xb)[R], unsigned moff, float (&xs)[R1 - R0][3]) {
+#pragma unroll
+    for (int r = R0; r < R1; r++) {
+        const f3u t = pk_ld<0>(xb[r], moff);
+        xs[r - R0][0] = t.x;
+        xs[r - R0][1] = t.y;
+        xs[r - R0][2] = t.z;
+    }
+}
+
+// The three blocks of a run of three consecutive offsets for cases r0..r1-1.
+template <int R, int R0, int R1>
+__device__ __forceinline__ void pk_run3(const float (&v0)[9], const float (&v1)[9], const float (&v2)[9],
+                                        const BHIP_GLOBAL f3u *const (&xb)[R], unsigned o0, unsigned o2,
+                                        float (&a0)[R], float (&a1)[R], float (&a2)[R]) {
+    constexpr int N = R1 - R0;
+    float x0[N][3], x1[N][3], x2[N][3];
+    pk_gather<R, R0, R1>(xb, o0, x0);
+    pk_gather<R, R0, R1>(xb, o2, x2);
+#pragma unroll
+    for (int r = 0; r < N; r++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) x1[r][d] = pk_lane_next(pk_lane_prev(x2[r][d]), x0[r][d]);
+    pk_block_rows<R, R0, R1>(v0, x0, a0, a1, a2);
+    pk_block_rows<R, R0, R1>(v1, x1, a0, a1, a2);
+    pk_block_rows<R, R0, R1>(v2, x2, a0, a1, a2);
+}
+
+// Each run length has its own branch that issues every load of the run
+// before its multiply-adds.
+template <int R, int NT>
+__device__ __forceinline__ void pk_rows_union(const BHIP_GLOBAL int *__restrict__ of, const BHIP_GLOBAL float *__restrict__ sval,
+                                              long long nsell, const BHIP_GLOBAL float *__restrict__ x, long long ld,
+                                              long long node, long long last, long long kb, int W, int sel, int nsel,
+                                              float (&a0)[R], float (&a1)[R], float (&a2)[R]) {
+    const int lane = threadIdx.x & 63;
+    const BHIP_GLOBAL float *vb[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) vb[e] = pk_uniform(sval + e * nsell);
+    const BHIP_GLOBAL f3u *xb[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) xb[r] = pk_uniform((const BHIP_GLOBAL f3u *)(x + r * ld));
+    int run = 0;
+    for (int j = 0; j < W; run++) {
+        const int o = of[j];
+        const int len = (j + 2 < W && of[j + 2] == o + 2) ? 3 : (j + 1 < W && of[j + 1] == o + 1) ? 2 : 1;
+        if (run % nsel == sel) {
+            const unsigned koff = (unsigned)(kb + 64LL * j + lane) * 4u;
+            long long m0 = node + o, m2 = node + o + len - 1;
+            m0 = m0 < 0 ? 0 : (m0 > last ? last : m0);
+            m2 = m2 < 0 ? 0 : (m2 > last ? last : m2);
+            const unsigned o0 = (unsigned)m0 * 12u, o2 = (unsigned)m2 * 12u;
+            if (len == 3) {
+                // the cases in two halves: the second half's gathers reuse
+                // the first half's registers
+                float v0[9], v1[9], v2[9];
+                pk_load_block<NT>(vb, koff, v0);
+                pk_load_block<NT>(vb, koff + 256u, v1);
+                pk_load_block<NT>(vb, koff + 512u, v2);
+                if constexpr (R >= 2) {
+                    pk_run3<R, 0, R / 2>(v0, v1, v2, xb, o0, o2, a0, a1, a2);
+                    pk_run3<R, R / 2, R>(v0, v1, v2, xb, o0, o2, a0, a1, a2);
+                } else {
+                    pk_run3<R, 0, R>(v0, v1, v2, xb, o0, o2, a0, a1, a2);
+                }
+            } else if (len == 2) {
+                float v0[9], v1[9], x0[R][3], x1[R][3];
+                pk_load_block<NT>(vb, koff, v0);
+                pk_load_block<NT>(vb, koff + 256u, v1);
+                pk_gather<R, 0, R>(xb, o0, x0);
+                pk_gather<R, 0, R>(xb, o2, x1);
+                pk_block_rows<R, 0, R>(v0, x0, a0, a1, a2);
+                pk_block_rows<R, 0, R>(v1, x1, a0, a1, a2);
+            } else {
+                float v0[9], x0[R][3];
+                pk_load_block<NT>(vb, koff, v0);
+                pk_gather<R, 0, R>(xb, o0, x0);
+                pk_block_rows<R, 0, R>(v0, x0, a0, a1, a2);
+            }
+        }
+        j += len;
+    }
+}
+
 // q_r = K p_r for every case of every live system; record of p_r.q_r per
 // item.  Small systems: the four waves share the item's slice, each taking
 // every fourth block column, and wave 0 adds the others' row sums in wave
@@ -168,11 +305,15 @@ __device__ __forceinline__ void pk_rows(const BHIP_GLOBAL CT *__restrict__ scol,
 // UNR block columns per loop trip (1: fewer registers, more waves per SIMD;
 // 3: more loads of each wave in flight, for systems whose slices each get a
 // single wave).  At least 4 waves per SIMD: every form fits 128 VGPRs.
+// UNR = 0: a pack whose systems all have union slots (no rank-slot path
+// compiled in).
 template <int R, int NT, int UNR>
-__global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restrict__ hdr) {
+__global__ void __launch_bounds__(kThreads, UNR == 0 && R <= 6 ? 5 : 4) k_pk_spmv(const PackHdr *__restrict__ hdr) {
     __shared__ float red[3 * R][kThreads];
     __shared__ double sm[16 * R];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wave index as a uniform value: the slice, its pointers and its offset
+    // list then come from scalar loads
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int n_items = hdr->n_sitems;
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const BHIP_GLOBAL PackSysDesc &D = to_global(hdr->sys)[to_global(hdr->sitem_sys)[item]];
@@ -193,7 +334,15 @@ __global__ void __launch_bounds__(kThreads, 4) k_pk_spmv(const PackHdr *__restri
             dots[r] = 0;
             a0[r] = a1[r] = a2[r] = 0.f;
         }
-        if (slice < D.n_slices && node < D.n_nodes) {
+        if (UNR == 0 || D.uoff) {
+            // union slots: all 64 lanes of the slice walk its runs
+            if (slice < D.n_slices) {
+                const int kb = to_global(D.sptr)[slice], W = (to_global(D.sptr)[slice + 1] - kb) >> 6;
+                pk_rows_union<R, NT>(to_global(D.uoff) + (kb >> 6), to_global(D.sval), D.nsell, to_global(D.p), ld,
+                                     node, (long long)D.n_nodes - 1, kb, W, split ? wave : 0, split ? 4 : 1, a0, a1,
+                                     a2);
+            }
+        } else if (UNR > 0 && slice < D.n_slices && node < D.n_nodes) {
             const int kb = to_global(D.sptr)[slice], ke = to_global(D.sptr)[slice + 1];
             const int k0 = kb + lane + (split ? 64 * wave : 0), kstep = split ? 256 : 64;
             if (D.narrow)
@@ -412,7 +561,10 @@ __global__ void __launch_bounds__(kThreads) k_pk_init_state(const PackHdr *__res
 
 template <int R>
 void enqueue_pk_iteration(int grid, hipStream_t s, const PackHdr *hdr, int par, bool nt, int unr) {
-    if (unr == 1) {
+    if (unr == 0) {
+        if (nt) k_pk_spmv<R, 1, 0><<<grid, kThreads, 0, s>>>(hdr);
+        else k_pk_spmv<R, 0, 0><<<grid, kThreads, 0, s>>>(hdr);
+    } else if (unr == 1) {
         if (nt) k_pk_spmv<R, 1, 1><<<grid, kThreads, 0, s>>>(hdr);
         else k_pk_spmv<R, 0, 1><<<grid, kThreads, 0, s>>>(hdr);
     } else if (unr == 3) {
@@ -497,7 +649,7 @@ struct PackEngine {
     size_t h_resid_cap = 0;
     double *h_scale = nullptr;      // per system: R scales then R norms
     size_t h_scale_cap = 0;
-    hipGraphExec_t graph[6 * (kMaxR + 1)] = {};  // by (R, streaming, unroll 1-3)
+    hipGraphExec_t graph[8 * (kMaxR + 1)] = {};  // by (R, streaming, unroll 0-3)
 };
 
 namespace {
@@ -657,6 +809,10 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
         }
         // descriptor of the systems in this inner solve
         const int ns = (int)run.size();
+        // every system with union slots: the union-only SpMV (unroll 0)
+        bool all_union = true;
+        for (int i : run) all_union &= ctxs[i]->sell_union;
+        const int run_unr = all_union ? 0 : unr;
         int n_s = 0, n_v = 0;
         long long n_rec = 0;
         std::vector<PackSysDesc> desc(ns);
@@ -667,6 +823,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
             D.sptr = c->sell_ptr.p;
             D.scol = c->sell_col.p;
             D.scol16 = c->sell_col16.p;
+            D.uoff = c->sell_union ? c->sell_uoff.p : nullptr;
             D.narrow = c->sell_narrow ? 1 : 0;
             D.pad0 = 0;
             D.sval = c->bval32.p;
@@ -732,11 +889,11 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
         // chunks of kChunk iterations until every case of every system is done
         hipGraphExec_t g = nullptr;
         if (graphs_enabled()) {
-            hipGraphExec_t &slot = pe->graph[6 * R + (nt ? 3 : 0) + (unr - 1)];
+            hipGraphExec_t &slot = pe->graph[8 * R + (nt ? 4 : 0) + run_unr];
             if (!slot) {
                 hipGraph_t graph;
                 PK_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt, unr);
+                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt, run_unr);
                 hipError_t e = hipStreamEndCapture(s, &graph);
                 PK_TRY(e);
                 e = hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0);
@@ -764,7 +921,7 @@ int bhip_pack_run(PackEngine *pe, bhip_ctx *const *ctxs, int n_all, int R, const
             if (g) {
                 PK_TRY(hipGraphLaunch(g, s));
             } else {
-                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt, unr);
+                for (int k = 0; k < kChunk; k++) enqueue_iteration(R, pe->grid, s, hdr, k & 1, nt, run_unr);
             }
             PK_TRY(hipGetLastError());
             for (int k = 0; k < ns; k++) stats[run[k]].spmv_launches += kChunk;

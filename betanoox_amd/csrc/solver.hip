@@ -285,6 +285,166 @@ __global__ void __launch_bounds__(64 * SPB) k_sell_spmv(int n_nodes, int n_slice
     }
 }
 
+// Union-slot SpMV (B = 3, 16-bit layout with union slots).  Every lane of a
+// slice walks the same slots, whose offsets are the slice's offset list; the
+// list is cut greedily into runs of up to three consecutive offsets.  For a
+// run (o, o+1, o+2) every lane gathers x at node + o and node + o + 2 and
+// takes x at node + o + 1 from the next lane's first gather (lane 63: from
+// lane 62's second gather) by a DPP wave shift; a run of two gathers both
+// nodes, a single offset one.  Lanes whose node lacks a neighbour hold zero
+// blocks there and gather a clamped index, and all 64 lanes run the loop, so
+// the shifted values always come from loaded lanes.  Each lane adds its
+// blocks in slot order with the expression of k_sell_spmv, so the result,
+// the partials and the residual records are those of k_sell_spmv on the same
+// matrix, to the bit.
+template <class VT>
+__device__ __forceinline__ void rows_from_block(const VT (&v)[9], const VT (&xv)[3], VT (&a)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        VT t = v[c * 3] * xv[0];
+#pragma unroll
+        for (int d = 1; d < 3; d++) t += v[c * 3 + d] * xv[d];
+        a[c] += t;
+    }
+}
+
+// Lane l receives src of lane l + 1; lane 63 keeps old.
+__device__ __forceinline__ float lane_next(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x130, 0xf, 0xf, false));
+}
+// Lane l receives src of lane l - 1 (lane 0: zero).
+__device__ __forceinline__ float lane_prev(float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(src), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_next(double old, double src) {
+    const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_prev(double src) {
+    const long long v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int NT, class VT>
+__device__ __forceinline__ void load_block3(const VT *__restrict__ sval, long long nsell, long long k, VT (&v)[9]) {
+#pragma unroll
+    for (int e = 0; e < 9; e++) v[e] = ld_stream<NT>(sval + e * nsell + k);
+}
+
+template <class VT>
+__device__ __forceinline__ void gather3(const VT *__restrict__ x, long long m, VT (&xv)[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) xv[d] = x[3 * m + d];
+}
+
+template <int MODE, int SPB, int NT, class VT>
+__global__ void __launch_bounds__(64 * SPB) k_sell_spmv_union(int n_nodes, int n_slices, const int *__restrict__ sptr,
+                                                            const int *__restrict__ uoff,
+                                                            const VT *__restrict__ sval, long long nsell,
+                                                            const VT *__restrict__ x, VT *__restrict__ y,
+                                                            const VT *__restrict__ b, const PcgState *st,
+                                                            double *partials) {
+    if (MODE == 1 && st->done) return;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long s = (long long)lb * SPB + wave;
+    double acc = 0, acc_b = 0;
+    if (s < n_slices) {
+        const long long node = s * 64 + lane;
+        const int kb = sptr[s];
+        const int W = (sptr[s + 1] - kb) >> 6;
+        const int *of = uoff + (kb >> 6);
+        const long long last = n_nodes - 1;
+        VT a[3] = {0, 0, 0};
+        for (int j = 0; j < W;) {
+            const int o = of[j];
+            const int len = (j + 2 < W && of[j + 2] == o + 2) ? 3 : (j + 1 < W && of[j + 1] == o + 1) ? 2 : 1;
+            const long long k = (long long)kb + 64LL * j + lane;
+            long long m0 = node + o, m2 = node + o + len - 1;
+            m0 = m0 < 0 ? 0 : (m0 > last ? last : m0);
+            m2 = m2 < 0 ? 0 : (m2 > last ? last : m2);
+            if (len == 3) {
+                VT v0[9], v1[9], v2[9], x0[3], x2[3], x1[3];
+                load_block3<NT>(sval, nsell, k, v0);
+                load_block3<NT>(sval, nsell, k + 64, v1);
+                load_block3<NT>(sval, nsell, k + 128, v2);
+                gather3(x, m0, x0);
+                gather3(x, m2, x2);
+#pragma unroll
+                for (int d = 0; d < 3; d++) x1[d] = lane_next(lane_prev(x2[d]), x0[d]);
+                rows_from_block(v0, x0, a);
+                rows_from_block(v1, x1, a);
+                rows_from_block(v2, x2, a);
+            } else if (len == 2) {
+                VT v0[9], v1[9], x0[3], x1[3];
+                load_block3<NT>(sval, nsell, k, v0);
+                load_block3<NT>(sval, nsell, k + 64, v1);
+                gather3(x, m0, x0);
+                gather3(x, m2, x1);
+                rows_from_block(v0, x0, a);
+                rows_from_block(v1, x1, a);
+            } else {
+                VT v0[9], x0[3];
+                load_block3<NT>(sval, nsell, k, v0);
+                gather3(x, m0, x0);
+                rows_from_block(v0, x0, a);
+            }
+            j += len;
+        }
+        if (node < n_nodes) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if (MODE == 2) {
+                    const VT bi = b[3 * node + c], ri = bi - a[c];
+                    y[3 * node + c] = ri;
+                    acc += (double)ri * (double)ri;
+                    acc_b += (double)bi * (double)bi;
+                } else {
+                    y[3 * node + c] = a[c];
+                    if (MODE == 1) acc += (double)x[3 * node + c] * a[c];
+                }
+            }
+        }
+    }
+    if (MODE == 1) {
+        __shared__ double sm[SPB];
+        acc = wave_sum(acc);
+        if (lane == 0) sm[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x < SPB / 4) {
+            double t = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) t += sm[4 * threadIdx.x + w];
+            partials[(long long)lb * (SPB / 4) + threadIdx.x] = t;
+        }
+    } else if (MODE == 2) {
+        __shared__ double sm[SPB][2];
+        acc = wave_sum(acc);
+        acc_b = wave_sum(acc_b);
+        if (lane == 0) {
+            sm[wave][0] = acc;
+            sm[wave][1] = acc_b;
+        }
+        __syncthreads();
+        if (threadIdx.x < SPB / 4) {
+            double t0 = 0, t1 = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                t0 += sm[4 * threadIdx.x + w][0];
+                t1 += sm[4 * threadIdx.x + w][1];
+            }
+            const long long g = (long long)lb * (SPB / 4) + threadIdx.x;
+            partials[2 * g] = t0;
+            partials[2 * g + 1] = t1;
+        }
+    }
+}
+
 // --------------------------------------------------------- vector kernels
 // x += alpha p; r -= alpha q; z = r*dinv; partials r.z, r.r, x.x.
 template <class T>
@@ -600,6 +760,52 @@ __global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, CT *scol)
     }
 }
 
+// Union slots, one wave per slice: every lane walks its node's sorted
+// neighbour list; each step takes the smallest offset (m - n) still at the
+// head of any lane's list as the next slot, and the lanes holding it advance.
+// Without uoff (first pass) the slice width 64 |union| goes to width[s];
+// with it, the offsets go to uoff[sptr[s] / 64 + u] and the slot of every
+// node pair to pair_slot.  A union above kMaxUnionSlots stops early and
+// reports kMaxUnionSlots + 1 slots (the caller then keeps rank slots).
+constexpr int kMaxUnionSlots = 255;
+__global__ void __launch_bounds__(256) k_union_slots(int n_nodes, int n_slices, const int *nbr_ptr, const int *nbr,
+                                                     const int *sptr, int *width, int *uoff,
+                                                     unsigned char *pair_slot) {
+    const int lane = threadIdx.x & 63;
+    const long long s = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (s >= n_slices) return;  // whole waves leave together
+    const long long n = s * 64 + lane;
+    int p = 0, pe = 0;
+    if (n < n_nodes) {
+        p = nbr_ptr[n];
+        pe = nbr_ptr[n + 1];
+    }
+    constexpr int kNone = 0x7fffffff;
+    int head = p < pe ? nbr[p] - (int)n : kNone;
+    const long long ub = uoff ? sptr[s] / 64 : 0;
+    int u = 0;
+    for (;;) {
+        int mn = head;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mn = min(mn, __shfl_xor(mn, off, 64));
+        if (mn == kNone) break;
+        if (u == kMaxUnionSlots) {
+            u++;
+            break;
+        }
+        if (uoff) {
+            if (lane == 0) uoff[ub + u] = mn;
+            if (head == mn) pair_slot[p] = (unsigned char)u;
+        }
+        if (head == mn) {
+            p++;
+            head = p < pe ? nbr[p] - (int)n : kNone;
+        }
+        u++;
+    }
+    if (!uoff && lane == 0) width[s] = 64 * u;
+}
+
 // Largest |neighbour - node| over the node adjacency (into *out, zeroed
 // before): grid-stride per thread, wave maximum, one atomic per wave.
 __global__ void k_max_offset(long long n_pairs, const int *pair_row, const int *nbr, int *out) {
@@ -693,17 +899,18 @@ __global__ void k_csr_to_runs(int n_nodes, const int *nbr_ptr, const int *nbr, c
 }
 
 // FP64 CSR -> SELL-64 B x B blocks of VT (SoA); restrained entries zero,
-// restrained diagonal 1.  Pair p = (node n, k-th neighbour m).  sval64, when
+// restrained diagonal 1.  Pair p = (node n, k-th neighbour m) goes to slot k
+// of its slice, or to slot pair_slot[p] in the union layout.  sval64, when
 // given, receives the same blocks in FP64 in the same pass.
 template <int B, class CT, class VT>
 __global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
                               const int *nbr_off, const int *rowptr, const unsigned char *rmask,
                               const double *val, const int *sptr, long long nsell, CT *scol, VT *sval,
-                              double *sval64) {
+                              double *sval64, const unsigned char *pair_slot) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_pairs) return;
     int n = pair_row[p], m = nbr[p];
-    int k = (int)(p - nbr_ptr[n]);
+    int k = pair_slot ? (int)pair_slot[p] : (int)(p - nbr_ptr[n]);
     long long dst = (long long)sptr[n / 64] + 64LL * k + (n % 64);
     if constexpr (sizeof(CT) == 2) scol[dst] = (CT)(m - n);
     else scol[dst] = m;
@@ -797,9 +1004,24 @@ void launch_csr(bhip_ctx *ctx, const VT *val, const XT *x, XT *y, const double *
 #undef BHIP_CSR_CASE
 }
 
+// UNR = 0: runs of union slots (k_sell_spmv_union) where the layout has
+// union slots, else four block columns per trip.
 template <int B, int MODE, int SPB, int UNR, int NT, class VT>
 void launch_sell_t(bhip_ctx *ctx, const VT *sval, const VT *x, VT *y, const PcgState *st,
                    const VT *b = nullptr) {
+    if constexpr (UNR == 0) {
+        if constexpr (B == 3) {
+            if (ctx->sell_union) {
+                long long g = (ctx->n_slices + SPB - 1) / SPB;
+                k_sell_spmv_union<MODE, SPB, NT, VT><<<(int)(g > 0 ? g : 1), 64 * SPB, 0, ctx->stream>>>(
+                    ctx->n_nodes, ctx->n_slices, ctx->sell_ptr.p, ctx->sell_uoff.p, sval, ctx->n_sell, x, y, b, st,
+                    ctx->partials.p);
+                return;
+            }
+        }
+        launch_sell_t<B, MODE, SPB, 4, NT, VT>(ctx, sval, x, y, st, b);
+        return;
+    }
     long long g = (ctx->n_slices + SPB - 1) / SPB;
     const int grid = (int)(g > 0 ? g : 1);
     if constexpr (B == 1) {
@@ -821,11 +1043,15 @@ void launch_sell_t(bhip_ctx *ctx, const VT *sval, const VT *x, VT *y, const PcgS
 }
 
 // Tile shapes in shape-id order (bhip_tune_spmv reports and sets the id).
-constexpr int kNumSellShapes = 18;
+// Shapes 18-20 (unroll 0) walk runs of union slots; on a layout without
+// union slots they run as unroll 4.
+constexpr int kNumSellShapes = 21;
+constexpr int kFirstUnionShape = 18;
 #define BHIP_SELL_SHAPES(X)                                                                               \
     X(0, 4, 4, 0) X(1, 4, 2, 0) X(2, 4, 8, 0) X(3, 8, 4, 0) X(4, 8, 2, 0) X(5, 8, 8, 0) X(6, 16, 4, 0)    \
     X(7, 16, 2, 0) X(8, 16, 8, 0) X(9, 4, 4, 1) X(10, 4, 2, 1) X(11, 4, 8, 1) X(12, 8, 4, 1)             \
-    X(13, 8, 2, 1) X(14, 8, 8, 1) X(15, 16, 4, 1) X(16, 16, 2, 1) X(17, 16, 8, 1)
+    X(13, 8, 2, 1) X(14, 8, 8, 1) X(15, 16, 4, 1) X(16, 16, 2, 1) X(17, 16, 8, 1) X(18, 4, 0, 1)         \
+    X(19, 8, 0, 1) X(20, 16, 0, 1)
 
 template <int B, class VT>
 void launch_sell_b(bhip_ctx *ctx, int mode, int shape, const VT *sval, const VT *x, VT *y, const PcgState *st) {
@@ -867,7 +1093,7 @@ void launch_sell64(bhip_ctx *ctx, int mode, const double *x, double *y, const Pc
 // |r|^2 and |b|^2 for k_fin_resid over sell_groups records.
 void launch_sell64_resid(bhip_ctx *ctx, const double *x, double *r, const double *b) {
     const double *v = ctx->bval64.p;
-    if (ctx->sell_b == 3) launch_sell_t<3, 2, 4, 4, 1>(ctx, v, x, r, nullptr, b);
+    if (ctx->sell_b == 3) launch_sell_t<3, 2, 4, 0, 1>(ctx, v, x, r, nullptr, b);
     else if (ctx->sell_b == 2) launch_sell_t<2, 2, 4, 4, 1>(ctx, v, x, r, nullptr, b);
     else launch_sell_t<1, 2, 4, 4, 1>(ctx, v, x, r, nullptr, b);
 }
@@ -963,6 +1189,13 @@ int tuned_default_shape(int b, long long n_slices) {
     return 0;
 }
 
+// Default shape of a layout with union slots: the runs form, four slices per
+// workgroup.
+int tuned_default_union_shape(long long n_slices) {
+    (void)n_slices;
+    return kFirstUnionShape;
+}
+
 // Slice widths (x64) into ctx->tmp_i by the given kernel, returned on the host.
 int slice_widths(bhip_ctx *ctx, bool runs, std::vector<int> &hw) {
     hipStream_t s = ctx->stream;
@@ -988,6 +1221,34 @@ int slice_widths(bhip_ctx *ctx, bool runs, std::vector<int> &hw) {
 bool runs_enabled() {
     const char *e = getenv("BHIP_SELL_RUNS");
     return !(e && e[0] == '0');
+}
+
+// Union slots replace rank slots for three-DOF matrices with 16-bit offsets
+// when they store at most 10% more slots (structured hex8 meshes: within
+// 1%); BHIP_SELL_UNION=0 keeps rank slots.
+bool union_enabled() {
+    const char *e = getenv("BHIP_SELL_UNION");
+    return !(e && e[0] == '0');
+}
+
+// Union widths (x64) of every slice into hw; false when a slice's union
+// exceeds kMaxUnionSlots.
+int union_widths(bhip_ctx *ctx, std::vector<int> &hw, bool *ok) {
+    hipStream_t s = ctx->stream;
+    const int ns = ctx->n_slices;
+    dvec<int> &width = ctx->tmp_i;
+    BHIP_TRY(ctx, width.resize(ns + 1));
+    BHIP_TRY(ctx, hipMemsetAsync(width.p, 0, sizeof(int) * (ns + 1), s));
+    k_union_slots<<<grid_for((long long)ns * 64, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, ctx->nbr.p,
+                                                                    nullptr, width.p, nullptr, nullptr);
+    BHIP_TRY(ctx, hipGetLastError());
+    hw.assign(ns + 1, 0);
+    BHIP_TRY(ctx, hipMemcpyAsync(hw.data(), width.p, sizeof(int) * (ns + 1), hipMemcpyDeviceToHost, s));
+    BHIP_TRY(ctx, hipStreamSynchronize(s));
+    *ok = true;
+    for (int i = 0; i < ns; i++)
+        if (hw[i] > 64 * kMaxUnionSlots) *ok = false;
+    return BHIP_OK;
 }
 
 int build_sell(bhip_ctx *ctx) {
@@ -1020,6 +1281,21 @@ int build_sell(bhip_ctx *ctx) {
             tot = tr;
         }
     }
+    bool uni = false;
+    // (the union kernels address block values and x entries by 32-bit byte
+    // offsets: FP64 values need 8 n_sell < 2^32, the x gathers 24 n_nodes < 2^32)
+    if (ctx->sell_b == 3 && narrow && ns > 0 && union_enabled() && tot < (1LL << 29) &&
+        (long long)ctx->n_nodes < (1LL << 27)) {
+        std::vector<int> hu;
+        bool ok = false;
+        BHIP_PROPAGATE(union_widths(ctx, hu, &ok));
+        long long tu = 0;
+        for (int i = 0; i < ns; i++) tu += hu[i];
+        if (ok && (double)tu <= 1.1 * (double)tot && tu < (1LL << 29)) {
+            uni = true;
+            hw.swap(hu);
+        }
+    }
     std::vector<int> hp(ns + 1);
     tot = 0;
     for (int i = 0; i < ns; i++) {
@@ -1031,6 +1307,7 @@ int build_sell(bhip_ctx *ctx) {
     ctx->n_sell = tot;
     ctx->sell_runs = runs;
     ctx->sell_narrow = narrow;
+    ctx->sell_union = uni;
     bhip_invalidate_graphs(ctx);
     BHIP_TRY(ctx, ctx->sell_ptr.resize(ns + 1));
     BHIP_TRY(ctx, hipMemcpyAsync(ctx->sell_ptr.p, hp.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, s));
@@ -1044,8 +1321,17 @@ int build_sell(bhip_ctx *ctx) {
         k_sell_pad<int><<<ns > 0 ? ns : 1, 256, 0, s>>>(ctx->n_nodes, ns, ctx->sell_ptr.p, ctx->sell_col.p);
     BHIP_TRY(ctx, hipMemsetAsync(ctx->bval32.p, 0, ctx->bval32.bytes(), s));
     BHIP_TRY(ctx, hipGetLastError());
+    if (uni) {
+        BHIP_TRY(ctx, ctx->sell_uoff.resize(tot / 64 > 0 ? tot / 64 : 1));
+        BHIP_TRY(ctx, ctx->pair_slot.resize(ctx->n_pairs > 0 ? ctx->n_pairs : 1));
+        k_union_slots<<<grid_for((long long)ns * 64, 256), 256, 0, s>>>(
+            ctx->n_nodes, ns, ctx->nbr_ptr.p, ctx->nbr.p, ctx->sell_ptr.p, nullptr, ctx->sell_uoff.p,
+            ctx->pair_slot.p);
+        BHIP_TRY(ctx, hipGetLastError());
+    }
     // (the B = 1 rows of the table are measured on column runs, the form hex8 diffusion takes)
-    if (!ctx->sell_shape_set) ctx->sell_shape = tuned_default_shape(ctx->sell_b, ctx->n_slices);
+    if (!ctx->sell_shape_set)
+        ctx->sell_shape = uni ? tuned_default_union_shape(ctx->n_slices) : tuned_default_shape(ctx->sell_b, ctx->n_slices);
     ctx->sell_layout_ok = true;
     ctx->sell_cols_ok = false;
     ctx->bval64_layout_ok = false;
@@ -1085,7 +1371,7 @@ void enqueue_to_sell(bhip_ctx *ctx, VT *sval, double *sval64 = nullptr) {
     k_csr_to_sell<BB, CT, VT><<<grid, B, 0, ctx->stream>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, \
                                                            ctx->nbr_off.p, ctx->rowptr.p, ctx->node_rmask.p,          \
                                                            ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, COL, sval, \
-                                                           sval64)
+                                                           sval64, ctx->sell_union ? ctx->pair_slot.p : nullptr)
     if (ctx->sell_narrow) {
         if (nb == 3) BHIP_TO_SELL(3, short, ctx->sell_col16.p);
         else if (nb == 2) BHIP_TO_SELL(2, short, ctx->sell_col16.p);
@@ -1239,7 +1525,7 @@ void graph_key(bhip_ctx *ctx, uint64_t *k) {
     put((uint64_t)ctx->n_slices);
     put((uint64_t)ctx->n_sell);
     put((uint64_t)ctx->sell_b | (uint64_t)ctx->sell_shape << 8 | (uint64_t)ctx->use_bsr << 16 |
-        (uint64_t)ctx->sell_narrow << 17 | (uint64_t)ctx->sell_runs << 18);
+        (uint64_t)ctx->sell_narrow << 17 | (uint64_t)ctx->sell_runs << 18 | (uint64_t)ctx->sell_union << 19);
     ptr(ctx->state.p);
     ptr(ctx->partials.p);
     ptr(ctx->x.p);
@@ -1261,6 +1547,7 @@ void graph_key(bhip_ctx *ctx, uint64_t *k) {
     ptr(ctx->sell_ptr.p);
     ptr(ctx->sell_col.p);
     ptr(ctx->sell_col16.p);
+    ptr(ctx->sell_uoff.p);
     while (i < kGraphKeyWords) put(0);
 }
 

@@ -54,8 +54,11 @@ def test_spmv_shape_table():
     """The tile shapes the tuner chooses from (no GPU needed)."""
     from betanoox_amd import capi
     shapes = capi.spmv_shapes()
-    assert len(shapes) == 18
-    assert [s["id"] for s in shapes] == list(range(18))
+    assert len(shapes) == 21
+    assert [s["id"] for s in shapes] == list(range(21))
     assert shapes[0]["slices_per_block"] == 4 and shapes[0]["unroll"] == 4 and shapes[0]["nontemporal"] == 0
-    triples = {(s["slices_per_block"], s["unroll"], s["nontemporal"]) for s in shapes}
+    triples = {(s["slices_per_block"], s["unroll"], s["nontemporal"]) for s in shapes[:18]}
     assert triples == {(a, b, c) for a in (4, 8, 16) for b in (2, 4, 8) for c in (0, 1)}
+    # 18-20: runs of union slots (unroll 0), streaming loads
+    assert [(s["slices_per_block"], s["unroll"], s["nontemporal"]) for s in shapes[18:]] == [(4, 0, 1), (8, 0, 1),
+                                                                                             (16, 0, 1)]

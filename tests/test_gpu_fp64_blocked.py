@@ -29,7 +29,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-N_SHAPES = 18
+N_SHAPES = 21
 
 
 def _problem(kind):

@@ -27,7 +27,7 @@ from betanoox_amd import capi, meshgen, setup_problem
 
 pytestmark = pytest.mark.gpu
 
-N_SHAPES = 18
+N_SHAPES = 21
 
 
 def _permuted(prob, seed):
