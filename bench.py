@@ -556,8 +556,11 @@ def main():
     _, stream_gbps = ctx.bench_stream(1 << 30, 20)
     _, read_gbps = ctx.bench_read(2 << 30, 10)
     sh = table[chosen]
-    kname = "k_sell_spmv<3, 1, %d, %d, %d, %s, float>" % (sh["slices_per_block"], sh["unroll"], sh["nontemporal"],
-                                                          "short" if sh["column_bytes"] == 2 else "int")
+    if sh["unroll"] == 0:  # runs of union slots
+        kname = "k_sell_spmv_union<1, %d, %d, float>" % (sh["slices_per_block"], sh["nontemporal"])
+    else:
+        kname = "k_sell_spmv<3, 1, %d, %d, %d, %s, float>" % (sh["slices_per_block"], sh["unroll"], sh["nontemporal"],
+                                                              "short" if sh["column_bytes"] == 2 else "int")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.n, kname),
                 "kernel": kname + " (FP32 3x3-block sliced-ELL SpMV with p.Ap partials)",

@@ -1100,7 +1100,12 @@ void launch_sell64_resid(bhip_ctx *ctx, const double *x, double *r, const double
 
 // Algorithmic bytes of one SpMV (values + indices + row pointers + x + y);
 // for the blocked form the stored (padded) blocks are counted.
+// The union runs read no per-slot columns: values, slice pointers, the
+// slices' offset lists (one int per slot position) and the vectors.
 double sell_bytes(bhip_ctx *ctx, int value_bytes) {
+    if (ctx->sell_union && ctx->sell_b == 3 && ctx->sell_shape >= kFirstUnionShape)
+        return (double)ctx->n_sell * sell_vals(ctx) * value_bytes + (ctx->n_slices + 1.0) * 4 +
+               (double)(ctx->n_sell / 64) * 4 + 2.0 * ctx->n_dof * value_bytes;
     return (double)ctx->n_sell * (sell_vals(ctx) * value_bytes + (ctx->sell_narrow ? 2 : 4)) +
            (ctx->n_slices + 1.0) * 4 + 2.0 * ctx->n_dof * value_bytes;
 }

@@ -251,7 +251,9 @@ int bhip_bench_spmv(bhip_ctx *ctx, int fp32, int reps, double *ms_avg, double *b
 typedef struct bhip_spmv_shape {
     int    id;               /* shape id (0: the default shape) */
     int    slices_per_block; /* 64-node slices (one wave each) per workgroup: 4, 8 or 16 */
-    int    unroll;           /* block columns per loop trip */
+    int    unroll;           /* block columns per loop trip; 0: runs of up to three union slots
+                                per trip, x gathered once per run (matrices with three DOFs
+                                per node whose layout has union slots; unroll 4 otherwise) */
     int    nontemporal;      /* 1: streaming loads of block values and columns */
     int    column_bytes;     /* bytes per stored block column of the tuned matrix: 2 (16-bit
                                 offsets from the row's node) or 4 (node indices); 0 from

@@ -7,8 +7,9 @@ consecutive offsets share one gather per lane (the middle value comes from the
 next lane).  Checked here:
   - a structured hex8 cube and an RVE request take union slots: the FP32 SpMV
     bytes equal a restatement of the layout (per 64-node slice, the number of
-    distinct neighbour offsets), and with BHIP_SELL_UNION=0 they equal the
-    rank-slot restatement (the widest row of each slice);
+    distinct neighbour offsets; the union runs read values and the offset
+    lists, no per-slot columns), and with BHIP_SELL_UNION=0 they equal the
+    rank-slot restatement (the widest row of each slice, 16-bit columns);
   - every tile shape, the union runs included, gives the same bits on union
     slots, and a refined solve on union slots equals the solve on rank slots
     (same iteration counts, same values: the slots are summed in the same
@@ -50,8 +51,10 @@ def _assembled(factory, prob):
 
 def _layout_bytes(prob):
     """FP32 SpMV bytes of both slot layouts from the mesh: per 64-node slice,
-    rank slots = its widest row, union slots = its distinct neighbour offsets;
-    36 B of values + a 16-bit offset per slot, slice pointers, x and y."""
+    rank slots = its widest row, union slots = its distinct neighbour offsets.
+    Rank slots: 36 B of values + a 16-bit offset per slot; union runs: the
+    values and one 4-byte offset per slot position of a slice (no per-slot
+    columns); slice pointers, x and y for both."""
     n = prob.n_nodes
     conn = prob.conn.astype(np.int64)
     k = conn.shape[1]
@@ -67,7 +70,7 @@ def _layout_bytes(prob):
     uk = np.unique((a // 64) * span + (b - a + 2 * n))
     union_w = np.bincount(uk // span, minlength=ns)
     tail = (ns + 1) * 4 + 2 * prob.n_dof * 4
-    return int(64 * rank_w.sum()) * 38 + tail, int(64 * union_w.sum()) * 38 + tail
+    return int(64 * rank_w.sum()) * 38 + tail, int(64 * union_w.sum()) * 36 + int(union_w.sum()) * 4 + tail
 
 
 def _solve(ctx, tol=1e-8, precision=capi.PREC_FP32_IR):
