@@ -12,13 +12,17 @@
 //     their coordinates, materials and Gauss-point gradients (and J2
 //     tangents) in LDS and integrates each element once into its 36
 //     canonical 3x3 blocks (one per unordered pair of local nodes, rows of the
-//     lower global node), written to HBM;
-//  2. gather stage (k_asm_pairs): one thread per (node n, neighbour m) entry
-//     of the node adjacency adds the blocks of the elements holding both
-//     nodes in ascending element order -- the reference's serial summation
-//     order, element matrices first -- and writes the block once into the
-//     full symmetric CSR.  The pair (m,n) reads the same canonical blocks and
-//     writes the transpose, so the stored matrix is exactly symmetric.
+//     lower global node), written to HBM pair-major (one local pair of
+//     consecutive elements contiguous);
+//  2. gather stage: for every (node n, neighbour m) entry of the node
+//     adjacency, adds the blocks of the elements holding both nodes in
+//     ascending element order (the reference's serial summation order,
+//     element matrices first) and writes the block once into the full
+//     symmetric CSR (k_asm_pairs, one thread per pair in pair order), and
+//     also into the solver's SELL blocks when that layout is current
+//     (k_asm_rows, lanes on consecutive nodes, one neighbour slot per wave).
+//     The pair (m,n) reads the same canonical blocks and writes the
+//     transpose, so the stored matrix is exactly symmetric.
 // The lift and element-force vectors (k_gather_rhs) and the lift table read
 // the same element blocks.
 #include <algorithm>
@@ -41,7 +45,9 @@ struct AsmArgs {
     // Gauss tables (ctx->gauss): weights, derivatives [ip][xi|eta|zeta][8]
     // (quad4: [ip][xi|eta][4]), shape values [ip][node]
     const double *gw, *gdN, *gN;
-    // element-stage blocks: element e, local pair pr at (e * NP + pr) * NBS
+    // element-stage blocks, pair-major: local pair pr of element e at
+    // (pr * n_el + e) * NBS, so that the blocks of one local pair of
+    // consecutive elements are contiguous
     const double *eblk;
     // local index of each (node, incident element) entry of the incidence lists
     const unsigned char *n2e_loc;
@@ -309,14 +315,26 @@ struct HeatHex8 {
     }
 };
 
+// Gauss-point data of one element in LDS: nip x G doubles padded to a
+// stride of 1 (mod 32) doubles, so that lanes reading the same entry of
+// consecutive elements hit consecutive bank pairs.
+__host__ __device__ __forceinline__ int gs_stride(int nip, int g) {
+    const int n = nip * g;
+    return n + (((1 - n) % 32) + 32) % 32;
+}
+
 // Element stage: workgroup of 256 threads, epw elements.  LDS: coordinates
-// (epw x DIM x NN), material (epw x MS), Gauss-point data (epw x nip x G).
+// (epw x DIM x NN), material (epw x MS), Gauss-point data (epw x gs_stride).
+// The blocks are computed pair by pair across the workgroup's elements (a
+// wave holds a few local pairs of 16 consecutive elements), so the writes of
+// one local pair are contiguous in the pair-major block array.
 template <class P>
 __global__ void __launch_bounds__(256) k_elem_blocks(AsmArgs A, const double *__restrict__ gtab, int epw,
                                                      double *eblk) {
     bind_tables(A, gtab);
     extern __shared__ double sh[];
     const int nip = A.nip;
+    const int gsp = gs_stride(nip, P::G);
     double *xs = sh;
     double *ms = xs + epw * P::DIM * P::NN;
     double *gs = ms + epw * P::MS;
@@ -338,11 +356,11 @@ __global__ void __launch_bounds__(256) k_elem_blocks(AsmArgs A, const double *__
         const int el = t / nip, ip = t - el * nip;
         const long long e = e0 + el;
         if (e < A.n_el)
-            bad |= P::stage_ip(A, (int)e, ip, xs + el * P::DIM * P::NN, gs + (long long)(el * nip + ip) * P::G);
+            bad |= P::stage_ip(A, (int)e, ip, xs + el * P::DIM * P::NN, gs + (long long)el * gsp + ip * P::G);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < epw * P::NP; t += blockDim.x) {
-        const int el = t / P::NP, pr = t - el * P::NP;
+        const int pr = t / epw, el = t - pr * epw;
         const long long e = e0 + el;
         if (e >= A.n_el) continue;
         int i, j;
@@ -353,8 +371,8 @@ __global__ void __launch_bounds__(256) k_elem_blocks(AsmArgs A, const double *__
 #pragma unroll
         for (int q = 0; q < P::NB; q++) K[q] = K2[q] = 0;
         const double *m = ms + el * P::MS;
-        for (int ip = 0; ip < nip; ip++) P::add_ip(m, gs + (long long)(el * nip + ip) * P::G, a, b, K, K2);
-        double *out = eblk + ((long long)e * P::NP + pr) * P::NBS;
+        for (int ip = 0; ip < nip; ip++) P::add_ip(m, gs + (long long)el * gsp + ip * P::G, a, b, K, K2);
+        double *out = eblk + ((long long)pr * A.n_el + e) * P::NBS;
 #pragma unroll
         for (int q = 0; q < P::NB; q++) out[q] = K[q];
         if (P::NBS > P::NB) out[P::NB] = K2[0];
@@ -396,7 +414,7 @@ __device__ __forceinline__ void pair_block(const AsmArgs &A, const int *n2e_ptr,
             if (b >= 0) {
                 const int la_ = la[a];
                 const int pr = la_ <= b ? pair_index(P::NN, la_, b) : pair_index(P::NN, b, la_);
-                P::load(A.eblk + ((long long)el[a] * P::NP + pr) * P::NBS, K, K2);
+                P::load(A.eblk + ((long long)pr * A.n_el + el[a]) * P::NBS, K, K2);
             }
         }
         i = ie;  // done
@@ -410,7 +428,7 @@ __device__ __forceinline__ void pair_block(const AsmArgs &A, const int *n2e_ptr,
         } else {
             const int a = A.n2e_loc[i], b = A.n2e_loc[j];
             const int pr = a <= b ? pair_index(P::NN, a, b) : pair_index(P::NN, b, a);
-            P::load(A.eblk + ((long long)el * P::NP + pr) * P::NBS, K, K2);
+            P::load(A.eblk + ((long long)pr * A.n_el + el) * P::NBS, K, K2);
             i++;
             j++;
         }
@@ -446,25 +464,21 @@ struct AsmOut {
     float *sval32;
     double *sval64;
     const int *sptr, *nbr_ptr;
-    const unsigned char *pair_slot;  // union slots (nullptr: slot = the pair's rank in its row)
     long long nsell;
 };
 
+// One node pair (n, m), pair index p: the canonical block summed over the
+// elements holding both nodes in ascending element order, written into the
+// CSR row(s) of n and, when dst >= 0, into the SELL block values at dst.
 template <class P>
-__global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const int *n2e_ptr,
-                                                   const int *n2e, const double *__restrict__ gtab) {
-    bind_tables(A, gtab);
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= O.n_pairs) return;
-    int n = O.pair_row[p], m = O.nbr[p];
+__device__ __forceinline__ void asm_pair(const AsmArgs &A, const AsmOut &O, const int *n2e_ptr, const int *n2e,
+                                         long long p, int n, int m, long long dst) {
     int lo = n < m ? n : m, hi = n < m ? m : n;
     double K[P::NB], K2[P::NB];
     pair_block<P>(A, n2e_ptr, n2e, n, lo, hi, K, K2);
     const bool nlo = (n == lo);
     unsigned rn = O.rmask[n], rm = O.rmask[m];
-    if (O.mode == 0 && O.sval32) {
-        const long long slot = O.pair_slot ? (long long)O.pair_slot[p] : p - O.nbr_ptr[n];
-        const long long dst = (long long)O.sptr[n / 64] + 64LL * slot + (n % 64);
+    if (dst >= 0) {
 #pragma unroll
         for (int c = 0; c < P::ND; c++)
 #pragma unroll
@@ -498,6 +512,60 @@ __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const in
             r++;
         }
     }
+}
+
+// Gather stage in pair order (no current SELL layout: the first assembly, and
+// heat): one thread per node pair, consecutive threads on consecutive
+// neighbours of a node, so the CSR writes of a wave are contiguous.
+template <class P>
+__global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const int *n2e_ptr, const int *n2e,
+                                                   const double *__restrict__ gtab) {
+    bind_tables(A, gtab);
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= O.n_pairs) return;
+    asm_pair<P>(A, O, n2e_ptr, n2e, p, O.pair_row[p], O.nbr[p], -1);
+}
+
+// Gather stage in row order, when the solver's SELL layout is current (every
+// reassembly after the first solve): a workgroup takes four slots of one
+// 64-node slice (blockIdx.y selects them), wave w slot 4 blockIdx.y + w, lane
+// l the slice's node 64 s + l.  In a slot the lanes hold consecutive nodes
+// and, away from mesh boundaries, the same relative neighbour, so they read
+// the blocks of consecutive elements, which are contiguous in the pair-major
+// block array, and write consecutive SELL entries.  Slots are those of the
+// layout: its union offsets, or the node's k-th neighbour.
+template <class P>
+__global__ void __launch_bounds__(256) k_asm_rows(AsmArgs A, AsmOut O, const int *n2e_ptr, const int *n2e,
+                                                  const double *__restrict__ gtab, int n_nodes, const int *uoff) {
+    bind_tables(A, gtab);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long s = blockIdx.x;
+    const int j = 4 * blockIdx.y + wave;
+    const int W = (O.sptr[s + 1] - O.sptr[s]) >> 6;
+    if (j >= W) return;
+    const long long n = s * 64 + lane;
+    if (n >= n_nodes) return;
+    const int pb = O.nbr_ptr[n], pe = O.nbr_ptr[n + 1];
+    long long p = -1;
+    int m = -1;
+    if (uoff) {
+        const int mm = (int)n + uoff[O.sptr[s] / 64 + j];
+        int lo = pb, hi = pe;  // first entry >= mm
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (O.nbr[mid] < mm) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < pe && O.nbr[lo] == mm) {
+            p = lo;
+            m = mm;
+        }
+    } else if (pb + j < pe) {
+        p = pb + j;
+        m = O.nbr[p];
+    }
+    if (p >= 0) asm_pair<P>(A, O, n2e_ptr, n2e, p, (int)n, m, (long long)O.sptr[s] + 64LL * j + lane);
 }
 
 // Per node: pseudo[I] for every free DOF I of the node =
@@ -565,13 +633,13 @@ int upload_gauss(bhip_ctx *ctx, int nnpe, int order, int *nip_out) {
 template <class P>
 int elem_stage_epw(int nip) {
     // at most 16 elements, one thread per Gauss point, and 60 KiB of LDS
-    const size_t per_el = sizeof(double) * ((size_t)P::DIM * P::NN + P::MS + (size_t)nip * P::G);
+    const size_t per_el = sizeof(double) * ((size_t)P::DIM * P::NN + P::MS + (size_t)gs_stride(nip, P::G));
     const int by_lds = (int)((60 * 1024) / per_el);
     return std::max(1, std::min(std::min(16, 256 / std::max(nip, 1)), by_lds));
 }
 template <class P>
 size_t elem_stage_lds(int epw, int nip) {
-    return sizeof(double) * ((size_t)epw * P::DIM * P::NN + (size_t)epw * P::MS + (size_t)epw * nip * P::G);
+    return sizeof(double) * ((size_t)epw * P::DIM * P::NN + (size_t)epw * P::MS + (size_t)epw * gs_stride(nip, P::G));
 }
 
 // Element stage into ctx->eblk (kept until the next assembly: the lift table
@@ -594,7 +662,14 @@ int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, d
     const int B = 256;
     hipStream_t s = ctx->stream;
     BHIP_PROPAGATE(element_stage<P>(ctx, A));
-    k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p);
+    if (O.mode == 0 && O.sval32) {
+        const int *uoff = ctx->sell_union ? ctx->sell_uoff.p : nullptr;
+        const dim3 grid((unsigned)ctx->n_slices, (unsigned)((ctx->sell_wmax + 3) / 4));
+        if (ctx->n_slices > 0 && ctx->sell_wmax > 0)
+            k_asm_rows<P><<<grid, B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p, ctx->n_nodes, uoff);
+    } else {
+        k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p);
+    }
     BHIP_TRY(ctx, hipGetLastError());
     k_restrained_diag<<<grid_for(ctx->n_dof, B), B, 0, s>>>(ctx->n_dof, ctx->rest.p, ctx->rowptr.p,
                                                             ctx->val.p, O.mode == 1 ? ctx->val_aux.p : nullptr);
@@ -657,7 +732,6 @@ int bhip_assemble_impl(bhip_ctx *ctx, int what, const double *coef, const double
     O.sval64 = nullptr;
     O.sptr = ctx->sell_ptr.p;
     O.nbr_ptr = ctx->nbr_ptr.p;
-    O.pair_slot = ctx->sell_union ? ctx->pair_slot.p : nullptr;
     O.nsell = ctx->n_sell;
     const bool to_sell = bhip_sell_values_direct(ctx) && (what == BHIP_ASM_STIFFNESS || what == BHIP_ASM_PLASTIC_TANGENT);
     if (to_sell) {

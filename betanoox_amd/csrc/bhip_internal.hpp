@@ -139,6 +139,7 @@ struct bhip_ctx {
     // w_s block columns; entry (k, lane) of slice s is at sell_ptr[s] + 64k + lane
     int n_slices = 0;
     long long n_sell = 0;    // padded block count
+    int sell_wmax = 0;       // widest slice (block columns)
     dvec<int> sell_ptr, sell_col;  // slice starts; block columns (node index)
     dvec<short> sell_col16;        // block columns as offsets from the row's node (narrow form)
     bool sell_narrow = false;      // 16-bit offsets in use

@@ -1309,6 +1309,8 @@ int build_sell(bhip_ctx *ctx) {
     }
     if (tot >= (1LL << 31)) return bhip_fail(ctx, BHIP_ERR_CAPACITY, "blocked matrix too large for int32 offsets");
     hp[ns] = (int)tot;
+    ctx->sell_wmax = 0;
+    for (int i = 0; i < ns; i++) ctx->sell_wmax = std::max(ctx->sell_wmax, hw[i] / 64);
     ctx->n_sell = tot;
     ctx->sell_runs = runs;
     ctx->sell_narrow = narrow;

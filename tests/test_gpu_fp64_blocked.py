@@ -179,8 +179,12 @@ def test_bench_bytes(hip_ctx_factory):
     ms64, by64 = ctx.bench_spmv(blocked64=True, reps=3)
     assert ms32 > 0 and ms64 > 0
     n_dof = ctx.n_dof
-    # per padded block: 36 B of FP32 values or 72 B of FP64 values, same
-    # 2 B column offset; x and y: 4 or 8 B per DOF
-    n_sell = (by32 - 8.0 * n_dof - 4 * ((n_dof // 3 + 63) // 64 + 1)) / 38.0
-    assert n_sell == int(n_sell) and n_sell >= n_dof // 3
-    assert by64 == pytest.approx(n_sell * 74 + 4 * ((n_dof // 3 + 63) // 64 + 1) + 16.0 * n_dof, rel=1e-12)
+    # the hex mesh has union slots and runs in the union shape: per padded
+    # block 36 B of FP32 values or 72 B of FP64 values and no column, one
+    # 4-byte offset per slot position of a slice; slice pointers; x and y:
+    # 4 or 8 B per DOF
+    ptrs = 4 * ((n_dof // 3 + 63) // 64 + 1)
+    n_sell = (by32 - 8.0 * n_dof - ptrs) / (36 + 4 / 64)
+    assert abs(n_sell - round(n_sell)) < 1e-6 and round(n_sell) % 64 == 0 and n_sell >= n_dof // 3
+    n_sell = round(n_sell)
+    assert by64 == pytest.approx(n_sell * 72 + (n_sell // 64) * 4 + ptrs + 16.0 * n_dof, rel=1e-12)
