@@ -528,20 +528,25 @@ __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const in
 
 // Gather stage in row order, when the solver's SELL layout is current (every
 // reassembly after the first solve): a workgroup takes four slots of one
-// 64-node slice (blockIdx.y selects them), wave w slot 4 blockIdx.y + w, lane
-// l the slice's node 64 s + l.  In a slot the lanes hold consecutive nodes
-// and, away from mesh boundaries, the same relative neighbour, so they read
-// the blocks of consecutive elements, which are contiguous in the pair-major
-// block array, and write consecutive SELL entries.  Slots are those of the
-// layout: its union offsets, or the node's k-th neighbour.
+// 64-node slice, wave w slot 4 g + w for the workgroup's group g, lane l the
+// slice's node 64 s + l.  In a slot the lanes hold consecutive nodes and,
+// away from mesh boundaries, the same relative neighbour, so they read the
+// blocks of consecutive elements, which are contiguous in the pair-major
+// block array, and write consecutive SELL entries.  The groups of one slice
+// are consecutive workgroups, so the slice's CSR rows (a contiguous range,
+// each row written in 3-entry pieces by its slots) fill in L2 at about the
+// same time.  Slots are those of the layout: its union offsets, or the
+// node's k-th neighbour.
 template <class P>
 __global__ void __launch_bounds__(256) k_asm_rows(AsmArgs A, AsmOut O, const int *n2e_ptr, const int *n2e,
-                                                  const double *__restrict__ gtab, int n_nodes, const int *uoff) {
+                                                  const double *__restrict__ gtab, int n_nodes, const int *uoff,
+                                                  int groups, long long slice0) {
     bind_tables(A, gtab);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long long s = blockIdx.x;
-    const int j = 4 * blockIdx.y + wave;
+    const unsigned sl = blockIdx.x / (unsigned)groups;
+    const int j = 4 * (int)(blockIdx.x - sl * (unsigned)groups) + wave;
+    const long long s = slice0 + sl;
     const int W = (O.sptr[s + 1] - O.sptr[s]) >> 6;
     if (j >= W) return;
     const long long n = s * 64 + lane;
@@ -664,9 +669,14 @@ int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, d
     BHIP_PROPAGATE(element_stage<P>(ctx, A));
     if (O.mode == 0 && O.sval32) {
         const int *uoff = ctx->sell_union ? ctx->sell_uoff.p : nullptr;
-        const dim3 grid((unsigned)ctx->n_slices, (unsigned)((ctx->sell_wmax + 3) / 4));
-        if (ctx->n_slices > 0 && ctx->sell_wmax > 0)
-            k_asm_rows<P><<<grid, B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p, ctx->n_nodes, uoff);
+        const int groups = (ctx->sell_wmax + 3) / 4;  // four slots of the widest slice per workgroup
+        // launches of at most 2^22 workgroups (the grid's x extent in threads stays below 2^32)
+        const long long per_launch = std::max(1LL, (1LL << 22) / groups);
+        for (long long s0 = 0; s0 < ctx->n_slices && ctx->sell_wmax > 0; s0 += per_launch) {
+            const long long cnt = std::min(per_launch, (long long)ctx->n_slices - s0);
+            k_asm_rows<P><<<(unsigned)(cnt * groups), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p,
+                                                               ctx->n_nodes, uoff, groups, s0);
+        }
     } else {
         k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p);
     }

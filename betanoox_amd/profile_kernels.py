@@ -9,7 +9,7 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   reps x FP64 CSR SpMV                    (k_csr_spmv<8,double,double,1>)
   reps x FP64 SELL-64 3x3-block SpMV      (k_sell_spmv<3, 1, ..., double>, BHIP_PREC_FP64_BLOCKED)
   one FP32-inner + FP64-refined solve     (unless --no-solve)
-  one reassembly into the solver's layout (unless --no-solve)
+  one reassembly into the solver's layout (k_elem_blocks, k_asm_rows)
 --workload cases (RVE request, --n 64, six load cases):
   one six-case solve in the packed solver (k_pk_spmv<6, ...>, k_pk_update<6>, k_pk_pupdate<6>)
 --workload heat (diffusion, --n 128, 1 DOF per node):
@@ -18,7 +18,7 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   10 backward-Euler steps                 (unless --no-solve)
 --workload plastic (J2 bar, --n 48, the first --increments of the
   20-increment path to 0.5% strain): per Newton iteration the FP64 return map
-  (k_plastic_update), the tangent assembly (k_elem_blocks / k_asm_pairs of
+  (k_plastic_update), the tangent assembly (k_elem_blocks, then k_asm_pairs or k_asm_rows of
   ElasHex8<J2>), the device residual (k_plastic_fe, k_plastic_residual) and an
   FP32-inner refined solve to 1e-8
 --workload served (RVE requests): --reps requests of the seeded probe draw
@@ -89,9 +89,10 @@ def main(argv=None):
             if not a.no_solve:
                 s = ctx.solve(tol=1e-8, precision=capi.PREC_FP32_IR, inner_tol=1e-3, maxit=40000)
                 out["solve_fp32_ir"] = {k: s[k] for k in ("iters", "outer_iters", "rel_resid", "ms_total")}
-                # a reassembly on the current blocked layout: the gather stage
-                # writes the FP32 and FP64 SELL values itself (no conversion)
-                ctx.assemble()
+            # a reassembly on the current blocked layout (built by the SpMV
+            # runs above): the gather stage writes the FP32 and FP64 SELL
+            # values itself (k_asm_rows, no conversion pass)
+            ctx.assemble()
         elif a.workload == "cases":
             n = a.n or 64
             prob = meshgen.rve_request(n, 7)
