@@ -532,21 +532,25 @@ __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const in
 // slice's node 64 s + l.  In a slot the lanes hold consecutive nodes and,
 // away from mesh boundaries, the same relative neighbour, so they read the
 // blocks of consecutive elements, which are contiguous in the pair-major
-// block array, and write consecutive SELL entries.  The groups of one slice
-// are consecutive workgroups, so the slice's CSR rows (a contiguous range,
-// each row written in 3-entry pieces by its slots) fill in L2 at about the
-// same time.  Slots are those of the layout: its union offsets, or the
-// node's k-th neighbour.
+// block array, and write consecutive SELL entries.  Workgroups go to the
+// eight XCDs in turn (workgroup b to XCD b mod 8), so the groups of one slice
+// are given workgroup numbers of one residue mod 8, consecutive among those:
+// the slice's CSR rows (a contiguous range, each row written in 3-entry
+// pieces by its slots) then fill in one XCD's L2 at about the same time and
+// leave it as whole lines.  Slots are those of the layout: its union
+// offsets, or the node's k-th neighbour.
 template <class P>
 __global__ void __launch_bounds__(256) k_asm_rows(AsmArgs A, AsmOut O, const int *n2e_ptr, const int *n2e,
                                                   const double *__restrict__ gtab, int n_nodes, const int *uoff,
-                                                  int groups, long long slice0) {
+                                                  int groups, long long slice0, long long slice_end) {
     bind_tables(A, gtab);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const unsigned sl = blockIdx.x / (unsigned)groups;
-    const int j = 4 * (int)(blockIdx.x - sl * (unsigned)groups) + wave;
-    const long long s = slice0 + sl;
+    const unsigned xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const unsigned q = k / (unsigned)groups;
+    const int j = 4 * (int)(k - q * (unsigned)groups) + wave;
+    const long long s = slice0 + 8LL * q + xcd;
+    if (s >= slice_end) return;
     const int W = (O.sptr[s + 1] - O.sptr[s]) >> 6;
     if (j >= W) return;
     const long long n = s * 64 + lane;
@@ -670,12 +674,14 @@ int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, d
     if (O.mode == 0 && O.sval32) {
         const int *uoff = ctx->sell_union ? ctx->sell_uoff.p : nullptr;
         const int groups = (ctx->sell_wmax + 3) / 4;  // four slots of the widest slice per workgroup
-        // launches of at most 2^22 workgroups (the grid's x extent in threads stays below 2^32)
-        const long long per_launch = std::max(1LL, (1LL << 22) / groups);
+        // launches of at most 2^22 workgroups (the grid's x extent in threads
+        // stays below 2^32), each over a multiple of 8 slices but the last
+        const long long per_launch = std::max(8LL, ((1LL << 22) / groups) & ~7LL);
         for (long long s0 = 0; s0 < ctx->n_slices && ctx->sell_wmax > 0; s0 += per_launch) {
             const long long cnt = std::min(per_launch, (long long)ctx->n_slices - s0);
-            k_asm_rows<P><<<(unsigned)(cnt * groups), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p,
-                                                               ctx->n_nodes, uoff, groups, s0);
+            const long long blocks = (cnt + 7) / 8 * 8 * groups;
+            k_asm_rows<P><<<(unsigned)blocks, B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p,
+                                                        ctx->n_nodes, uoff, groups, s0, s0 + cnt);
         }
     } else {
         k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p);
