@@ -25,8 +25,9 @@ can be attributed per kernel, and prints the HIP-event timings of each group.
   (n = 16..96, six load cases each) queued at once on a 4-slot server
   (pack 8), FP32-inner refined to 1e-8
 --workload plastic-kernels (J2 bar, --n 64): the per-Newton-iteration kernels
-  without the solve: reps x (return map with the algorithmic tangent from a
-  plastic trial increment, tangent assembly, device residual)
+  without the solve, on the solver's blocked layout: reps x (return map with
+  the algorithmic tangent from a plastic trial increment, tangent assembly
+  into the blocked layout, device residual)
 
 The SELL SpMV runs in the shape the library's tuned table gives the matrix,
 or in --shape ID (see bhip_tune_spmv); the shape is printed.
@@ -125,6 +126,12 @@ def main(argv=None):
             ctx.plastic_init()
             ctx.plastic_set_tangent(capi.TANGENT_ALGORITHMIC)
             ctx.set_load(prob.load)
+            # the solver's blocked layout, as after the first solve of a
+            # Newton loop: the tangent assemblies below then gather straight
+            # into its FP32 and FP64 blocks (k_asm_rows)
+            ctx.assemble()
+            ctx.bench_spmv(fp32=True, reps=1)
+            ctx.bench_spmv(blocked64=True, reps=1)
             # a uniform axial strain of 0.3 % with lateral contraction: every
             # Gauss point takes the plastic branch
             du = np.zeros(prob.n_dof)
