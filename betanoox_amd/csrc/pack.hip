@@ -48,7 +48,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxR = 8;
 constexpr int kChunk = 32;             // iterations per captured graph (even: r.z parity)
-constexpr int kSplitSlices = 2048;     // systems with fewer slices run four waves per slice
+constexpr int kSplitSlices = 1024;     // systems with fewer slices run four waves per slice
 constexpr double kStreamBytes = 240e6;
 constexpr int kPackUnroll = 2;         // SpMV block columns per loop trip (plain-load packs)
 constexpr int kPackUnrollStream = 3;   // the same for streaming packs (large systems)
