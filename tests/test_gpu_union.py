@@ -19,7 +19,9 @@ next lane).  Checked here:
     slice) equal the rank-slot solve; small systems (four waves share a slice,
     taking every fourth run) reach the FP64 solution to the solve tolerance;
   - a randomly renumbered mesh, whose slices have wide unions, keeps rank slots
-    (same bytes with and without the switch) and solves to the same answer.
+    (same bytes with and without the switch) and solves to the same answer;
+  - a reassembly gathers straight into the current layout's blocks (union or
+    rank slots) with the bits of a fresh context's conversion.
 """
 import dataclasses
 
@@ -170,3 +172,36 @@ def test_renumbered_mesh_keeps_rank_slots(hip_ctx_factory, monkeypatch):
     st_0, x_0 = _solve(c0, tol=1e-10)
     err = np.abs(x_q[dof_new] - x_0).max() / np.abs(x_0).max()
     assert err < 1e-7, err
+
+
+@pytest.mark.parametrize("kind", ["cube13", "rve12", "renumbered12"])
+def test_reassembly_into_the_layout(hip_ctx_factory, kind):
+    """A reassembly on a current layout gathers straight into its FP32 and
+    FP64 blocks, slot by slot (union slots: each slot's offset looked up in
+    the node's neighbour list; rank slots on the renumbered mesh: the node's
+    k-th neighbour).  Solves after it equal, bit for bit, the solves of a
+    fresh context, whose first assembly writes the CSR and converts it."""
+    if kind == "cube13":
+        a = meshgen.elastic_cube(13)
+    elif kind == "rve12":
+        a = meshgen.rve_request(12, 3)
+    else:
+        a, _ = _permuted(meshgen.elastic_cube(12), 4)
+    b = dataclasses.replace(a, mat=a.mat.copy())
+    b.mat[0, 0] *= 3.0  # the first material group's modulus
+    print("case reassembly", kind, a.n_dof, flush=True)
+    ctx, _ = _assembled(hip_ctx_factory, a)
+    assert (ctx.spmv_shape() == 18) == (kind != "renumbered12")
+    _, xa = _solve(ctx)
+    _, xa64 = _solve(ctx, tol=1e-10, precision=capi.PREC_FP64_BLOCKED)
+    ctx.set_materials(b.mat_kind, b.mat, b.extra.get("group"))
+    ctx.assemble(capi.ASM_STIFFNESS)
+    ctx.build_rhs()
+    _, xb = _solve(ctx)
+    _, xb64 = _solve(ctx, tol=1e-10, precision=capi.PREC_FP64_BLOCKED)
+    fb, _ = _assembled(hip_ctx_factory, b)
+    _, fxb = _solve(fb)
+    _, fxb64 = _solve(fb, tol=1e-10, precision=capi.PREC_FP64_BLOCKED)
+    assert np.array_equal(xb, fxb)
+    assert np.array_equal(xb64, fxb64)
+    assert not np.array_equal(xa, xb) and not np.array_equal(xa64, xb64)
