@@ -537,12 +537,14 @@ __global__ void __launch_bounds__(256) k_asm_pairs(AsmArgs A, AsmOut O, const in
 // are given workgroup numbers of one residue mod 8, consecutive among those:
 // the slice's CSR rows (a contiguous range, each row written in 3-entry
 // pieces by its slots) then fill in one XCD's L2 at about the same time and
-// leave it as whole lines.  Slots are those of the layout: its union
-// offsets, or the node's k-th neighbour.
+// leave it as whole lines.  Slots are those of the layout: union slots,
+// whose neighbour's rank in the node's list slot_rank holds (255: a zero
+// block, left as it is), or the node's k-th neighbour.
 template <class P>
 __global__ void __launch_bounds__(256) k_asm_rows(AsmArgs A, AsmOut O, const int *n2e_ptr, const int *n2e,
-                                                  const double *__restrict__ gtab, int n_nodes, const int *uoff,
-                                                  int groups, long long slice0, long long slice_end) {
+                                                  const double *__restrict__ gtab, int n_nodes,
+                                                  const unsigned char *slot_rank, int groups, long long slice0,
+                                                  long long slice_end) {
     bind_tables(A, gtab);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -555,26 +557,21 @@ __global__ void __launch_bounds__(256) k_asm_rows(AsmArgs A, AsmOut O, const int
     if (j >= W) return;
     const long long n = s * 64 + lane;
     if (n >= n_nodes) return;
+    const long long dst = (long long)O.sptr[s] + 64LL * j + lane;
     const int pb = O.nbr_ptr[n], pe = O.nbr_ptr[n + 1];
     long long p = -1;
     int m = -1;
-    if (uoff) {
-        const int mm = (int)n + uoff[O.sptr[s] / 64 + j];
-        int lo = pb, hi = pe;  // first entry >= mm
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (O.nbr[mid] < mm) lo = mid + 1;
-            else hi = mid;
-        }
-        if (lo < pe && O.nbr[lo] == mm) {
-            p = lo;
-            m = mm;
+    if (slot_rank) {
+        const int rk = slot_rank[dst];
+        if (rk != 255) {
+            p = pb + rk;
+            m = O.nbr[p];
         }
     } else if (pb + j < pe) {
         p = pb + j;
         m = O.nbr[p];
     }
-    if (p >= 0) asm_pair<P>(A, O, n2e_ptr, n2e, p, (int)n, m, (long long)O.sptr[s] + 64LL * j + lane);
+    if (p >= 0) asm_pair<P>(A, O, n2e_ptr, n2e, p, (int)n, m, dst);
 }
 
 // Per node: pseudo[I] for every free DOF I of the node =
@@ -672,7 +669,7 @@ int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, d
     hipStream_t s = ctx->stream;
     BHIP_PROPAGATE(element_stage<P>(ctx, A));
     if (O.mode == 0 && O.sval32) {
-        const int *uoff = ctx->sell_union ? ctx->sell_uoff.p : nullptr;
+        const unsigned char *srank = ctx->sell_union ? ctx->slot_rank.p : nullptr;
         const int groups = (ctx->sell_wmax + 3) / 4;  // four slots of the widest slice per workgroup
         // launches of at most 2^22 workgroups (the grid's x extent in threads
         // stays below 2^32), each over a multiple of 8 slices but the last
@@ -681,7 +678,7 @@ int launch_assembly(bhip_ctx *ctx, AsmArgs A, AsmOut O, const double *u_state, d
             const long long cnt = std::min(per_launch, (long long)ctx->n_slices - s0);
             const long long blocks = (cnt + 7) / 8 * 8 * groups;
             k_asm_rows<P><<<(unsigned)blocks, B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p,
-                                                        ctx->n_nodes, uoff, groups, s0, s0 + cnt);
+                                                        ctx->n_nodes, srank, groups, s0, s0 + cnt);
         }
     } else {
         k_asm_pairs<P><<<grid_for(O.n_pairs, B), B, 0, s>>>(A, O, ctx->n2e_ptr.p, ctx->n2e.p, ctx->gauss.p);

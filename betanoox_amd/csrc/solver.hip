@@ -764,13 +764,15 @@ __global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, CT *scol)
 // neighbour list; each step takes the smallest offset (m - n) still at the
 // head of any lane's list as the next slot, and the lanes holding it advance.
 // Without uoff (first pass) the slice width 64 |union| goes to width[s];
-// with it, the offsets go to uoff[sptr[s] / 64 + u] and the slot of every
-// node pair to pair_slot.  A union above kMaxUnionSlots stops early and
-// reports kMaxUnionSlots + 1 slots (the caller then keeps rank slots).
+// with it, the offsets go to uoff[sptr[s] / 64 + u], the slot of every
+// node pair to pair_slot and, for every slot and lane, the rank of its pair
+// in the node's list (255: none) to slot_rank.  A union above kMaxUnionSlots
+// stops early and reports kMaxUnionSlots + 1 slots (the caller then keeps
+// rank slots), so ranks stay below 255.
 constexpr int kMaxUnionSlots = 255;
 __global__ void __launch_bounds__(256) k_union_slots(int n_nodes, int n_slices, const int *nbr_ptr, const int *nbr,
                                                      const int *sptr, int *width, int *uoff,
-                                                     unsigned char *pair_slot) {
+                                                     unsigned char *pair_slot, unsigned char *slot_rank) {
     const int lane = threadIdx.x & 63;
     const long long s = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (s >= n_slices) return;  // whole waves leave together
@@ -782,6 +784,7 @@ __global__ void __launch_bounds__(256) k_union_slots(int n_nodes, int n_slices, 
     }
     constexpr int kNone = 0x7fffffff;
     int head = p < pe ? nbr[p] - (int)n : kNone;
+    const int p0 = p;
     const long long ub = uoff ? sptr[s] / 64 : 0;
     int u = 0;
     for (;;) {
@@ -796,6 +799,7 @@ __global__ void __launch_bounds__(256) k_union_slots(int n_nodes, int n_slices, 
         if (uoff) {
             if (lane == 0) uoff[ub + u] = mn;
             if (head == mn) pair_slot[p] = (unsigned char)u;
+            slot_rank[64 * (ub + u) + lane] = head == mn ? (unsigned char)(p - p0) : (unsigned char)255;
         }
         if (head == mn) {
             p++;
@@ -1245,7 +1249,7 @@ int union_widths(bhip_ctx *ctx, std::vector<int> &hw, bool *ok) {
     BHIP_TRY(ctx, width.resize(ns + 1));
     BHIP_TRY(ctx, hipMemsetAsync(width.p, 0, sizeof(int) * (ns + 1), s));
     k_union_slots<<<grid_for((long long)ns * 64, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, ctx->nbr.p,
-                                                                    nullptr, width.p, nullptr, nullptr);
+                                                                    nullptr, width.p, nullptr, nullptr, nullptr);
     BHIP_TRY(ctx, hipGetLastError());
     hw.assign(ns + 1, 0);
     BHIP_TRY(ctx, hipMemcpyAsync(hw.data(), width.p, sizeof(int) * (ns + 1), hipMemcpyDeviceToHost, s));
@@ -1331,9 +1335,10 @@ int build_sell(bhip_ctx *ctx) {
     if (uni) {
         BHIP_TRY(ctx, ctx->sell_uoff.resize(tot / 64 > 0 ? tot / 64 : 1));
         BHIP_TRY(ctx, ctx->pair_slot.resize(ctx->n_pairs > 0 ? ctx->n_pairs : 1));
+        BHIP_TRY(ctx, ctx->slot_rank.resize(tot > 0 ? tot : 1));
         k_union_slots<<<grid_for((long long)ns * 64, 256), 256, 0, s>>>(
             ctx->n_nodes, ns, ctx->nbr_ptr.p, ctx->nbr.p, ctx->sell_ptr.p, nullptr, ctx->sell_uoff.p,
-            ctx->pair_slot.p);
+            ctx->pair_slot.p, ctx->slot_rank.p);
         BHIP_TRY(ctx, hipGetLastError());
     }
     // (the B = 1 rows of the table are measured on column runs, the form hex8 diffusion takes)
