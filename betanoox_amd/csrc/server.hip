@@ -121,8 +121,8 @@ bool topo_same(const Seat &s, const Request &r) {
     return s.have_topo && s.n_nodes == r.n_nodes && s.n_el == r.n_el && s.conn == r.conn && s.rest == r.rest;
 }
 
-// Requests that go into packs: fewer node slices than the packed solver's
-// split threshold, at most 8 cases (one case group).
+// Requests that go into packs: fewer than 2048 node slices, at most 8 cases
+// (one case group).
 constexpr long long kPackMaxNodes = 2048LL * 64;
 bool packable(const bhip_server *srv, const Request &r) {
     return srv->opts.max_pack > 1 && srv->opts.precision == BHIP_PREC_FP32_IR && r.n_cases <= 8 &&
