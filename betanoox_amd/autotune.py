@@ -76,7 +76,7 @@ def tune_server(probe, slots_list=(2, 4, 8), pack_list=(1, 4, 8, 16), reps=1):
     requests for every (slots, max_pack); returns (best, table), best =
     {"slots", "max_pack", "req_per_s"}.  A warm-up run on the first
     combination comes first (kernel loading, host allocations).  Capacities
-    within 2% of the best tie; the tie goes to the fewest slots."""
+    within 2% of the best tie; the tie goes to the slot count nearest 4."""
     closed_loop(probe[: max(2, len(probe) // 4)], slots_list[0], pack_list[0])
     table = []
     for slots in slots_list:
@@ -85,11 +85,14 @@ def tune_server(probe, slots_list=(2, 4, 8), pack_list=(1, 4, 8, 16), reps=1):
             table.append({"slots": slots, "max_pack": mp, "req_per_s": len(probe) / best_t})
             log("server tuning: slots %d, max_pack %d: %.2f req/s" % (slots, mp, table[-1]["req_per_s"]))
     # capacities within 2% count as equal (run-to-run spread of a 32-request
-    # probe); among them the fewest slots, then the smallest pack, win: fewer
-    # requests share the GPU, so each finishes sooner (lower tail latency)
+    # probe); among them the slot count nearest 4, then the fewer slots, then
+    # the smallest pack win.  Measured on the seeded RVE stream at 70% of
+    # capacity (pack 1): 2 slots p50 48.7 ms / p99 636 ms (a large request
+    # holds one of the two workers and small ones queue behind it), 4 slots
+    # 40.9 / 565 ms, 8 slots 43.2 / 693 ms (more requests share the GPU).
     top = max(r["req_per_s"] for r in table)
     near = [r for r in table if r["req_per_s"] >= 0.98 * top]
-    best = min(near, key=lambda r: (r["slots"], r["max_pack"]))
+    best = min(near, key=lambda r: (abs(math.log2(r["slots"] / 4.0)), r["slots"], r["max_pack"]))
     return best, table
 
 
