@@ -488,8 +488,10 @@ __device__ __forceinline__ void asm_pair(const AsmArgs &A, const AsmOut &O, cons
                 if (!rc && !rd) v = oriented<P::ND>(K, nlo, c, d);
                 else if (rc && n == m && c == d) v = 1;
                 const long long o = (long long)(c * P::ND + d) * O.nsell + dst;
-                O.sval32[o] = (float)v;
-                if (O.sval64) O.sval64[o] = v;
+                // streaming stores: whole lines per wave, not read again
+                // before the solve, so they leave L2 to the CSR pieces
+                __builtin_nontemporal_store((float)v, O.sval32 + o);
+                if (O.sval64) __builtin_nontemporal_store(v, O.sval64 + o);
             }
     }
 #pragma unroll
