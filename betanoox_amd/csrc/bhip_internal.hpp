@@ -152,14 +152,12 @@ struct bhip_ctx {
     // union of the slice's neighbour offsets (zero where the lane's node has
     // no such neighbour, its column then the node itself), so slot order is
     // column order in every row.  sell_uoff[sell_ptr[s] / 64 + u] is that
-    // offset; pair_slot[p] is the slot of node pair p, and slot_rank (one
-    // byte per slot) the inverse: the rank of the slot's neighbour in its
-    // node's neighbour list, 255 for a zero block.  Runs of consecutive
-    // offsets let the SpMV gather x once per run and take the rest from the
-    // neighbouring lanes.
+    // offset; slot_rank (one byte per slot) holds the rank of the slot's
+    // neighbour in its node's neighbour list, 255 for a zero block.  Runs of
+    // consecutive offsets let the SpMV gather x once per run and take the
+    // rest from the neighbouring lanes.
     bool sell_union = false;
     dvec<int> sell_uoff;
-    dvec<unsigned char> pair_slot;
     dvec<unsigned char> slot_rank;
     dvec<float> bval32;      // SELL block values, SoA: sell_vals(ctx) x n_sell
     // ---- FP64 values in the same SELL layout (BHIP_PREC_FP64_BLOCKED)

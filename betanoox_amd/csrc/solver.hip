@@ -764,15 +764,14 @@ __global__ void k_sell_pad(int n_nodes, int n_slices, const int *sptr, CT *scol)
 // neighbour list; each step takes the smallest offset (m - n) still at the
 // head of any lane's list as the next slot, and the lanes holding it advance.
 // Without uoff (first pass) the slice width 64 |union| goes to width[s];
-// with it, the offsets go to uoff[sptr[s] / 64 + u], the slot of every
-// node pair to pair_slot and, for every slot and lane, the rank of its pair
-// in the node's list (255: none) to slot_rank.  A union above kMaxUnionSlots
+// with it, the offsets go to uoff[sptr[s] / 64 + u] and, for every slot and
+// lane, the rank of its pair in the node's list (255: none) to slot_rank.  A union above kMaxUnionSlots
 // stops early and reports kMaxUnionSlots + 1 slots (the caller then keeps
 // rank slots), so ranks stay below 255.
 constexpr int kMaxUnionSlots = 255;
 __global__ void __launch_bounds__(256) k_union_slots(int n_nodes, int n_slices, const int *nbr_ptr, const int *nbr,
                                                      const int *sptr, int *width, int *uoff,
-                                                     unsigned char *pair_slot, unsigned char *slot_rank) {
+                                                     unsigned char *slot_rank) {
     const int lane = threadIdx.x & 63;
     const long long s = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (s >= n_slices) return;  // whole waves leave together
@@ -798,7 +797,6 @@ __global__ void __launch_bounds__(256) k_union_slots(int n_nodes, int n_slices, 
         }
         if (uoff) {
             if (lane == 0) uoff[ub + u] = mn;
-            if (head == mn) pair_slot[p] = (unsigned char)u;
             slot_rank[64 * (ub + u) + lane] = head == mn ? (unsigned char)(p - p0) : (unsigned char)255;
         }
         if (head == mn) {
@@ -903,19 +901,43 @@ __global__ void k_csr_to_runs(int n_nodes, const int *nbr_ptr, const int *nbr, c
 }
 
 // FP64 CSR -> SELL-64 B x B blocks of VT (SoA); restrained entries zero,
-// restrained diagonal 1.  Pair p = (node n, k-th neighbour m) goes to slot k
-// of its slice, or to slot pair_slot[p] in the union layout.  sval64, when
-// given, receives the same blocks in FP64 in the same pass.
+// restrained diagonal 1.  Slot order: a workgroup takes four slots of one
+// 64-node slice (one per wave, lanes on consecutive nodes), so every store of
+// a wave fills 64 consecutive entries; the slot groups of one slice run on
+// one XCD (workgroup b goes to XCD b mod 8).  The pair in slot j of node n is
+// its j-th neighbour in the rank layout, or the neighbour of rank
+// slot_rank[dst] in the union layout (255: a zero block, left as the layout
+// set it).  sval64, when given, receives the same blocks in FP64 in the same
+// pass.
 template <int B, class CT, class VT>
-__global__ void k_csr_to_sell(long long n_pairs, const int *pair_row, const int *nbr, const int *nbr_ptr,
-                              const int *nbr_off, const int *rowptr, const unsigned char *rmask,
-                              const double *val, const int *sptr, long long nsell, CT *scol, VT *sval,
-                              double *sval64, const unsigned char *pair_slot) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pairs) return;
-    int n = pair_row[p], m = nbr[p];
-    int k = pair_slot ? (int)pair_slot[p] : (int)(p - nbr_ptr[n]);
-    long long dst = (long long)sptr[n / 64] + 64LL * k + (n % 64);
+__global__ void __launch_bounds__(256) k_csr_to_sell(int n_nodes, const int *nbr, const int *nbr_ptr,
+                                                     const int *nbr_off, const int *rowptr,
+                                                     const unsigned char *rmask, const double *val,
+                                                     const int *sptr, long long nsell, CT *scol, VT *sval,
+                                                     double *sval64, const unsigned char *slot_rank, int groups,
+                                                     long long slice0, long long slice_end) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
+    const unsigned q = kk / (unsigned)groups;
+    const int j = 4 * (int)(kk - q * (unsigned)groups) + wave;
+    const long long sl = slice0 + 8LL * q + xcd;
+    if (sl >= slice_end) return;
+    if (j >= ((sptr[sl + 1] - sptr[sl]) >> 6)) return;
+    const int n = (int)(sl * 64 + lane);
+    if (n >= n_nodes) return;
+    const long long dst = (long long)sptr[sl] + 64LL * j + lane;
+    const int pb = nbr_ptr[n];
+    long long p;
+    if (slot_rank) {
+        const int rk = slot_rank[dst];
+        if (rk == 255) return;
+        p = pb + rk;
+    } else {
+        if (pb + j >= nbr_ptr[n + 1]) return;
+        p = pb + j;
+    }
+    const int m = nbr[p];
     if constexpr (sizeof(CT) == 2) scol[dst] = (CT)(m - n);
     else scol[dst] = m;
     unsigned rn = rmask[n], rm = rmask[m];
@@ -1249,7 +1271,7 @@ int union_widths(bhip_ctx *ctx, std::vector<int> &hw, bool *ok) {
     BHIP_TRY(ctx, width.resize(ns + 1));
     BHIP_TRY(ctx, hipMemsetAsync(width.p, 0, sizeof(int) * (ns + 1), s));
     k_union_slots<<<grid_for((long long)ns * 64, 256), 256, 0, s>>>(ctx->n_nodes, ns, ctx->nbr_ptr.p, ctx->nbr.p,
-                                                                    nullptr, width.p, nullptr, nullptr, nullptr);
+                                                                    nullptr, width.p, nullptr, nullptr);
     BHIP_TRY(ctx, hipGetLastError());
     hw.assign(ns + 1, 0);
     BHIP_TRY(ctx, hipMemcpyAsync(hw.data(), width.p, sizeof(int) * (ns + 1), hipMemcpyDeviceToHost, s));
@@ -1334,11 +1356,10 @@ int build_sell(bhip_ctx *ctx) {
     BHIP_TRY(ctx, hipGetLastError());
     if (uni) {
         BHIP_TRY(ctx, ctx->sell_uoff.resize(tot / 64 > 0 ? tot / 64 : 1));
-        BHIP_TRY(ctx, ctx->pair_slot.resize(ctx->n_pairs > 0 ? ctx->n_pairs : 1));
         BHIP_TRY(ctx, ctx->slot_rank.resize(tot > 0 ? tot : 1));
         k_union_slots<<<grid_for((long long)ns * 64, 256), 256, 0, s>>>(
             ctx->n_nodes, ns, ctx->nbr_ptr.p, ctx->nbr.p, ctx->sell_ptr.p, nullptr, ctx->sell_uoff.p,
-            ctx->pair_slot.p, ctx->slot_rank.p);
+            ctx->slot_rank.p);
         BHIP_TRY(ctx, hipGetLastError());
     }
     // (the B = 1 rows of the table are measured on column runs, the form hex8 diffusion takes)
@@ -1377,23 +1398,31 @@ void enqueue_to_sell(bhip_ctx *ctx, VT *sval, double *sval64 = nullptr) {
             ctx->n_sell, ctx->sell_col16.p, sval, sval64);
         return;
     }
-    const int grid = grid_for(ctx->n_pairs, B);
     ctx->sell_cols_ok = true;
-#define BHIP_TO_SELL(BB, CT, COL)                                                                                   \
-    k_csr_to_sell<BB, CT, VT><<<grid, B, 0, ctx->stream>>>(ctx->n_pairs, ctx->pair_row.p, ctx->nbr.p, ctx->nbr_ptr.p, \
-                                                           ctx->nbr_off.p, ctx->rowptr.p, ctx->node_rmask.p,          \
-                                                           ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, COL, sval, \
-                                                           sval64, ctx->sell_union ? ctx->pair_slot.p : nullptr)
-    if (ctx->sell_narrow) {
-        if (nb == 3) BHIP_TO_SELL(3, short, ctx->sell_col16.p);
-        else if (nb == 2) BHIP_TO_SELL(2, short, ctx->sell_col16.p);
-        else BHIP_TO_SELL(1, short, ctx->sell_col16.p);
-    } else {
-        if (nb == 3) BHIP_TO_SELL(3, int, ctx->sell_col.p);
-        else if (nb == 2) BHIP_TO_SELL(2, int, ctx->sell_col.p);
-        else BHIP_TO_SELL(1, int, ctx->sell_col.p);
-    }
+    if (ctx->n_slices <= 0 || ctx->sell_wmax <= 0) return;
+    const unsigned char *srank = ctx->sell_union ? ctx->slot_rank.p : nullptr;
+    const int groups = (ctx->sell_wmax + 3) / 4;  // four slots of the widest slice per workgroup
+    // launches of at most 2^22 workgroups, each over a multiple of 8 slices but the last
+    const long long per_launch = std::max(8LL, ((1LL << 22) / groups) & ~7LL);
+    for (long long s0 = 0; s0 < ctx->n_slices; s0 += per_launch) {
+        const long long cnt = std::min(per_launch, (long long)ctx->n_slices - s0);
+        const unsigned blocks = (unsigned)((cnt + 7) / 8 * 8 * groups);
+#define BHIP_TO_SELL(BB, CT, COL)                                                                                \
+    k_csr_to_sell<BB, CT, VT><<<blocks, B, 0, ctx->stream>>>(ctx->n_nodes, ctx->nbr.p, ctx->nbr_ptr.p,          \
+                                                             ctx->nbr_off.p, ctx->rowptr.p, ctx->node_rmask.p, \
+                                                             ctx->val.p, ctx->sell_ptr.p, ctx->n_sell, COL, sval, \
+                                                             sval64, srank, groups, s0, s0 + cnt)
+        if (ctx->sell_narrow) {
+            if (nb == 3) BHIP_TO_SELL(3, short, ctx->sell_col16.p);
+            else if (nb == 2) BHIP_TO_SELL(2, short, ctx->sell_col16.p);
+            else BHIP_TO_SELL(1, short, ctx->sell_col16.p);
+        } else {
+            if (nb == 3) BHIP_TO_SELL(3, int, ctx->sell_col.p);
+            else if (nb == 2) BHIP_TO_SELL(2, int, ctx->sell_col.p);
+            else BHIP_TO_SELL(1, int, ctx->sell_col.p);
+        }
 #undef BHIP_TO_SELL
+    }
 }
 
 // FP64 block array sized and padded for the current SELL layout.
